@@ -1,0 +1,225 @@
+"""grace_amd -- MI355X GRMHD RHS + Kastaun con2prim (Python view of the C ABI).
+
+The compute path is libgrmhd_mi355x.so (hand-written gfx950 HIP kernels behind
+the C ABI declared in include/grmhd_abi.h).  This module only binds that ABI
+with ctypes and moves torch device tensors' pointers into it; it never falls
+back to a CPU implementation: if the library is missing, every call raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgrmhd_mi355x.so")
+
+# Variable indices (include/grmhd_abi.h)
+DENS, SX, SY, SZ, TAU, YESTAR, ENTROPYSTAR = range(7)
+GXX, GXY, GXZ, GYY, GYZ, GZZ, ALP, BETAX, BETAY, BETAZ = range(7, 17)
+KXX, KXY, KXZ, KYY, KYZ, KZZ = range(17, 23)
+NSTATE = 23
+NHRSC = 7
+(RHO, ZVECX, ZVECY, ZVECZ, BX, BY, BZ, YE, TEMP, ENTROPY, EPS, PRESS, BDIV, C2P_DENS_ERR,
+ C2P_ERR) = range(15)
+NAUX = 15
+# c2p_err bits (c2p.hh:95-125)
+C2P_BITS = ["RESET_DENS", "RESET_TAU", "RESET_STILDE", "RESET_ENTROPY", "RESET_YE",
+            "SIG_RHO_TOO_LOW", "SIG_RHO_TOO_HIGH", "SIG_EPS_TOO_LOW", "SIG_EPS_TOO_HIGH",
+            "SIG_YE_TOO_LOW", "SIG_YE_TOO_HIGH", "SIG_ENT_TOO_LOW", "SIG_ENT_TOO_HIGH",
+            "SIG_TEMP_TOO_LOW", "SIG_TEMP_TOO_HIGH", "SIG_PRESS_TOO_LOW", "SIG_PRESS_TOO_HIGH",
+            "SIG_VEL_TOO_HIGH", "SIG_SIGMA_TOO_HIGH", "ENT_BACKUP_USED", "ATMO_RESET", "T_FLOORED"]
+
+EOS_IDEAL_GAS, EOS_HYBRID_PWPOLY = 0, 1
+RECON_WENOZ, RECON_MC2, RECON_MINMOD, RECON_DONOR = 0, 1, 2, 3
+RIEMANN_HLLE, RIEMANN_LLF = 0, 1
+EMF_UCT, EMF_GS = 0, 1
+THERMO_TEMP, THERMO_PRESS = 0, 1
+BC_OUTFLOW, BC_PERIODIC = 0, 1
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("eos_kind", C.c_int32), ("gamma", C.c_double), ("eps_max", C.c_double),
+        ("entropy_min", C.c_double), ("gamma_th", C.c_double), ("pwp_n", C.c_int32),
+        ("pwp_kappa0", C.c_double), ("pwp_gamma", C.c_double * 8), ("pwp_rho", C.c_double * 8),
+        ("rho_fl", C.c_double), ("temp_fl", C.c_double), ("ye_fl", C.c_double), ("atmo_tol", C.c_double),
+        ("c2p_tol", C.c_double), ("max_lorentz", C.c_double), ("max_sigma", C.c_double),
+        ("beta_fallback", C.c_double), ("use_entropy_backup", C.c_int32), ("bh_alp_thresh", C.c_double),
+        ("excise_by_radius", C.c_int32), ("excision_radius", C.c_double), ("excision_lapse", C.c_double),
+        ("recon", C.c_int32), ("riemann", C.c_int32), ("emf", C.c_int32), ("thermo_recon", C.c_int32),
+        ("fd_order", C.c_int32), ("rusanov_c_limit", C.c_double), ("flat_metric", C.c_int32),
+    ]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("ngz", C.c_int32), ("nq", C.c_int64)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("nb", C.c_int32 * 3), ("q0", C.c_int64), ("bc", C.c_int32 * 6), ("idx", C.c_double * 3)]
+
+
+class C2PStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("n_cells", "n_fatal", "n_atmo_reset", "n_ent_backup", "n_guess_used",
+                                          "n_root_evals", "n_bracket_evals", "max_root_evals")]
+
+
+_lib = None
+
+ABI_SYMBOLS = [
+    "grmhd_default_params", "grmhd_ctx_create", "grmhd_ctx_destroy", "grmhd_last_error",
+    "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
+    "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
+    "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
+    "grmhd_setup_blast", "grmhd_setup_shocktube", "grmhd_kerr_schild", "grmhd_setup_c2p_states",
+]
+
+
+def lib():
+    """Load libgrmhd_mi355x.so; raise if it is not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m grace_amd.build` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    d = C.c_double
+    i32, i64 = C.c_int32, C.c_int64
+    L.grmhd_default_params.argtypes = [C.POINTER(Params)]
+    L.grmhd_ctx_create.argtypes = [C.POINTER(Params), C.POINTER(Grid), C.POINTER(Instance), i32, C.POINTER(P)]
+    L.grmhd_ctx_destroy.argtypes = [P]
+    L.grmhd_last_error.restype = C.c_char_p
+    L.grmhd_advance_substep.argtypes = [P, P, d, P, P, P * 3, P * 3, P, P]
+    L.grmhd_fill_ghosts.argtypes = [P, P, P * 3, P]
+    L.grmhd_compute_auxiliaries.argtypes = [P, P, P * 3, P, P]
+    L.grmhd_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P]
+    L.grmhd_prims_to_conservs_batch.argtypes = [P, i64, P, P, P, P]
+    L.grmhd_c2p_stats.argtypes = [P, C.POINTER(C2PStats), P]
+    L.grmhd_compute_dt.argtypes = [P, P, P, d, P, P]
+    L.grmhd_rk3_step.argtypes = [P, P, P, P * 3, P, P]
+    L.grmhd_rk3_step_graph.argtypes = [P, P, P, P * 3, P, P]
+    L.grmhd_set_timing.argtypes = [P, i32]
+    L.grmhd_kernel_time_ms.argtypes = [P, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
+    L.grmhd_setup_blast.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, P, P, d, d, d, d, P, d,
+                                    C.c_uint64, P, P, P, P, P]
+    L.grmhd_setup_shocktube.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, d, d, P, P, P, P, P, P, P]
+    L.grmhd_kerr_schild.argtypes = [d, d, d, d, P, P, P]
+    L.grmhd_setup_c2p_states.argtypes = [i64, C.c_uint64, d, d, P, P, P, P, P, P, P, P, P]
+    for name in ABI_SYMBOLS:
+        getattr(L, name)  # raises AttributeError if an exported symbol is missing
+    _lib = L
+    return L
+
+
+def default_params(**kw):
+    p = Params()
+    lib().grmhd_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        if k in ("pwp_gamma", "pwp_rho"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def check(status, what="call"):
+    if status != 0:
+        msg = lib().grmhd_last_error().decode(errors="replace")
+        raise RuntimeError(f"grmhd {what} failed with status {status}: {msg}")
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return C.c_void_p(t.data_ptr())
+    if hasattr(t, "ctypes"):
+        return C.c_void_p(t.ctypes.data)
+    return C.c_void_p(t)
+
+
+def ptr3(a, b, c):
+    return (C.c_void_p * 3)(_ptr(a), _ptr(b), _ptr(c))
+
+
+def make_instances(specs):
+    """specs: list of dict(nb=(bx,by,bz), q0=int, bc=(6 ints), idx=(3 floats))."""
+    arr = (Instance * max(1, len(specs)))()
+    for i, s in enumerate(specs):
+        for d in range(3):
+            arr[i].nb[d] = int(s["nb"][d])
+            arr[i].idx[d] = float(s["idx"][d])
+        arr[i].q0 = int(s["q0"])
+        for d in range(6):
+            arr[i].bc[d] = int(s["bc"][d])
+    return arr
+
+
+class Context:
+    """Owns a grmhd_ctx; methods take torch device tensors."""
+
+    def __init__(self, params, grid, instances):
+        self.L = lib()
+        self.params = params
+        self.grid = grid
+        self.instances = instances
+        h = C.c_void_p()
+        check(self.L.grmhd_ctx_create(C.byref(params), C.byref(grid), instances, len(instances), C.byref(h)),
+              "ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.grmhd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- entry points ---
+    def advance_substep(self, dt_dev, dtfact, new_state, old_state, newB, oldB, aux, stream=None):
+        check(self.L.grmhd_advance_substep(self.h, _ptr(dt_dev), dtfact, _ptr(new_state), _ptr(old_state),
+                                           ptr3(*newB), ptr3(*oldB), _ptr(aux), _ptr(stream)), "advance_substep")
+
+    def fill_ghosts(self, state, B, stream=None):
+        check(self.L.grmhd_fill_ghosts(self.h, _ptr(state), ptr3(*B), _ptr(stream)), "fill_ghosts")
+
+    def compute_auxiliaries(self, state, B, aux, stream=None):
+        check(self.L.grmhd_compute_auxiliaries(self.h, _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)),
+              "compute_auxiliaries")
+
+    def c2p_batch(self, n, cons, met, prims, err, floored, dry_run=0, r=None, guess=None, stream=None):
+        check(self.L.grmhd_c2p_batch(self.h, n, _ptr(cons), _ptr(met), _ptr(prims), _ptr(err), _ptr(floored),
+                                     int(dry_run), _ptr(r), _ptr(guess), _ptr(stream)), "c2p_batch")
+
+    def p2c_batch(self, n, prims, met, cons, stream=None):
+        check(self.L.grmhd_prims_to_conservs_batch(self.h, n, _ptr(prims), _ptr(met), _ptr(cons), _ptr(stream)),
+              "p2c_batch")
+
+    def c2p_stats(self, stream=None):
+        s = C2PStats()
+        check(self.L.grmhd_c2p_stats(self.h, C.byref(s), _ptr(stream)), "c2p_stats")
+        return {k: getattr(s, k) for k, _ in C2PStats._fields_}
+
+    def compute_dt(self, state, aux, cfl, dt_dev, stream=None):
+        check(self.L.grmhd_compute_dt(self.h, _ptr(state), _ptr(aux), float(cfl), _ptr(dt_dev), _ptr(stream)),
+              "compute_dt")
+
+    def rk3_step(self, dt_dev, state, B, aux, stream=None, graph=False):
+        fn = self.L.grmhd_rk3_step_graph if graph else self.L.grmhd_rk3_step
+        check(fn(self.h, _ptr(dt_dev), _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)), "rk3_step")
+
+    def set_timing(self, on):
+        check(self.L.grmhd_set_timing(self.h, int(on)), "set_timing")
+
+    def kernel_time_ms(self, fam):
+        ms = C.c_double()
+        n = C.c_int64()
+        check(self.L.grmhd_kernel_time_ms(self.h, fam.encode(), C.byref(ms), C.byref(n)), "kernel_time_ms")
+        return ms.value, n.value

@@ -1,0 +1,63 @@
+"""Build libgrmhd_mi355x.so (gfx950) in-tree with hipcc.
+
+    python -m grace_amd.build            # build the HIP library
+    python -m grace_amd.build --oracle   # also build the CPU oracle (test infrastructure)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgrmhd_mi355x.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("GRMHD_OFFLOAD_ARCH", "gfx950")
+
+# -packed-fp32-ops: hipcc must not generate v_pk_{add,mul,fma}_f32 (some of
+# their forms have given silently wrong results on gfx950); the host pass warns
+# that it does not know the feature, which is harmless.
+HIP_FLAGS = [
+    "-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+SOURCES = ["grmhd.hip", "mlp.hip", "setup.cpp"]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if out:
+        print(out, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_lib():
+    srcs = []
+    for s in SOURCES:
+        p = os.path.join(CSRC, s)
+        if os.path.exists(p):
+            srcs += ["-x", "hip", p]
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j4"])
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    build_lib()
+    if "--oracle" in argv:
+        build_oracle()
+
+
+if __name__ == "__main__":
+    main()

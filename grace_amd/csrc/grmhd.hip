@@ -1,0 +1,1319 @@
+// grmhd.hip -- gfx950 kernels and C ABI for the Cowling GRMHD RHS + con2prim.
+//
+// Kernel families (all FP64):
+//   k_flux<D>      one thread per face of direction D: 10-variable
+//                  reconstruction, face metric, EOS, HLLE/LLF fluxes, vbar/Eface
+//                  (reference: evolve.cpp:813-1068, grmhd.hh:551-970)
+//   k_emf_uct      one thread per edge triple: UCT edge EMFs (evolve.cpp:1130-1340)
+//   k_emf_gs       Gardiner-Stone edge EMFs (grmhd_helpers.hh:492-627)
+//   k_update       fused RK linear combination + geometric sources + flux
+//                  divergence + CT face update (tstep_utils.hh:140-220,
+//                  evolve.cpp:1342-1479)
+//   k_ghost        block ghost-zone gather (boundary_conditions.cpp:55-81)
+//   k_c2p_cells    c2p over cells + conserved rewrite + BDIV + E_center
+//                  (auxiliaries.cpp:81-180, grmhd.hh:306-434)
+//   k_c2p_batch    c2p over a flat batch (c2p.cpp:188-414)
+//   k_dt_*         CFL reduction (find_stable_timestep.cpp:72-126)
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <string>
+#include <vector>
+#include <map>
+#include "phys.hh"
+#include "../../include/grmhd_abi.h"
+
+using namespace gm;
+
+static thread_local char g_errbuf[512] = "";
+static void set_err(const char* fmt, ...) {
+    va_list ap; va_start(ap, fmt);
+    vsnprintf(g_errbuf, sizeof(g_errbuf), fmt, ap);
+    va_end(ap);
+}
+extern "C" const char* grmhd_last_error(void) { return g_errbuf; }
+
+#define HIP_TRY(x)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            set_err("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return GRMHD_ERR_HIP;                                                   \
+        }                                                                           \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device-side geometry of the block grid
+// ---------------------------------------------------------------------------
+struct DevGrid {
+    int n[3];          // interior cells per block
+    int ng;            // ghost layers
+    int N[3];          // padded extents
+    int64_t ncell;     // cells per block (padded)
+    int64_t nface[3];  // faces per block per direction
+    int64_t nedge;     // edges per block per component
+    int64_t nq;
+};
+struct DevInst { int nb[3]; int64_t q0; int bc[6]; double idx[3]; };
+struct DevBlk { int inst; int b[3]; };
+
+struct Opts {
+    Atmo atmo; C2PPar cp; ExcPar ex;
+    int riemann, emf, thermo, fd_order;
+    double c_limit;
+};
+
+GHD int64_t cidx(const DevGrid& G, int i, int j, int k) { return i + (int64_t)G.N[0] * (j + (int64_t)G.N[1] * k); }
+template <int D>
+GHD int64_t fidx(const DevGrid& G, int i, int j, int k) {
+    int ei = G.N[0] + (D == 0), ej = G.N[1] + (D == 1);
+    return i + (int64_t)ei * (j + (int64_t)ej * k);
+}
+GHD int64_t fidx_d(const DevGrid& G, int d, int i, int j, int k) {
+    int ei = G.N[0] + (d == 0), ej = G.N[1] + (d == 1);
+    return i + (int64_t)ei * (j + (int64_t)ej * k);
+}
+GHD int64_t eidx(const DevGrid& G, int i, int j, int k) {
+    return i + (int64_t)(G.N[0] + 1) * (j + (int64_t)(G.N[1] + 1) * k);
+}
+
+// Read the Cowling metric of a cell (state vars GXX_..BETAZ_).
+GHD void load_metric(Met& m, const double* __restrict__ ms, int64_t ncell, int64_t c) {
+    for (int a = 0; a < 6; ++a) m.g[a] = ms[(GXX_ + a) * ncell + c];
+    m.alp = ms[ALP_ * ncell + c];
+    for (int a = 0; a < 3; ++a) m.beta[a] = ms[(BETAX_ + a) * ncell + c];
+    m.init();
+}
+
+// ---------------------------------------------------------------------------
+// Face flux kernel
+// ---------------------------------------------------------------------------
+template <typename EOS, int REC, bool FLAT, int D>
+__global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
+                                              const double* __restrict__ mstate, int mnv,
+                                              const double* __restrict__ Bd, double* __restrict__ flux,
+                                              double* __restrict__ vbar) {
+    // face range: normal [ng, n+ng], transverse [0, N)
+    int R[3] = {G.N[0], G.N[1], G.N[2]};
+    R[D] = G.n[D] + 1;
+    int64_t per = (int64_t)R[0] * R[1] * R[2];
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * G.nq) return;
+    int64_t q = t / per;
+    int64_t r = t - q * per;
+    int i = (int)(r % R[0]);
+    int j = (int)((r / R[0]) % R[1]);
+    int k = (int)(r / ((int64_t)R[0] * R[1]));
+    if (D == 0) i += G.ng; else if (D == 1) j += G.ng; else k += G.ng;
+    const int64_t nc = G.ncell;
+    const double* A = aux + q * NAUX * nc;
+    const int64_t c0 = cidx(G, i, j, k);
+    const int64_t st = D == 0 ? 1 : (D == 1 ? G.N[0] : (int64_t)G.N[0] * G.N[1]);
+
+    Met mf;
+    bool excise_override = false;
+    if constexpr (FLAT) {
+        mf.flat();
+    } else {
+        const double* M = mstate + q * (int64_t)mnv * nc;
+        double v[10];
+        const int comps[10] = {GXX_, GXY_, GXZ_, GYY_, GYZ_, GZZ_, BETAX_, BETAY_, BETAZ_, ALP_};
+#pragma unroll
+        for (int a = 0; a < 10; ++a) {
+            const double* P = M + comps[a] * nc + c0;
+            double si = P[-st] + P[0];
+            double so = P[-2 * st] + P[st];
+            v[a] = 0.5625 * si + (-0.0625) * so;
+        }
+        for (int a = 0; a < 6; ++a) mf.g[a] = v[a];
+        for (int a = 0; a < 3; ++a) mf.beta[a] = v[6 + a];
+        mf.alp = v[9];
+        mf.init();
+        double alpL = M[ALP_ * nc + c0 - st], alpR = M[ALP_ * nc + c0];
+        excise_override = (alpL <= o.ex.alp_ex) || (alpR <= o.ex.alp_ex);
+    }
+    // reconstruction of RHO Z(3) YE THERMO ENT B(3)
+    const int th_aux = o.thermo ? PRESS_ : TEMP_;
+    const int ridx[10] = {RHO_, ZVECX_, ZVECY_, ZVECZ_, YE_, th_aux, ENTROPY_, BX_, BY_, BZ_};
+    const int rloc[10] = {RHOL, ZXL, ZYL, ZZL, YEL, o.thermo ? PRESSL : TEMPL, ENTL, BXL, BYL, BZL};
+    double pL[NPRIM], pR[NPRIM];
+#pragma unroll
+    for (int v = 0; v < 10; ++v) {
+        const double* P = A + ridx[v] * nc + c0;
+        double uL, uR;
+        if (excise_override) { uL = P[-st]; uR = P[0]; }
+        else recon<REC>(P[-3 * st], P[-2 * st], P[-st], P[0], P[st], P[2 * st], uL, uR);
+        pL[rloc[v]] = uL; pR[rloc[v]] = uR;
+    }
+    double Bn = Bd[q * G.nface[D] + fidx<D>(G, i, j, k)] / mf.sqrtg;
+    pL[BXL + D] = Bn; pR[BXL + D] = Bn;
+    unsigned ee = 0;
+    if (o.thermo) {
+        double h, s;
+        pL[EPSL] = eos.eps_from_press(h, pL[CS2L], pL[TEMPL], s, pL[PRESSL], pL[RHOL], ee);
+        pR[EPSL] = eos.eps_from_press(h, pR[CS2L], pR[TEMPL], s, pR[PRESSL], pR[RHOL], ee);
+    } else {
+        pL[PRESSL] = eos.press_from_temp(pL[EPSL], pL[CS2L], pL[TEMPL], pL[RHOL], ee);
+        pR[PRESSL] = eos.press_from_temp(pR[EPSL], pR[CS2L], pR[TEMPL], pR[RHOL], ee);
+    }
+    // compute_mhd_fluxes (grmhd.hh:774-970)
+    const double *zl = &pL[ZXL], *zr = &pR[ZXL], *Bl = &pL[BXL], *Br = &pR[BXL];
+    double wl = lorentz(mf.g, zl), wr = lorentz(mf.g, zr);
+    double bl[4], br[4], b2l, b2r;
+    smallb(mf, Bl, zl, wl, bl, b2l);
+    smallb(mf, Br, zr, wr, br, b2r);
+    double vtl[3], vtr[3];
+    vtilde(mf, zl, wl, vtl);
+    vtilde(mf, zr, wr, vtr);
+    double cml, cpl, cmr, cpr, cmin, cmax;
+    cmcp(mf, pL[RHOL], pL[PRESSL], pL[EPSL], pL[CS2L], wl, b2l, vtl, D, cml, cpl);
+    cmcp(mf, pR[RHOL], pR[PRESSL], pR[EPSL], pR[CS2L], wr, b2r, vtr, D, cmr, cpr);
+    if (o.riemann == 1) {
+        cmax = fmax(fmax(fabs(cml), fabs(cmr)), fmax(fabs(cpl), fabs(cpr)));
+        if (fmin(pL[RHOL], pR[RHOL]) < o.c_limit) {
+            double gdd = D == 0 ? mf.gi[0] : (D == 1 ? mf.gi[3] : mf.gi[5]);
+            cmax = fmax(cmax, mf.alp * sqrt(gdd) + fabs(mf.beta[D]));
+        }
+        cmin = cmax;
+    } else {
+        cmin = -fmin(0., fmin(cml, cmr));
+        cmax = fmax(0., fmax(cpl, cpr));
+    }
+    if (cmin < 1e-12 && cmax < 1e-12) { cmin = 1; cmax = 1; }
+    const int64_t fo = fidx<D>(G, i, j, k);
+    const int64_t nf = G.nface[D];
+    double* V = vbar + q * 4 * nf + fo;
+    constexpr int J0 = D == 0 ? 1 : 0, J1 = D == 2 ? 1 : 2;
+    if (o.emf == 0) {  // UCT: HLL-averaged transverse vtilde + wave speeds, slots strided by nf
+        V[0] = hll(vtl[J0], vtr[J0], 0, 0, cmin, cmax);
+        V[nf] = hll(vtl[J1], vtr[J1], 0, 0, cmin, cmax);
+        V[2 * nf] = cmin; V[3 * nf] = cmax;
+    }
+    double Dl, taul, Sl[3], Esl, Dr, taur, Sr[3], Esr;
+    conserved(mf, pL[RHOL], pL[PRESSL], pL[EPSL], zl, pL[ENTL], wl, b2l, bl, Dl, taul, Sl, Esl);
+    conserved(mf, pR[RHOL], pR[PRESSL], pR[EPSL], zr, pR[ENTL], wr, b2r, br, Dr, taur, Sr, Esr);
+    // fluxes along D (grmhd_subexpressions.hh:97-156)
+    double bsh[3]; lower3(mf.g, mf.beta[0], mf.beta[1], mf.beta[2], bsh);
+    double ia2 = 1. / (mf.alp * mf.alp);
+    double Fl[7], Fr[7];
+    {
+        double rhoh = b2l + pL[EPSL] * pL[RHOL] + pL[PRESSL] + pL[RHOL];
+        double ps = 0.5 * b2l + pL[PRESSL];
+        double W2rh = wl * wl * rhoh;
+        double fD = vtl[D] * Dl;
+        Fl[0] = fD;
+        Fl[4] = W2rh * vtl[D] + ps * mf.beta[D] - mf.alp * mf.alp * bl[0] * bl[1 + D] - fD;
+        double blo[3]; lower3(mf.g, bl[1], bl[2], bl[3], blo);
+        double vlo[3]; lower3(mf.g, vtl[0], vtl[1], vtl[2], vlo);
+        double c = vtl[D] * W2rh * ia2;
+        for (int a = 0; a < 3; ++a)
+            Fl[1 + a] = mf.alp * (-bl[1 + D] * (blo[a] + bl[0] * bsh[a]) + (a == D ? ps : 0.0) + c * (vlo[a] + bsh[a]));
+        Fl[6] = vtl[D] * Esl;
+        Fl[5] = pL[YEL] * fD;
+    }
+    {
+        double rhoh = b2r + pR[EPSL] * pR[RHOL] + pR[PRESSL] + pR[RHOL];
+        double ps = 0.5 * b2r + pR[PRESSL];
+        double W2rh = wr * wr * rhoh;
+        double fD = vtr[D] * Dr;
+        Fr[0] = fD;
+        Fr[4] = W2rh * vtr[D] + ps * mf.beta[D] - mf.alp * mf.alp * br[0] * br[1 + D] - fD;
+        double blo[3]; lower3(mf.g, br[1], br[2], br[3], blo);
+        double vlo[3]; lower3(mf.g, vtr[0], vtr[1], vtr[2], vlo);
+        double c = vtr[D] * W2rh * ia2;
+        for (int a = 0; a < 3; ++a)
+            Fr[1 + a] = mf.alp * (-br[1 + D] * (blo[a] + br[0] * bsh[a]) + (a == D ? ps : 0.0) + c * (vlo[a] + bsh[a]));
+        Fr[6] = vtr[D] * Esr;
+        Fr[5] = pR[YEL] * fD;
+    }
+    double* F = flux + q * NHRSC * nf + fo;
+    const double sg = mf.sqrtg;
+    F[DENS_ * nf] = sg * hll(Fl[0], Fr[0], Dl, Dr, cmin, cmax);
+    F[ENTROPYSTAR_ * nf] = sg * hll(Fl[6], Fr[6], Esl, Esr, cmin, cmax);
+    F[TAU_ * nf] = sg * hll(Fl[4], Fr[4], taul, taur, cmin, cmax);
+    F[SX_ * nf] = sg * hll(Fl[1], Fr[1], Sl[0], Sr[0], cmin, cmax);
+    F[SY_ * nf] = sg * hll(Fl[2], Fr[2], Sl[1], Sr[1], cmin, cmax);
+    F[SZ_ * nf] = sg * hll(Fl[3], Fr[3], Sl[2], Sr[2], cmin, cmax);
+    F[YESTAR_ * nf] = sg * hll(Fl[5], Fr[5], pL[YEL] * Dl, pR[YEL] * Dr, cmin, cmax);
+    if (o.emf == 1) {  // GS face EMFs (grmhd.hh:937-969)
+        if (D == 0) {
+            double a = -Bl[0] * vtl[2] + Bl[2] * vtl[0], b = -Br[0] * vtr[2] + Br[2] * vtr[0];
+            V[0] = sg * hll(a, b, Bl[2], Br[2], cmin, cmax);
+            a = -Bl[0] * vtl[1] + Bl[1] * vtl[0]; b = -Br[0] * vtr[1] + Br[1] * vtr[0];
+            V[nf] = -sg * hll(a, b, Bl[1], Br[1], cmin, cmax);
+        } else if (D == 1) {
+            double a = -Bl[1] * vtl[2] + Bl[2] * vtl[1], b = -Br[1] * vtr[2] + Br[2] * vtr[1];
+            V[0] = -sg * hll(a, b, Bl[2], Br[2], cmin, cmax);
+            a = Bl[0] * vtl[1] - Bl[1] * vtl[0]; b = Br[0] * vtr[1] - Br[1] * vtr[0];
+            V[nf] = sg * hll(a, b, Bl[0], Br[0], cmin, cmax);
+        } else {
+            double a = Bl[1] * vtl[2] - Bl[2] * vtl[1], b = Br[1] * vtr[2] - Br[2] * vtr[1];
+            V[0] = sg * hll(a, b, Bl[1], Br[1], cmin, cmax);
+            a = Bl[0] * vtl[2] - Bl[2] * vtl[0]; b = Br[0] * vtr[2] - Br[2] * vtr[0];
+            V[nf] = -sg * hll(a, b, Bl[0], Br[0], cmin, cmax);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UCT edge EMFs.  Face arrays: B faces (1 var), vbar (4 vars, stride nf).
+// ---------------------------------------------------------------------------
+template <int REC>
+GHD void rec6(const double* P, int64_t s, double& uL, double& uR) {
+    recon<REC>(P[-3 * s], P[-2 * s], P[-s], P[0], P[s], P[2 * s], uL, uR);
+}
+
+template <int REC>
+__global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __restrict__ Bx,
+                                                 const double* __restrict__ By, const double* __restrict__ Bz,
+                                                 const double* __restrict__ vx, const double* __restrict__ vy,
+                                                 const double* __restrict__ vz, double* __restrict__ emf) {
+    int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    int64_t per = (int64_t)R0 * R1 * R2;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * G.nq) return;
+    int64_t q = t / per, r = t - q * per;
+    int i = G.ng + (int)(r % R0), j = G.ng + (int)((r / R0) % R1), k = G.ng + (int)(r / ((int64_t)R0 * R1));
+    const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2];
+    const double* bx = Bx + q * nfx; const double* by = By + q * nfy; const double* bz = Bz + q * nfz;
+    const double* Vx = vx + q * 4 * nfx; const double* Vy = vy + q * 4 * nfy; const double* Vz = vz + q * 4 * nfz;
+    // strides of the face arrays along each axis
+    const int64_t sx_x = 1, sx_y = G.N[0] + 1, sx_z = (int64_t)(G.N[0] + 1) * G.N[1];
+    const int64_t sy_x = 1, sy_y = G.N[0], sy_z = (int64_t)G.N[0] * (G.N[1] + 1);
+    const int64_t sz_x = 1, sz_y = G.N[0], sz_z = (int64_t)G.N[0] * G.N[1];
+    double* E = emf + q * 3 * G.nedge;
+    const int64_t eo = eidx(G, i, j, k);
+    if (i < G.n[0] + G.ng) {  // E^x at (i, j-1/2, k-1/2)
+        int64_t fy = fidx<1>(G, i, j, k), fz = fidx<2>(G, i, j, k);
+        double ByL, ByR, vzL, vzR, BzL, BzR, vyL, vyR;
+        rec6<REC>(by + fy, sy_z, ByL, ByR);
+        rec6<REC>(Vy + nfy + fy, sy_z, vzL, vzR);
+        rec6<REC>(bz + fz, sz_y, BzL, BzR);
+        rec6<REC>(Vz + nfz + fz, sz_y, vyL, vyR);
+        double cmin_y = fmax(Vy[2 * nfy + fy], Vy[2 * nfy + fy - sy_z]);
+        double cmax_y = fmax(Vy[3 * nfy + fy], Vy[3 * nfy + fy - sy_z]);
+        double cmin_z = fmax(Vz[2 * nfz + fz], Vz[2 * nfz + fz - sz_y]);
+        double cmax_z = fmax(Vz[3 * nfz + fz], Vz[3 * nfz + fz - sz_y]);
+        E[eo] = hll(vzL * ByL, vzR * ByR, ByL, ByR, cmin_z, cmax_z) - hll(vyL * BzL, vyR * BzR, BzL, BzR, cmin_y, cmax_y);
+    }
+    if (j < G.n[1] + G.ng) {  // E^y at (i-1/2, j, k-1/2)
+        int64_t fz = fidx<2>(G, i, j, k), fx = fidx<0>(G, i, j, k);
+        double BzL, BzR, vxL, vxR, BxL, BxR, vzL, vzR;
+        rec6<REC>(bz + fz, sz_x, BzL, BzR);
+        rec6<REC>(Vz + fz, sz_x, vxL, vxR);
+        rec6<REC>(bx + fx, sx_z, BxL, BxR);
+        rec6<REC>(Vx + nfx + fx, sx_z, vzL, vzR);
+        double cmin_z = fmax(Vz[2 * nfz + fz], Vz[2 * nfz + fz - sz_x]);
+        double cmax_z = fmax(Vz[3 * nfz + fz], Vz[3 * nfz + fz - sz_x]);
+        double cmin_x = fmax(Vx[2 * nfx + fx], Vx[2 * nfx + fx - sx_z]);
+        double cmax_x = fmax(Vx[3 * nfx + fx], Vx[3 * nfx + fx - sx_z]);
+        E[G.nedge + eo] = hll(vxL * BzL, vxR * BzR, BzL, BzR, cmin_x, cmax_x) - hll(vzL * BxL, vzR * BxR, BxL, BxR, cmin_z, cmax_z);
+    }
+    if (k < G.n[2] + G.ng) {  // E^z at (i-1/2, j-1/2, k)
+        int64_t fx = fidx<0>(G, i, j, k), fy = fidx<1>(G, i, j, k);
+        double BxL, BxR, vyL, vyR, ByL, ByR, vxL, vxR;
+        rec6<REC>(bx + fx, sx_y, BxL, BxR);
+        rec6<REC>(Vx + fx, sx_y, vyL, vyR);
+        rec6<REC>(by + fy, sy_x, ByL, ByR);
+        rec6<REC>(Vy + fy, sy_x, vxL, vxR);
+        double cmin_x = fmax(Vx[2 * nfx + fx], Vx[2 * nfx + fx - sx_y]);
+        double cmax_x = fmax(Vx[3 * nfx + fx], Vx[3 * nfx + fx - sx_y]);
+        double cmin_y = fmax(Vy[2 * nfy + fy], Vy[2 * nfy + fy - sy_x]);
+        double cmax_y = fmax(Vy[3 * nfy + fy], Vy[3 * nfy + fy - sy_x]);
+        E[2 * G.nedge + eo] = hll(vyL * BxL, vyR * BxR, BxL, BxR, cmin_y, cmax_y) - hll(vxL * ByL, vxR * ByR, ByL, ByR, cmin_x, cmax_x);
+    }
+}
+
+// Gardiner-Stone edge EMF (grmhd_helpers.hh:492-627).  Eface slots 0/1 of each
+// face direction, Ecenter (3 comps, cell arrays), DENS fluxes.
+__global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restrict__ ef0, const double* __restrict__ ef1,
+                                                const double* __restrict__ ef2, const double* __restrict__ ec,
+                                                const double* __restrict__ f0, const double* __restrict__ f1,
+                                                const double* __restrict__ f2, double* __restrict__ emf) {
+    int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    int64_t per = (int64_t)R0 * R1 * R2;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * G.nq) return;
+    int64_t q = t / per, r = t - q * per;
+    int i = G.ng + (int)(r % R0), j = G.ng + (int)((r / R0) % R1), k = G.ng + (int)(r / ((int64_t)R0 * R1));
+    const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2], nc = G.ncell;
+    auto EF = [&](int d, int c, int ii, int jj, int kk) {
+        const double* base = d == 0 ? ef0 : (d == 1 ? ef1 : ef2);
+        int64_t nf = G.nface[d];
+        return base[q * 4 * nf + c * nf + fidx_d(G, d, ii, jj, kk)];
+    };
+    auto EC = [&](int c, int ii, int jj, int kk) { return ec[q * 3 * nc + c * nc + cidx(G, ii, jj, kk)]; };
+    auto FD = [&](int d, int ii, int jj, int kk) {
+        const double* base = d == 0 ? f0 : (d == 1 ? f1 : f2);
+        int64_t nf = G.nface[d];
+        return base[q * NHRSC * nf + DENS_ * nf + fidx_d(G, d, ii, jj, kk)];
+    };
+    (void)nfx; (void)nfy; (void)nfz;
+    double* E = emf + q * 3 * G.nedge;
+    const int64_t eo = eidx(G, i, j, k);
+    if (i < G.n[0] + G.ng) {
+        double Exzf = EF(2, 0, i, j, k), Exzfm = EF(2, 0, i, j - 1, k), Exyf = EF(1, 0, i, j, k), Exyfm = EF(1, 0, i, j, k - 1);
+        double NE = EC(0, i, j, k), NW = EC(0, i, j - 1, k), SE = EC(0, i, j, k - 1), SW = EC(0, i, j - 1, k - 1);
+        double Eavg = 0.25 * (Exzf + Exzfm + Exyf + Exyfm);
+        double Sy = copysign(1., FD(1, i, j, k)), Sym = copysign(1., FD(1, i, j, k - 1));
+        double dN = (1. - Sy) * (NE - Exzf) + (1. + Sy) * (NW - Exzfm);
+        double dS = (1. - Sym) * (Exzf - SE) + (1. + Sym) * (Exzfm - SW);
+        double Sz = copysign(1., FD(2, i, j, k)), Szm = copysign(1., FD(2, i, j - 1, k));
+        double dE = (1. - Sz) * (NE - Exyf) + (1. + Sz) * (SE - Exyfm);
+        double dW = (1. - Szm) * (Exyf - NW) + (1. + Szm) * (Exyfm - SW);
+        E[eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
+    }
+    if (j < G.n[1] + G.ng) {
+        double Eyzf = EF(2, 1, i, j, k), Eyzfm = EF(2, 1, i - 1, j, k), Eyxf = EF(0, 0, i, j, k), Eyxfm = EF(0, 0, i, j, k - 1);
+        double NE = EC(1, i, j, k), NW = EC(1, i - 1, j, k), SE = EC(1, i, j, k - 1), SW = EC(1, i - 1, j, k - 1);
+        double Eavg = 0.25 * (Eyzf + Eyzfm + Eyxf + Eyxfm);
+        double Sx = copysign(1., FD(0, i, j, k)), Sxm = copysign(1., FD(0, i, j, k - 1));
+        double dN = (1. - Sx) * (NE - Eyzf) + (1. + Sx) * (NW - Eyzfm);
+        double dS = (1. - Sxm) * (Eyzf - SE) + (1. + Sxm) * (Eyzfm - SW);
+        double Sz = copysign(1., FD(2, i, j, k)), Szm = copysign(1., FD(2, i - 1, j, k));
+        double dE = (1. - Sz) * (NE - Eyxf) + (1. + Sz) * (SE - Eyxfm);
+        double dW = (1. - Szm) * (Eyxf - NW) + (1. + Szm) * (Eyxfm - SW);
+        E[G.nedge + eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
+    }
+    if (k < G.n[2] + G.ng) {
+        double Ezyf = EF(1, 1, i, j, k), Ezyfm = EF(1, 1, i - 1, j, k), Ezxf = EF(0, 1, i, j, k), Ezxfm = EF(0, 1, i, j - 1, k);
+        double NE = EC(2, i, j, k), NW = EC(2, i - 1, j, k), SE = EC(2, i, j - 1, k), SW = EC(2, i - 1, j - 1, k);
+        double Eavg = 0.25 * (Ezyf + Ezyfm + Ezxf + Ezxfm);
+        double Sx = copysign(1., FD(0, i, j, k)), Sxm = copysign(1., FD(0, i, j - 1, k));
+        double dN = (1. - Sx) * (NE - Ezyf) + (1. + Sx) * (NW - Ezyfm);
+        double dS = (1. - Sxm) * (Ezyf - SE) + (1. + Sxm) * (Ezyfm - SW);
+        double Sy = copysign(1., FD(1, i, j, k)), Sym = copysign(1., FD(1, i - 1, j, k));
+        double dE = (1. - Sy) * (NE - Ezxf) + (1. + Sy) * (SE - Ezxfm);
+        double dW = (1. - Sym) * (Ezxf - NW) + (1. + Sym) * (Ezxfm - SW);
+        E[2 * G.nedge + eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused update: new = lin(A,B) + sqrtg dt dtfact S + dt dtfact divF (cells),
+//               Bnew = lin(BA,BB) + dt dtfact curlE (faces).
+// lin(A,B) = A when cb == 0, else ca A + cb B.
+// ---------------------------------------------------------------------------
+struct UpdArgs {
+    double* nst; int nnv;            // new state, vars per block
+    const double* ast; int anv;      // A
+    const double* bst; int bnv;      // B (may be null)
+    double ca, cb;
+    const double* mstate; int mnv;   // metric source
+    const double* aux;
+    const double* flux[3];
+    const double* emf;
+    double* nB[3]; const double* aB[3]; const double* bB[3];
+    const double* dt; double dtfact;
+    const DevInst* inst; const DevBlk* blk;
+};
+
+GHD double fdder(const double* P, int64_t s, double ih, int order) {
+    if (order == 2) return (1.0 / 2.0) * ih * ((P[s] - P[-s]));
+    if (order == 4) return (1.0 / 12.0) * ih * (+8.0 * (P[s] - P[-s]) - (P[2 * s] - P[-2 * s]));
+    return (1.0 / 60.0) * ih * (+45.0 * (P[s] - P[-s]) - 9.0 * (P[2 * s] - P[-2 * s]) + (P[3 * s] - P[-3 * s]));
+}
+
+// geometric sources (same formulas as the reference generator output,
+// grmhd_subexpressions.hh:158-316; derivation from T^{mu nu})
+GHD void geom_sources(const Met& m, const double K[6], const double da[3], const double dg[18], const double db[9],
+                      double rho, double press, double eps, const double B[3], const double z[3], double W,
+                      double& stau, double ss[3]) {
+    double b[4], b2; smallb(m, B, z, W, b, b2);
+    double rhoh = b2 + eps * rho + press + rho, ps = press + 0.5 * b2;
+    double ia2 = 1. / (m.alp * m.alp);
+    double u[4] = {W / m.alp, z[0] - W * m.beta[0] / m.alp, z[1] - W * m.beta[1] / m.alp, z[2] - W * m.beta[2] / m.alp};
+    const int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    double T00 = rhoh * u[0] * u[0] + ps * (-ia2) - b[0] * b[0];
+    double T0i[3], Tij[6];
+    for (int i = 0; i < 3; ++i) T0i[i] = rhoh * u[0] * u[1 + i] + ps * (m.beta[i] * ia2) - b[0] * b[1 + i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j)
+            Tij[sidx[i][j]] = rhoh * u[1 + i] * u[1 + j] + ps * (m.gi[sidx[i][j]] - m.beta[i] * m.beta[j] * ia2) - b[1 + i] * b[1 + j];
+    double Kb[3];
+    for (int i = 0; i < 3; ++i) Kb[i] = K[sidx[i][0]] * m.beta[0] + K[sidx[i][1]] * m.beta[1] + K[sidx[i][2]] * m.beta[2];
+    double bKb = m.beta[0] * Kb[0] + m.beta[1] * Kb[1] + m.beta[2] * Kb[2];
+    double T0K = 0, TK = 0, Tda = 0;
+    for (int i = 0; i < 3; ++i) {
+        T0K += T0i[i] * Kb[i];
+        for (int j = 0; j < 3; ++j) TK += Tij[sidx[i][j]] * K[sidx[i][j]];
+        Tda += (T00 * m.beta[i] + T0i[i]) * da[i];
+    }
+    stau = m.alp * ((T00 * bKb + 2. * T0K + TK) - Tda);
+    double bsh[3]; lower3(m.g, m.beta[0], m.beta[1], m.beta[2], bsh);
+    double T0l[3];
+    for (int i = 0; i < 3; ++i)
+        T0l[i] = bsh[i] * T00 + m.g[sidx[i][0]] * T0i[0] + m.g[sidx[i][1]] * T0i[1] + m.g[sidx[i][2]] * T0i[2];
+    for (int k = 0; k < 3; ++k) {
+        const double* dgk = dg + 6 * k;
+        double bdgb = 0, T0dgb = 0, Tdg = 0, T0db = 0;
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) {
+                double d = dgk[sidx[i][j]];
+                bdgb += m.beta[i] * m.beta[j] * d;
+                T0dgb += T0i[i] * m.beta[j] * d;
+                Tdg += Tij[sidx[i][j]] * d;
+            }
+            T0db += T0l[i] * db[i + 3 * k];
+        }
+        ss[k] = m.alp * (T00 * (0.5 * bdgb - m.alp * da[k]) + T0dgb + T0db + 0.5 * Tdg);
+    }
+}
+
+template <bool FLAT>
+__global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order) {
+    int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    int64_t per = (int64_t)R0 * R1 * R2;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * G.nq) return;
+    int64_t q = t / per, r = t - q * per;
+    int i = G.ng + (int)(r % R0), j = G.ng + (int)((r / R0) % R1), k = G.ng + (int)(r / ((int64_t)R0 * R1));
+    const DevInst& I = a.inst[a.blk[q].inst];
+    const double ix = I.idx[0], iy = I.idx[1], iz = I.idx[2];
+    const double dtf = (*a.dt) * a.dtfact;
+    const bool ci = i < G.n[0] + G.ng, cj = j < G.n[1] + G.ng, ck = k < G.n[2] + G.ng;
+    const int64_t nc = G.ncell;
+    if (ci && cj && ck) {
+        const int64_t c = cidx(G, i, j, k);
+        double* Nn = a.nst + q * (int64_t)a.nnv * nc + c;
+        const double* An = a.ast + q * (int64_t)a.anv * nc + c;
+        const double* Bn = a.bst ? a.bst + q * (int64_t)a.bnv * nc + c : nullptr;
+        double u[NHRSC];
+#pragma unroll
+        for (int v = 0; v < NHRSC; ++v) u[v] = (a.cb == 0.0) ? An[v * nc] : a.ca * An[v * nc] + a.cb * Bn[v * nc];
+        if constexpr (!FLAT) {
+            const double* M = a.mstate + q * (int64_t)a.mnv * nc + c;
+            const double* X = a.aux + q * (int64_t)NAUX * nc + c;
+            Met m;
+            for (int b = 0; b < 6; ++b) m.g[b] = M[(GXX_ + b) * nc];
+            m.alp = M[ALP_ * nc];
+            for (int b = 0; b < 3; ++b) m.beta[b] = M[(BETAX_ + b) * nc];
+            m.init();
+            double z[3] = {X[ZVECX_ * nc], X[ZVECY_ * nc], X[ZVECZ_ * nc]};
+            double B[3] = {X[BX_ * nc], X[BY_ * nc], X[BZ_ * nc]};
+            double W = lorentz(m.g, z);
+            double da[3], dg[18], db[9], K[6];
+            const int64_t sd[3] = {1, G.N[0], (int64_t)G.N[0] * G.N[1]};
+            for (int d = 0; d < 3; ++d) da[d] = fdder(M + ALP_ * nc, sd[d], ix, fd_order);
+            for (int d = 0; d < 3; ++d)
+                for (int cc = 0; cc < 3; ++cc) db[cc + 3 * d] = fdder(M + (BETAX_ + cc) * nc, sd[d], ix, fd_order);
+            for (int d = 0; d < 3; ++d)
+                for (int b = 0; b < 6; ++b) dg[b + 6 * d] = fdder(M + (GXX_ + b) * nc, sd[d], ix, fd_order);
+            for (int b = 0; b < 6; ++b) K[b] = M[(KXX_ + b) * nc];
+            double stau, ss[3];
+            geom_sources(m, K, da, dg, db, X[RHO_ * nc], X[PRESS_ * nc], X[EPS_ * nc], B, z, W, stau, ss);
+            u[TAU_] += m.sqrtg * (*a.dt) * a.dtfact * stau;
+            u[SX_] += m.sqrtg * (*a.dt) * a.dtfact * ss[0];
+            u[SY_] += m.sqrtg * (*a.dt) * a.dtfact * ss[1];
+            u[SZ_] += m.sqrtg * (*a.dt) * a.dtfact * ss[2];
+        }
+        const int64_t f0 = fidx<0>(G, i, j, k), f1 = fidx<1>(G, i, j, k), f2 = fidx<2>(G, i, j, k);
+        const double* F0 = a.flux[0] + q * NHRSC * G.nface[0];
+        const double* F1 = a.flux[1] + q * NHRSC * G.nface[1];
+        const double* F2 = a.flux[2] + q * NHRSC * G.nface[2];
+        const int64_t s1 = G.N[0], s2 = (int64_t)G.N[0] * (G.N[1] + 0);
+#pragma unroll
+        for (int v = 0; v < NHRSC; ++v) {
+            double dFx = (F0[v * G.nface[0] + f0] - F0[v * G.nface[0] + f0 + 1]) * ix;
+            double dFy = (F1[v * G.nface[1] + f1] - F1[v * G.nface[1] + f1 + s1]) * iy;
+            double dFz = (F2[v * G.nface[2] + f2] - F2[v * G.nface[2] + f2 + s2]) * iz;
+            u[v] += dtf * ((dFx + dFy) + dFz);
+        }
+#pragma unroll
+        for (int v = 0; v < NHRSC; ++v) Nn[v * nc] = u[v];
+    }
+    const double* E = a.emf + q * 3 * G.nedge;
+    auto Ex = [&](int ii, int jj, int kk) { return E[eidx(G, ii, jj, kk)]; };
+    auto Ey = [&](int ii, int jj, int kk) { return E[G.nedge + eidx(G, ii, jj, kk)]; };
+    auto Ez = [&](int ii, int jj, int kk) { return E[2 * G.nedge + eidx(G, ii, jj, kk)]; };
+    if (cj && ck) {  // x face (i in [ng, n+ng])
+        int64_t f = q * G.nface[0] + fidx<0>(G, i, j, k);
+        double b = (a.cb == 0.0) ? a.aB[0][f] : a.ca * a.aB[0][f] + a.cb * a.bB[0][f];
+        a.nB[0][f] = b + dtf * ((Ey(i, j, k + 1) - Ey(i, j, k)) * iz + (Ez(i, j, k) - Ez(i, j + 1, k)) * iy);
+    }
+    if (ci && ck) {
+        int64_t f = q * G.nface[1] + fidx<1>(G, i, j, k);
+        double b = (a.cb == 0.0) ? a.aB[1][f] : a.ca * a.aB[1][f] + a.cb * a.bB[1][f];
+        a.nB[1][f] = b + dtf * ((Ez(i + 1, j, k) - Ez(i, j, k)) * ix + (Ex(i, j, k) - Ex(i, j, k + 1)) * iz);
+    }
+    if (ci && cj) {
+        int64_t f = q * G.nface[2] + fidx<2>(G, i, j, k);
+        double b = (a.cb == 0.0) ? a.aB[2][f] : a.ca * a.aB[2][f] + a.cb * a.bB[2][f];
+        a.nB[2][f] = b + dtf * ((Ex(i, j + 1, k) - Ex(i, j, k)) * iy + (Ey(i, j, k) - Ey(i + 1, j, k)) * ix);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost gather.  Index space per block: [cells | x faces | y faces | z faces].
+// Each ghost element copies the element at the same global position (clamped
+// for outflow, wrapped for periodic) from the owning block.
+// ---------------------------------------------------------------------------
+GHD int map_cc(int g, int N, int lo, int hi) {
+    if (g < 0) return lo == 1 ? ((g % N) + N) % N : 0;
+    if (g >= N) return hi == 1 ? g % N : N - 1;
+    return g;
+}
+GHD int map_fc(int f, int N, int lo, int hi) {
+    if (f < 0) return lo == 1 ? ((f % N) + N) % N : 0;
+    if (f > N) return hi == 1 ? f % N : N;
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_ghost(DevGrid G, const DevInst* __restrict__ inst, const DevBlk* __restrict__ blk,
+                                               double* st, int nv, int v0, int vcount, double* Bx, double* By,
+                                               double* Bz, double* aux, int a0, int acount) {
+    int64_t per = G.ncell + G.nface[0] + G.nface[1] + G.nface[2];
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * G.nq) return;
+    int64_t q = t / per, r = t - q * per;
+    int fd = -1;
+    int64_t off = r;
+    if (off >= G.ncell) { off -= G.ncell; fd = 0;
+        if (off >= G.nface[0]) { off -= G.nface[0]; fd = 1;
+            if (off >= G.nface[1]) { off -= G.nface[1]; fd = 2; } } }
+    if (fd >= 0 && Bx == nullptr) return;
+    int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
+    int loc[3] = {(int)(off % E0), (int)((off / E0) % E1), (int)(off / ((int64_t)E0 * E1))};
+    bool ghost = false;
+    for (int d = 0; d < 3; ++d) {
+        int hi = G.n[d] + G.ng + (d == fd ? 1 : 0);
+        ghost |= (loc[d] < G.ng || loc[d] >= hi);
+    }
+    if (!ghost) return;
+    const DevBlk b = blk[q];
+    const DevInst& I = inst[b.inst];
+    int sb[3], sl[3];
+    for (int d = 0; d < 3; ++d) {
+        int Nt = I.nb[d] * G.n[d];
+        int g = b.b[d] * G.n[d] + loc[d] - G.ng;
+        if (d == fd) {
+            int m = map_fc(g, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
+            if (m == Nt) { sb[d] = I.nb[d] - 1; sl[d] = G.n[d] + G.ng; }
+            else { sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng; }
+        } else {
+            int m = map_cc(g, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
+            sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng;
+        }
+    }
+    int64_t qs = I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]);
+    if (fd < 0) {
+        int64_t cd = cidx(G, loc[0], loc[1], loc[2]), cs = cidx(G, sl[0], sl[1], sl[2]);
+        if (st)
+            for (int v = v0; v < v0 + vcount; ++v) st[(q * nv + v) * G.ncell + cd] = st[(qs * nv + v) * G.ncell + cs];
+        if (aux)
+            for (int v = a0; v < a0 + acount; ++v) aux[(q * NAUX + v) * G.ncell + cd] = aux[(qs * NAUX + v) * G.ncell + cs];
+    } else {
+        double* Bd = fd == 0 ? Bx : (fd == 1 ? By : Bz);
+        int64_t nf = G.nface[fd];
+        Bd[q * nf + fidx_d(G, fd, loc[0], loc[1], loc[2])] = Bd[qs * nf + fidx_d(G, fd, sl[0], sl[1], sl[2])];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// c2p statistics (global counters, cleared by a kernel)
+// ---------------------------------------------------------------------------
+struct DevStats {
+    unsigned long long n_cells, n_fatal, n_atmo, n_ent, n_guess, n_root, n_bracket, max_root;
+};
+__global__ void k_clear_stats(DevStats* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *s = DevStats{0, 0, 0, 0, 0, 0, 0, 0};
+}
+// Wave-level reduction of the per-lane counters, one atomic per wavefront.
+__device__ inline void stats_add(DevStats* s, bool fatal, uint64_t err, const C2PCount& cnt) {
+    unsigned long long a[7] = {1ull, (unsigned long long)fatal, (unsigned long long)((err >> E_ATMO_RESET) & 1),
+                               (unsigned long long)((err >> E_ENT_BACKUP_USED) & 1), (unsigned long long)cnt.guess_used,
+                               (unsigned long long)cnt.root, (unsigned long long)cnt.bracket};
+    unsigned long long mx = (unsigned long long)cnt.root;
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int x = 0; x < 7; ++x) a[x] += __shfl_down(a[x], off, 64);
+        unsigned long long o = __shfl_down(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&s->n_cells, a[0]); atomicAdd(&s->n_fatal, a[1]); atomicAdd(&s->n_atmo, a[2]);
+        atomicAdd(&s->n_ent, a[3]); atomicAdd(&s->n_guess, a[4]); atomicAdd(&s->n_root, a[5]);
+        atomicAdd(&s->n_bracket, a[6]); atomicMax(&s->max_root, mx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// c2p over cells (compute_auxiliaries)
+// ---------------------------------------------------------------------------
+template <typename EOS, bool FLAT>
+__global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, const DevInst* __restrict__ inst,
+                                                   const DevBlk* __restrict__ blk, double* st, int snv,
+                                                   const double* __restrict__ mstate, int mnv,
+                                                   const double* __restrict__ Bx, const double* __restrict__ By,
+                                                   const double* __restrict__ Bz, double* aux, double* ecen,
+                                                   DevStats* stats) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = t < G.ncell * G.nq;
+    int64_t tt = active ? t : 0;
+    int64_t q = tt / G.ncell, c = tt - q * G.ncell;
+    int i = (int)(c % G.N[0]), j = (int)((c / G.N[0]) % G.N[1]), k = (int)(c / ((int64_t)G.N[0] * G.N[1]));
+    const int64_t nc = G.ncell;
+    Met m;
+    if constexpr (FLAT) m.flat();
+    else load_metric(m, mstate + q * (int64_t)mnv * nc, nc, c);
+    double* S = st + q * (int64_t)snv * nc + c;
+    double* A = aux + q * (int64_t)NAUX * nc + c;
+    const double* bx = Bx + q * G.nface[0]; const double* by = By + q * G.nface[1]; const double* bz = Bz + q * G.nface[2];
+    int64_t fx = fidx<0>(G, i, j, k), fy = fidx<1>(G, i, j, k), fz = fidx<2>(G, i, j, k);
+    double bx0 = bx[fx], bx1 = bx[fx + 1], by0 = by[fy], by1 = by[fy + G.N[0]];
+    double bz0 = bz[fz], bz1 = bz[fz + (int64_t)G.N[0] * G.N[1]];
+    double cons[NCONS];
+    for (int v = 0; v < NHRSC; ++v) cons[v] = S[v * nc];
+    // cons order DENS SX SY SZ TAU YE* S* matches the state order
+    cons[BSXL] = 0.5 * (bx0 + bx1); cons[BSYL] = 0.5 * (by0 + by1); cons[BSZL] = 0.5 * (bz0 + bz1);
+    double Bc0 = cons[BSXL], Bc1 = cons[BSYL], Bc2 = cons[BSZL];
+    uint64_t err = 0;
+    double pr[NPRIM];
+    C2PCount cnt;
+    C2POut out{false, false};
+    if (active) {
+        A[BX_ * nc] = Bc0 / m.sqrtg; A[BY_ * nc] = Bc1 / m.sqrtg; A[BZ_ * nc] = Bc2 / m.sqrtg;
+        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, 1e300, err, false, cnt, 0.0, 0.0);
+        A[RHO_ * nc] = pr[RHOL]; A[EPS_ * nc] = pr[EPSL]; A[PRESS_ * nc] = pr[PRESSL];
+        A[TEMP_ * nc] = pr[TEMPL]; A[ENTROPY_ * nc] = pr[ENTL]; A[YE_ * nc] = pr[YEL];
+        A[ZVECX_ * nc] = pr[ZXL]; A[ZVECY_ * nc] = pr[ZYL]; A[ZVECZ_ * nc] = pr[ZZL];
+        if (err & BIT(E_RESET_DENS)) { A[C2P_DENS_ERR_ * nc] += S[DENS_ * nc] - cons[DENSL]; S[DENS_ * nc] = cons[DENSL]; }
+        if (err & BIT(E_RESET_STILDE)) { S[SX_ * nc] = cons[STXL]; S[SY_ * nc] = cons[STYL]; S[SZ_ * nc] = cons[STZL]; }
+        if (err & BIT(E_RESET_TAU)) S[TAU_ * nc] = cons[TAUL];
+        if (err & BIT(E_RESET_ENTROPY)) S[ENTROPYSTAR_ * nc] = cons[ENTSL];
+        if (err & BIT(E_RESET_YE)) S[YESTAR_ * nc] = cons[YESL];
+        uint64_t prev = (uint64_t)A[C2P_ERR_ * nc];
+        A[C2P_ERR_ * nc] = (double)(prev | err);
+        const DevInst& I = inst[blk[q].inst];
+        A[BDIV_ * nc] = ((bx1 - bx0) * I.idx[0] + (by1 - by0) * I.idx[1] + (bz1 - bz0) * I.idx[2]) / m.sqrtg;
+        if (ecen) {
+            double z[3] = {pr[ZXL], pr[ZYL], pr[ZZL]};
+            double W = sqrt(1. + m.sqv(z[0], z[1], z[2]));
+            double vt[3] = {m.alp * z[0] / W - m.beta[0], m.alp * z[1] / W - m.beta[1], m.alp * z[2] / W - m.beta[2]};
+            double* Ec = ecen + q * 3 * nc + c;
+            Ec[0] = Bc1 * vt[2] - Bc2 * vt[1];
+            Ec[nc] = -Bc0 * vt[2] + Bc2 * vt[0];
+            Ec[2 * nc] = Bc0 * vt[1] - Bc1 * vt[0];
+        }
+    }
+    if (!active) { cnt = C2PCount(); }
+    if (stats) {
+        if (active) stats_add(stats, out.fatal, err, cnt);
+        else stats_add(stats, false, 0, cnt);
+    }
+}
+
+// Remove the per-lane "n_cells" count of inactive lanes: done on the host by
+// recording the launch size; the device count includes inactive lanes, so the
+// host overwrites n_cells with the true count.
+
+// ---------------------------------------------------------------------------
+// c2p over a flat batch
+// ---------------------------------------------------------------------------
+template <typename EOS>
+__global__ __launch_bounds__(256) void k_c2p_batch(int64_t n, EOS eos, Opts o, double* cons_soa,
+                                                   const double* __restrict__ met, double* prims,
+                                                   uint64_t* errb, uint8_t* floored, int dry_run,
+                                                   const double* __restrict__ rad, const double* __restrict__ guess,
+                                                   DevStats* stats) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = t < n;
+    int64_t s = active ? t : 0;
+    Met m;
+    for (int a = 0; a < 6; ++a) m.g[a] = met[a * n + s];
+    m.alp = met[6 * n + s];
+    for (int a = 0; a < 3; ++a) m.beta[a] = met[(7 + a) * n + s];
+    m.init();
+    double cons[NCONS], pr[NPRIM];
+    for (int v = 0; v < NCONS; ++v) cons[v] = cons_soa[v * n + s];
+    for (int v = 0; v < NPRIM; ++v) pr[v] = 0.0;
+    double glo = 0, ghi = 0;
+    if (guess) { glo = guess[s]; ghi = guess[n + s]; }
+    uint64_t err = 0;
+    C2PCount cnt;
+    C2POut out{false, false};
+    if (active) {
+        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, rad ? rad[s] : 1e300, err, dry_run != 0, cnt, glo, ghi);
+        for (int v = 0; v < NPRIM; ++v) prims[v * n + s] = pr[v];
+        for (int v = 0; v < NCONS; ++v) cons_soa[v * n + s] = cons[v];
+        errb[s] = err;
+        floored[s] = out.floored ? 1 : 0;
+    } else {
+        cnt = C2PCount();
+    }
+    if (stats) stats_add(stats, active && out.fatal, active ? err : 0, cnt);
+}
+
+__global__ __launch_bounds__(256) void k_p2c_batch(int64_t n, const double* __restrict__ prims,
+                                                   const double* __restrict__ met, double* cons_soa) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    Met m;
+    for (int a = 0; a < 6; ++a) m.g[a] = met[a * n + s];
+    m.alp = met[6 * n + s];
+    for (int a = 0; a < 3; ++a) m.beta[a] = met[(7 + a) * n + s];
+    m.init();
+    double pr[NPRIM], cons[NCONS];
+    for (int v = 0; v < NPRIM; ++v) pr[v] = prims[v * n + s];
+    for (int v = 0; v < NCONS; ++v) cons[v] = cons_soa[v * n + s];
+    p2c(pr, cons, m);
+    for (int v = 0; v < 7; ++v) cons_soa[v * n + s] = cons[v];
+}
+
+// ---------------------------------------------------------------------------
+// dt: per-WG minima then a single-WG reduce
+// ---------------------------------------------------------------------------
+template <typename EOS, bool FLAT>
+__global__ __launch_bounds__(256) void k_dt_partial(DevGrid G, EOS eos, const DevInst* __restrict__ inst,
+                                                    const DevBlk* __restrict__ blk, const double* __restrict__ mstate,
+                                                    int mnv, const double* __restrict__ aux, double cfl,
+                                                    double* __restrict__ part) {
+    __shared__ double red[256];
+    int64_t per = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double dtl = 1e300;
+    if (t < per * G.nq) {
+        int64_t q = t / per, r = t - q * per;
+        int i = G.ng + (int)(r % G.n[0]), j = G.ng + (int)((r / G.n[0]) % G.n[1]), k = G.ng + (int)(r / ((int64_t)G.n[0] * G.n[1]));
+        const int64_t nc = G.ncell, c = cidx(G, i, j, k);
+        Met m;
+        if constexpr (FLAT) m.flat(); else load_metric(m, mstate + q * (int64_t)mnv * nc, nc, c);
+        const double* X = aux + q * (int64_t)NAUX * nc + c;
+        double z[3] = {X[ZVECX_ * nc], X[ZVECY_ * nc], X[ZVECZ_ * nc]}, B[3] = {X[BX_ * nc], X[BY_ * nc], X[BZ_ * nc]};
+        double rho = X[RHO_ * nc], T = X[TEMP_ * nc], eps = X[EPS_ * nc], press = X[PRESS_ * nc];
+        double h, cs2; unsigned e = 0;
+        eos.press_h_cs2_from_temp(h, cs2, T, rho, e);
+        double W = lorentz(m.g, z);
+        double b[4], b2; smallb(m, B, z, W, b, b2);
+        double vt[3]; vtilde(m, z, W, vt);
+        double cmax = 0;
+        for (int d = 0; d < 3; ++d) {
+            double cm, cp;
+            cmcp(m, rho, press, eps, cs2, W, b2, vt, d, cm, cp);
+            cmax = fmax(cmax, fmax(fabs(cp), fabs(cm)));
+        }
+        double dx0 = 1.0 / inst[blk[q].inst].idx[0];
+        dtl = cfl / cmax * dx0;
+    }
+    red[threadIdx.x] = dtl;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ part, int64_t np, double* dt) {
+    __shared__ double red[256];
+    double v = 1e300;
+    for (int64_t x = threadIdx.x; x < np; x += blockDim.x) v = fmin(v, part[x]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dt = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_reset_diag(int64_t ncell, int64_t nq, double* aux) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncell * nq) return;
+    int64_t q = t / ncell, c = t - q * ncell;
+    aux[(q * NAUX + C2P_DENS_ERR_) * ncell + c] = 0.0;
+    aux[(q * NAUX + C2P_ERR_) * ncell + c] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Host side: context
+// ---------------------------------------------------------------------------
+struct TimerRec { std::string fam; hipEvent_t a, b; };
+
+struct grmhd_ctx {
+    grmhd_params P;
+    DevGrid G;
+    Opts O;
+    EosIdeal ideal;
+    EosHybrid hybrid;
+    std::vector<DevInst> inst_h;
+    std::vector<DevBlk> blk_h;
+    DevInst* inst_d = nullptr;
+    DevBlk* blk_d = nullptr;
+    double* flux[3] = {nullptr, nullptr, nullptr};
+    double* vbar[3] = {nullptr, nullptr, nullptr};
+    double* emf = nullptr;
+    double* ecen = nullptr;
+    double* U1 = nullptr; double* U2 = nullptr;        // stage buffers (7 vars)
+    double* B1[3] = {nullptr, nullptr, nullptr};
+    double* B2[3] = {nullptr, nullptr, nullptr};
+    double* dtpart = nullptr; int64_t n_dtpart = 0;
+    DevStats* stats = nullptr;
+    int64_t last_c2p_n = 0;
+    bool timing = false;
+    std::vector<TimerRec> timers;
+    std::vector<hipEvent_t> ev_pool;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void* gkey[8] = {0};
+};
+
+static hipEvent_t pool_event(grmhd_ctx* c) {
+    if (!c->ev_pool.empty()) { hipEvent_t e = c->ev_pool.back(); c->ev_pool.pop_back(); return e; }
+    hipEvent_t e; (void)hipEventCreate(&e); return e;
+}
+struct Scope {
+    grmhd_ctx* c; const char* fam; hipStream_t s; hipEvent_t a = nullptr, b = nullptr;
+    Scope(grmhd_ctx* c_, const char* f, hipStream_t s_) : c(c_), fam(f), s(s_) {
+        if (c->timing) { a = pool_event(c); (void)hipEventRecord(a, s); }
+    }
+    ~Scope() {
+        if (c->timing) { b = pool_event(c); (void)hipEventRecord(b, s); c->timers.push_back({fam, a, b}); }
+    }
+};
+
+static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+extern "C" void grmhd_default_params(grmhd_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->eos_kind = GRMHD_EOS_IDEAL_GAS;
+    p->gamma = 2.0;
+    p->eps_max = 1e5;
+    p->entropy_min = 1e-13;
+    p->gamma_th = 1.8;
+    p->pwp_n = 1;
+    p->pwp_kappa0 = 1.685819e+02;
+    p->pwp_gamma[0] = 2.0;
+    p->rho_fl = 1e-14; p->temp_fl = 0.0; p->ye_fl = 0.0; p->atmo_tol = 1e-3;
+    p->c2p_tol = 1e-10; p->max_lorentz = 50; p->max_sigma = 100; p->beta_fallback = 1e-2;
+    p->use_entropy_backup = 1; p->bh_alp_thresh = 0.2;
+    p->excise_by_radius = 0; p->excision_radius = 1.0; p->excision_lapse = 0.05;
+    p->recon = GRMHD_RECON_WENOZ; p->riemann = GRMHD_RIEMANN_HLLE; p->emf = GRMHD_EMF_UCT;
+    p->thermo_recon = GRMHD_THERMO_TEMP; p->fd_order = 6; p->rusanov_c_limit = 0.0;
+    p->flat_metric = 0;
+}
+
+static void build_eos(grmhd_ctx* c) {
+    const grmhd_params& p = c->P;
+    c->ideal.gm1 = p.gamma - 1.0;
+    c->ideal.gamma = c->ideal.gm1 + 1.0;
+    c->ideal.ggm1 = c->ideal.gamma * c->ideal.gm1;
+    c->ideal.eps_max = p.eps_max;
+    c->ideal.temp_atm = p.temp_fl;
+    c->ideal.ent_min = (p.temp_fl > 0.0) ? (p.temp_fl / c->ideal.gm1) : 1e-100;
+    // hybrid over a piecewise polytrope (eos_storage.cpp:52-99)
+    EosHybrid& h = c->hybrid;
+    h.n = p.pwp_n;
+    double rh[8] = {0};
+    rh[0] = 0.0;
+    for (int i = 1; i < h.n; ++i) rh[i] = p.pwp_rho[i - 1];
+    h.K[0] = p.pwp_kappa0; h.ep[0] = 0.0;
+    for (int i = 0; i < h.n; ++i) { h.G[i] = p.pwp_gamma[i]; h.rh[i] = rh[i]; }
+    for (int i = 1; i < h.n; ++i) {
+        h.K[i] = h.K[i - 1] * pow(rh[i], h.G[i - 1] - h.G[i]);
+        h.ep[i] = h.ep[i - 1] + h.K[i - 1] * pow(rh[i], h.G[i - 1] - 1.) / (h.G[i - 1] - 1.)
+                  - h.K[i] * pow(rh[i], h.G[i] - 1.) / (h.G[i] - 1.);
+    }
+    h.gth_m1 = p.gamma_th - 1.0;
+    h.entropy_min = p.entropy_min;
+    h.eps_max = p.eps_max;
+    h.temp_atm = p.temp_fl;
+}
+
+static void free_ctx(grmhd_ctx* c) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats);
+    for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto& t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+}
+
+extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid* g, const grmhd_instance* inst,
+                                         int32_t n_inst, grmhd_ctx** out) {
+    if (!p || !g || !out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    *out = nullptr;
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->ngz < 3 || g->nq < 1) {
+        set_err("invalid grid (need n>=1, ngz>=3, nq>=1): %d %d %d ngz=%d nq=%lld", g->nx, g->ny, g->nz, g->ngz,
+                (long long)g->nq);
+        return GRMHD_ERR_INVALID;
+    }
+    if (p->eos_kind != GRMHD_EOS_IDEAL_GAS && p->eos_kind != GRMHD_EOS_HYBRID_PWPOLY) { set_err("eos_kind"); return GRMHD_ERR_UNSUPPORTED; }
+    if (p->recon < 0 || p->recon > 3 || p->riemann < 0 || p->riemann > 1 || p->emf < 0 || p->emf > 1 ||
+        p->thermo_recon < 0 || p->thermo_recon > 1 || (p->fd_order != 2 && p->fd_order != 4 && p->fd_order != 6)) {
+        set_err("invalid scheme options"); return GRMHD_ERR_INVALID;
+    }
+    if (p->eos_kind == GRMHD_EOS_HYBRID_PWPOLY && (p->pwp_n < 1 || p->pwp_n > 8)) { set_err("pwp_n"); return GRMHD_ERR_INVALID; }
+    if (p->eos_kind == GRMHD_EOS_IDEAL_GAS && !(p->gamma > 1.0)) { set_err("gamma must be > 1"); return GRMHD_ERR_INVALID; }
+    if (p->fd_order / 2 > g->ngz) { set_err("fd_order needs more ghosts"); return GRMHD_ERR_INVALID; }
+    // instance / block tables
+    std::vector<DevBlk> blk(g->nq, DevBlk{-1, {0, 0, 0}});
+    for (int a = 0; a < n_inst; ++a) {
+        const grmhd_instance& I = inst[a];
+        int64_t nb = (int64_t)I.nb[0] * I.nb[1] * I.nb[2];
+        if (I.nb[0] < 1 || I.nb[1] < 1 || I.nb[2] < 1 || I.q0 < 0 || I.q0 + nb > g->nq) {
+            set_err("instance %d out of range", a); return GRMHD_ERR_INVALID;
+        }
+        for (int d = 0; d < 6; ++d) if (I.bc[d] != 0 && I.bc[d] != 1) { set_err("bc kind"); return GRMHD_ERR_INVALID; }
+        for (int d = 0; d < 3; ++d) if (!(I.idx[d] > 0)) { set_err("idx must be > 0"); return GRMHD_ERR_INVALID; }
+        for (int bk = 0; bk < I.nb[2]; ++bk)
+            for (int bj = 0; bj < I.nb[1]; ++bj)
+                for (int bi = 0; bi < I.nb[0]; ++bi) {
+                    int64_t q = I.q0 + bi + (int64_t)I.nb[0] * (bj + (int64_t)I.nb[1] * bk);
+                    if (blk[q].inst >= 0) { set_err("block %lld in two instances", (long long)q); return GRMHD_ERR_INVALID; }
+                    blk[q] = DevBlk{a, {bi, bj, bk}};
+                }
+    }
+    for (int64_t q = 0; q < g->nq; ++q) if (blk[q].inst < 0) { set_err("block %lld not covered", (long long)q); return GRMHD_ERR_INVALID; }
+    grmhd_ctx* c = new grmhd_ctx();
+    c->P = *p;
+    DevGrid& G = c->G;
+    G.n[0] = g->nx; G.n[1] = g->ny; G.n[2] = g->nz; G.ng = g->ngz; G.nq = g->nq;
+    for (int d = 0; d < 3; ++d) G.N[d] = G.n[d] + 2 * G.ng;
+    G.ncell = (int64_t)G.N[0] * G.N[1] * G.N[2];
+    for (int d = 0; d < 3; ++d) G.nface[d] = (int64_t)(G.N[0] + (d == 0)) * (G.N[1] + (d == 1)) * (G.N[2] + (d == 2));
+    G.nedge = (int64_t)(G.N[0] + 1) * (G.N[1] + 1) * (G.N[2] + 1);
+    Opts& O = c->O;
+    O.atmo = Atmo{p->rho_fl, p->temp_fl, p->ye_fl, p->atmo_tol};
+    O.cp = C2PPar{p->c2p_tol, p->max_lorentz, p->max_sigma, p->beta_fallback, p->bh_alp_thresh, p->use_entropy_backup};
+    O.ex = ExcPar{p->excision_radius, p->excision_lapse, p->excise_by_radius};
+    O.riemann = p->riemann; O.emf = p->emf; O.thermo = p->thermo_recon; O.fd_order = p->fd_order;
+    O.c_limit = p->rusanov_c_limit;
+    build_eos(c);
+    c->blk_h = blk;
+    for (int a = 0; a < n_inst; ++a) {
+        DevInst d;
+        for (int x = 0; x < 3; ++x) { d.nb[x] = inst[a].nb[x]; d.idx[x] = inst[a].idx[x]; }
+        for (int x = 0; x < 6; ++x) d.bc[x] = inst[a].bc[x];
+        d.q0 = inst[a].q0;
+        c->inst_h.push_back(d);
+    }
+    auto fail = [&](const char* what, hipError_t e) {
+        set_err("%s: %s", what, hipGetErrorString(e));
+        free_ctx(c); delete c;
+        return GRMHD_ERR_ALLOC;
+    };
+    hipError_t e;
+    if (n_inst > 0) {
+        if ((e = hipMalloc(&c->inst_d, sizeof(DevInst) * n_inst)) != hipSuccess) return fail("inst", e);
+        if ((e = hipMemcpy(c->inst_d, c->inst_h.data(), sizeof(DevInst) * n_inst, hipMemcpyHostToDevice)) != hipSuccess) return fail("inst copy", e);
+    }
+    if ((e = hipMalloc(&c->blk_d, sizeof(DevBlk) * g->nq)) != hipSuccess) return fail("blk", e);
+    if ((e = hipMemcpy(c->blk_d, blk.data(), sizeof(DevBlk) * g->nq, hipMemcpyHostToDevice)) != hipSuccess) return fail("blk copy", e);
+    for (int d = 0; d < 3; ++d) {
+        if ((e = hipMalloc(&c->flux[d], sizeof(double) * NHRSC * G.nface[d] * G.nq)) != hipSuccess) return fail("flux", e);
+        if ((e = hipMalloc(&c->vbar[d], sizeof(double) * 4 * G.nface[d] * G.nq)) != hipSuccess) return fail("vbar", e);
+    }
+    if ((e = hipMalloc(&c->emf, sizeof(double) * 3 * G.nedge * G.nq)) != hipSuccess) return fail("emf", e);
+    if ((e = hipMalloc(&c->ecen, sizeof(double) * 3 * G.ncell * G.nq)) != hipSuccess) return fail("ecen", e);
+    // E_center starts at zero, as the reference's freshly allocated view
+    if ((e = hipMemset(c->ecen, 0, sizeof(double) * 3 * G.ncell * G.nq)) != hipSuccess) return fail("ecen clear", e);
+    if ((e = hipMemset(c->emf, 0, sizeof(double) * 3 * G.nedge * G.nq)) != hipSuccess) return fail("emf clear", e);
+    c->n_dtpart = nblk((int64_t)G.n[0] * G.n[1] * G.n[2] * G.nq);
+    if ((e = hipMalloc(&c->dtpart, sizeof(double) * c->n_dtpart)) != hipSuccess) return fail("dtpart", e);
+    if ((e = hipMalloc(&c->stats, sizeof(DevStats))) != hipSuccess) return fail("stats", e);
+    if ((e = hipMemset(c->stats, 0, sizeof(DevStats))) != hipSuccess) return fail("stats clear", e);
+    *out = c;
+    return GRMHD_OK;
+}
+
+extern "C" void grmhd_ctx_destroy(grmhd_ctx* c) {
+    if (!c) return;
+    (void)hipDeviceSynchronize();
+    free_ctx(c);
+    delete c;
+}
+
+static grmhd_status launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_err("%s launch: %s", what, hipGetErrorString(e)); return GRMHD_ERR_HIP; }
+    return GRMHD_OK;
+}
+
+// ---- dispatch helpers -------------------------------------------------------
+template <typename EOS, int REC, bool FLAT>
+static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
+                         const double* const B[3], hipStream_t s) {
+    const DevGrid& G = c->G;
+    int64_t nx = (int64_t)(G.n[0] + 1) * G.N[1] * G.N[2] * G.nq;
+    int64_t ny = (int64_t)G.N[0] * (G.n[1] + 1) * G.N[2] * G.nq;
+    int64_t nz = (int64_t)G.N[0] * G.N[1] * (G.n[2] + 1) * G.nq;
+    k_flux<EOS, REC, FLAT, 0><<<nblk(nx), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0]);
+    k_flux<EOS, REC, FLAT, 1><<<nblk(ny), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1]);
+    k_flux<EOS, REC, FLAT, 2><<<nblk(nz), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2]);
+}
+template <typename EOS, bool FLAT>
+static void flux_rec(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
+                     const double* const B[3], hipStream_t s) {
+    switch (c->P.recon) {
+        case 0: launch_flux3<EOS, 0, FLAT>(c, eos, aux, ms, mnv, B, s); break;
+        case 1: launch_flux3<EOS, 1, FLAT>(c, eos, aux, ms, mnv, B, s); break;
+        case 2: launch_flux3<EOS, 2, FLAT>(c, eos, aux, ms, mnv, B, s); break;
+        default: launch_flux3<EOS, 3, FLAT>(c, eos, aux, ms, mnv, B, s); break;
+    }
+}
+static void launch_fluxes(grmhd_ctx* c, const double* aux, const double* ms, int mnv, const double* const B[3],
+                          hipStream_t s) {
+    Scope sc(c, "flux", s);
+    bool flat = c->P.flat_metric != 0;
+    if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) {
+        if (flat) flux_rec<EosIdeal, true>(c, c->ideal, aux, ms, mnv, B, s);
+        else flux_rec<EosIdeal, false>(c, c->ideal, aux, ms, mnv, B, s);
+    } else {
+        if (flat) flux_rec<EosHybrid, true>(c, c->hybrid, aux, ms, mnv, B, s);
+        else flux_rec<EosHybrid, false>(c, c->hybrid, aux, ms, mnv, B, s);
+    }
+}
+static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
+    Scope sc(c, "emf", s);
+    const DevGrid& G = c->G;
+    int64_t n = (int64_t)(G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1) * G.nq;
+    if (c->P.emf == GRMHD_EMF_GS) {
+        k_emf_gs<<<nblk(n), 256, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
+                                         c->flux[2], c->emf);
+        return;
+    }
+    switch (c->P.recon) {
+        case 0: k_emf_uct<0><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
+        case 1: k_emf_uct<1><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
+        case 2: k_emf_uct<2><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
+        default: k_emf_uct<3><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
+    }
+}
+static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
+    Scope sc(c, "update", s);
+    const DevGrid& G = c->G;
+    a.inst = c->inst_d; a.blk = c->blk_d;
+    for (int d = 0; d < 3; ++d) a.flux[d] = c->flux[d];
+    a.emf = c->emf;
+    int64_t n = (int64_t)(G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1) * G.nq;
+    if (c->P.flat_metric) k_update<true><<<nblk(n), 256, 0, s>>>(G, a, c->P.fd_order);
+    else k_update<false><<<nblk(n), 256, 0, s>>>(G, a, c->P.fd_order);
+}
+static void launch_ghost(grmhd_ctx* c, double* st, int nv, int v0, int vc, double* const B[3], double* aux, int a0,
+                         int ac, hipStream_t s) {
+    Scope sc(c, "ghost", s);
+    const DevGrid& G = c->G;
+    int64_t n = (G.ncell + G.nface[0] + G.nface[1] + G.nface[2]) * G.nq;
+    k_ghost<<<nblk(n), 256, 0, s>>>(G, c->inst_d, c->blk_d, st, nv, v0, vc, B ? B[0] : nullptr, B ? B[1] : nullptr,
+                                    B ? B[2] : nullptr, aux, a0, ac);
+}
+static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms, int mnv, const double* const B[3],
+                             double* aux, hipStream_t s) {
+    Scope sc(c, "c2p", s);
+    const DevGrid& G = c->G;
+    int64_t n = G.ncell * G.nq;
+    double* ec = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
+    k_clear_stats<<<1, 64, 0, s>>>(c->stats);
+    bool flat = c->P.flat_metric != 0;
+#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<nblk(n), 256, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, c->stats)
+    if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
+    else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
+#undef C2PL
+    c->last_c2p_n = n;
+}
+
+// ---- ABI entry points -------------------------------------------------------
+extern "C" grmhd_status grmhd_advance_substep(grmhd_ctx* c, const double* dt_dev, double dtfact, double* new_state,
+                                              const double* old_state, double* const new_B[3],
+                                              const double* const old_B[3], const double* aux, grmhd_stream_t stream) {
+    if (!c || !dt_dev || !new_state || !old_state || !new_B || !old_B || !aux) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    hipStream_t s = (hipStream_t)stream;
+    launch_fluxes(c, aux, old_state, NSTATE, old_B, s);
+    launch_emf(c, old_B, s);
+    UpdArgs a{};
+    a.nst = new_state; a.nnv = NSTATE;
+    a.ast = new_state; a.anv = NSTATE; a.bst = nullptr; a.bnv = 0; a.ca = 1.0; a.cb = 0.0;
+    a.mstate = old_state; a.mnv = NSTATE; a.aux = aux;
+    for (int d = 0; d < 3; ++d) { a.nB[d] = new_B[d]; a.aB[d] = new_B[d]; a.bB[d] = nullptr; }
+    a.dt = dt_dev; a.dtfact = dtfact;
+    launch_update(c, a, s);
+    return launch_check("advance_substep");
+}
+
+extern "C" grmhd_status grmhd_fill_ghosts(grmhd_ctx* c, double* state, double* const B[3], grmhd_stream_t stream) {
+    if (!c || !state || !B) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    launch_ghost(c, state, NSTATE, 0, NHRSC, B, nullptr, 0, 0, (hipStream_t)stream);
+    return launch_check("fill_ghosts");
+}
+
+extern "C" grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* c, double* state, const double* const B[3], double* aux,
+                                                  grmhd_stream_t stream) {
+    if (!c || !state || !B || !aux) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    launch_c2p_cells(c, state, NSTATE, state, NSTATE, B, aux, (hipStream_t)stream);
+    return launch_check("compute_auxiliaries");
+}
+
+extern "C" grmhd_status grmhd_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_soa, const double* metric_soa,
+                                        double* prims_soa, uint64_t* err_bits, uint8_t* floored, int32_t dry_run,
+                                        const double* r, const double* guess, grmhd_stream_t stream) {
+    if (!c || n < 0 || (n > 0 && (!cons_soa || !metric_soa || !prims_soa || !err_bits || !floored))) {
+        set_err("invalid c2p batch arguments"); return GRMHD_ERR_INVALID;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    Scope sc(c, "c2p", s);
+    k_clear_stats<<<1, 64, 0, s>>>(c->stats);
+    if (n > 0) {
+        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
+            k_c2p_batch<EosIdeal><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa, err_bits,
+                                                         floored, dry_run, r, guess, c->stats);
+        else
+            k_c2p_batch<EosHybrid><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa, prims_soa, err_bits,
+                                                          floored, dry_run, r, guess, c->stats);
+    }
+    c->last_c2p_n = n;
+    return launch_check("c2p_batch");
+}
+
+extern "C" grmhd_status grmhd_prims_to_conservs_batch(grmhd_ctx* c, int64_t n, const double* prims_soa,
+                                                      const double* metric_soa, double* cons_soa, grmhd_stream_t stream) {
+    if (!c || n < 0 || (n > 0 && (!prims_soa || !metric_soa || !cons_soa))) { set_err("invalid p2c arguments"); return GRMHD_ERR_INVALID; }
+    if (n > 0) k_p2c_batch<<<nblk(n), 256, 0, (hipStream_t)stream>>>(n, prims_soa, metric_soa, cons_soa);
+    return launch_check("p2c_batch");
+}
+
+extern "C" grmhd_status grmhd_c2p_stats(grmhd_ctx* c, grmhd_c2p_stats_t* out, grmhd_stream_t stream) {
+    if (!c || !out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    DevStats h;
+    HIP_TRY(hipMemcpyAsync(&h, c->stats, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    out->n_cells = c->last_c2p_n;
+    out->n_fatal = (int64_t)h.n_fatal;
+    out->n_atmo_reset = (int64_t)h.n_atmo;
+    out->n_ent_backup = (int64_t)h.n_ent;
+    out->n_guess_used = (int64_t)h.n_guess;
+    out->n_root_evals = (int64_t)h.n_root;
+    out->n_bracket_evals = (int64_t)h.n_bracket;
+    out->max_root_evals = (int64_t)h.max_root;
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_compute_dt(grmhd_ctx* c, const double* state, const double* aux, double cfl,
+                                         double* dt_dev, grmhd_stream_t stream) {
+    if (!c || !state || !aux || !dt_dev || !(cfl > 0)) { set_err("invalid dt arguments"); return GRMHD_ERR_INVALID; }
+    hipStream_t s = (hipStream_t)stream;
+    const DevGrid& G = c->G;
+    int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2] * G.nq;
+    bool flat = c->P.flat_metric != 0;
+#define DTL(EOSV, FL) k_dt_partial<decltype(EOSV), FL><<<nblk(n), 256, 0, s>>>(G, EOSV, c->inst_d, c->blk_d, state, NSTATE, aux, cfl, c->dtpart)
+    if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) DTL(c->ideal, true); else DTL(c->ideal, false); }
+    else { if (flat) DTL(c->hybrid, true); else DTL(c->hybrid, false); }
+#undef DTL
+    k_dt_final<<<1, 256, 0, s>>>(c->dtpart, nblk(n), dt_dev);
+    return launch_check("compute_dt");
+}
+
+static grmhd_status ensure_stage_buffers(grmhd_ctx* c) {
+    if (c->U1) return GRMHD_OK;
+    const DevGrid& G = c->G;
+    HIP_TRY(hipMalloc(&c->U1, sizeof(double) * NHRSC * G.ncell * G.nq));
+    HIP_TRY(hipMalloc(&c->U2, sizeof(double) * NHRSC * G.ncell * G.nq));
+    for (int d = 0; d < 3; ++d) {
+        HIP_TRY(hipMalloc(&c->B1[d], sizeof(double) * G.nface[d] * G.nq));
+        HIP_TRY(hipMalloc(&c->B2[d], sizeof(double) * G.nface[d] * G.nq));
+    }
+    return GRMHD_OK;
+}
+
+// The fused RK3 step.  Stage buffers hold only the 7 hydro conserved; the
+// static metric is read from `state` throughout.
+static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, double* const B[3], double* aux,
+                         hipStream_t s) {
+    const DevGrid& G = c->G;
+    k_reset_diag<<<nblk(G.ncell * G.nq), 256, 0, s>>>(G.ncell, G.nq, aux);
+    const double* const Bc[3] = {B[0], B[1], B[2]};
+    const double* const B1c[3] = {c->B1[0], c->B1[1], c->B1[2]};
+    const double* const B2c[3] = {c->B2[0], c->B2[1], c->B2[2]};
+    struct Stage { double* nst; int nnv; const double* ast; int anv; const double* bst; int bnv; double ca, cb, dtf;
+                   double* const* nB; const double* const* aB; const double* const* bB;
+                   const double* ost; int onv; const double* const* oB; };
+    // stage 1: U1 = U0 + dt L(U0)
+    // stage 2: U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L(U1)
+    // stage 3: U0 = 1/3 U0 + 2/3 U2 + 2/3 dt L(U2)
+    double* const nB1[3] = {c->B1[0], c->B1[1], c->B1[2]};
+    double* const nB2[3] = {c->B2[0], c->B2[1], c->B2[2]};
+    Stage st[3] = {
+        {c->U1, NHRSC, state, NSTATE, nullptr, 0, 1.0, 0.0, 1.0, nB1, Bc, nullptr, state, NSTATE, Bc},
+        {c->U2, NHRSC, state, NSTATE, c->U1, NHRSC, 0.75, 0.25, 0.25, nB2, Bc, B1c, c->U1, NHRSC, B1c},
+        {state, NSTATE, state, NSTATE, c->U2, NHRSC, 1. / 3, 2. / 3., 2. / 3., B, Bc, B2c, c->U2, NHRSC, B2c},
+    };
+    for (int sidx = 0; sidx < 3; ++sidx) {
+        Stage& S = st[sidx];
+        launch_fluxes(c, aux, state, NSTATE, S.oB, s);
+        launch_emf(c, S.oB, s);
+        UpdArgs a{};
+        a.nst = S.nst; a.nnv = S.nnv; a.ast = S.ast; a.anv = S.anv; a.bst = S.bst; a.bnv = S.bnv;
+        a.ca = S.ca; a.cb = S.cb; a.mstate = state; a.mnv = NSTATE; a.aux = aux;
+        for (int d = 0; d < 3; ++d) { a.nB[d] = S.nB[d]; a.aB[d] = S.aB[d]; a.bB[d] = S.bB ? S.bB[d] : nullptr; }
+        a.dt = dt_dev; a.dtfact = S.dtf;
+        launch_update(c, a, s);
+        launch_ghost(c, S.nst, S.nnv, 0, NHRSC, S.nB, nullptr, 0, 0, s);
+        const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
+        launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, s);
+    }
+}
+
+extern "C" grmhd_status grmhd_rk3_step(grmhd_ctx* c, const double* dt_dev, double* state, double* const B[3],
+                                       double* aux, grmhd_stream_t stream) {
+    if (!c || !dt_dev || !state || !B || !aux) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    grmhd_status r = ensure_stage_buffers(c);
+    if (r != GRMHD_OK) return r;
+    hipStream_t s = (hipStream_t)stream;
+    {
+        Scope sc(c, "step", s);
+        rk3_launches(c, dt_dev, state, B, aux, s);
+    }
+    return launch_check("rk3_step");
+}
+
+extern "C" grmhd_status grmhd_rk3_step_graph(grmhd_ctx* c, const double* dt_dev, double* state, double* const B[3],
+                                             double* aux, grmhd_stream_t stream) {
+    if (!c || !dt_dev || !state || !B || !aux) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    if (!stream) { set_err("graph capture needs a non-default stream"); return GRMHD_ERR_INVALID; }
+    grmhd_status r = ensure_stage_buffers(c);
+    if (r != GRMHD_OK) return r;
+    hipStream_t s = (hipStream_t)stream;
+    const void* key[8] = {dt_dev, state, B[0], B[1], B[2], aux, nullptr, nullptr};
+    bool same = c->gexec && std::memcmp(key, c->gkey, sizeof(key)) == 0;
+    if (!same) {
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+        bool tim = c->timing;
+        c->timing = false;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rk3_launches(c, dt_dev, state, B, aux, s);
+        hipError_t e = hipStreamEndCapture(s, &c->graph);
+        c->timing = tim;
+        if (e != hipSuccess) { set_err("capture: %s", hipGetErrorString(e)); return GRMHD_ERR_HIP; }
+        HIP_TRY(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        std::memcpy(c->gkey, key, sizeof(key));
+    }
+    Scope sc(c, "step", s);
+    HIP_TRY(hipGraphLaunch(c->gexec, s));
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_set_timing(grmhd_ctx* c, int32_t on) {
+    if (!c) { set_err("null ctx"); return GRMHD_ERR_INVALID; }
+    c->timing = on != 0;
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_kernel_time_ms(grmhd_ctx* c, const char* fam, double* ms_total, int64_t* n_launch) {
+    if (!c || !fam || !ms_total || !n_launch) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    double tot = 0; int64_t n = 0;
+    std::vector<TimerRec> keep;
+    for (auto& t : c->timers) {
+        HIP_TRY(hipEventSynchronize(t.b));
+        if (t.fam == fam) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, t.a, t.b));
+            tot += ms; n++;
+            c->ev_pool.push_back(t.a); c->ev_pool.push_back(t.b);
+        } else keep.push_back(t);
+    }
+    c->timers.swap(keep);
+    *ms_total = tot; *n_launch = n;
+    return GRMHD_OK;
+}

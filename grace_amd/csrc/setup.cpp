@@ -1,0 +1,258 @@
+// setup.cpp -- host-side input generators for the GRMHD workloads.
+//
+// Initial data follow the reference's initial-data path: primitives at every
+// cell incl. ghosts (src/physics/grmhd.cpp:359-527), face B = B * sqrt(gamma)
+// at the staggered positions ("direct" initialisation, :457-490), cell B from
+// the face average and conserved from prims_to_conservs
+// (set_conservs_from_prims, :676-759).  Noise is drawn from a counter-based
+// SplitMix64 keyed by (seed, instance, padded global cell index) so any block
+// order gives bit-identical inputs.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include "phys.hh"
+#include "../../include/grmhd_setup.h"
+
+using namespace gm;
+
+namespace {
+
+inline uint64_t splitmix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// uniform in [0,1) from (seed, a, b, c)
+inline double u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+    uint64_t h = splitmix(seed ^ splitmix(a + 0x632BE59BD9B4E019ull * (b + 1)) ^ splitmix(c * 0x8CB92BA72F3D8DD7ull + 7));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+struct Idx {
+    int N[3]; int64_t ncell, nface[3];
+    Idx(const grmhd_grid* g) {
+        N[0] = g->nx + 2 * g->ngz; N[1] = g->ny + 2 * g->ngz; N[2] = g->nz + 2 * g->ngz;
+        ncell = (int64_t)N[0] * N[1] * N[2];
+        for (int d = 0; d < 3; ++d) nface[d] = (int64_t)(N[0] + (d == 0)) * (N[1] + (d == 1)) * (N[2] + (d == 2));
+    }
+    int64_t c(int i, int j, int k) const { return i + (int64_t)N[0] * (j + (int64_t)N[1] * k); }
+    int64_t f(int d, int i, int j, int k) const {
+        int e0 = N[0] + (d == 0), e1 = N[1] + (d == 1);
+        return i + (int64_t)e0 * (j + (int64_t)e1 * k);
+    }
+};
+
+// Block of instance containing q
+bool find_block(const grmhd_instance* inst, int n_inst, int64_t q, int& a, int b[3]) {
+    for (a = 0; a < n_inst; ++a) {
+        const grmhd_instance& I = inst[a];
+        int64_t nb = (int64_t)I.nb[0] * I.nb[1] * I.nb[2];
+        if (q >= I.q0 && q < I.q0 + nb) {
+            int64_t r = q - I.q0;
+            b[0] = (int)(r % I.nb[0]); b[1] = (int)((r / I.nb[0]) % I.nb[1]); b[2] = (int)(r / ((int64_t)I.nb[0] * I.nb[1]));
+            return true;
+        }
+    }
+    return false;
+}
+
+void flat_metric(double* st, const Idx& X, int64_t q, int64_t c) {
+    const double v[16] = {1, 0, 0, 1, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 16; ++a) st[(q * NSTATE + GXX_ + a) * X.ncell + c] = v[a];
+}
+
+// cell B from faces, conserved from primitives (set_conservs_from_prims)
+void finish_cells(const grmhd_grid* g, const Idx& X, double* st, double* const B[3], double* aux, const double* idx_q,
+                  int64_t q) {
+    for (int k = 0; k < X.N[2]; ++k)
+        for (int j = 0; j < X.N[1]; ++j)
+            for (int i = 0; i < X.N[0]; ++i) {
+                int64_t c = X.c(i, j, k);
+                Met m;
+                for (int a = 0; a < 6; ++a) m.g[a] = st[(q * NSTATE + GXX_ + a) * X.ncell + c];
+                m.alp = st[(q * NSTATE + ALP_) * X.ncell + c];
+                for (int a = 0; a < 3; ++a) m.beta[a] = st[(q * NSTATE + BETAX_ + a) * X.ncell + c];
+                m.init();
+                double bx0 = B[0][q * X.nface[0] + X.f(0, i, j, k)], bx1 = B[0][q * X.nface[0] + X.f(0, i + 1, j, k)];
+                double by0 = B[1][q * X.nface[1] + X.f(1, i, j, k)], by1 = B[1][q * X.nface[1] + X.f(1, i, j + 1, k)];
+                double bz0 = B[2][q * X.nface[2] + X.f(2, i, j, k)], bz1 = B[2][q * X.nface[2] + X.f(2, i, j, k + 1)];
+                auto A = [&](int v) -> double& { return aux[(q * NAUX + v) * X.ncell + c]; };
+                A(BX_) = 0.5 * (bx0 + bx1) / m.sqrtg;
+                A(BY_) = 0.5 * (by0 + by1) / m.sqrtg;
+                A(BZ_) = 0.5 * (bz0 + bz1) / m.sqrtg;
+                A(BDIV_) = ((bx1 - bx0) * idx_q[0] + (by1 - by0) * idx_q[1] + (bz1 - bz0) * idx_q[2]) / m.sqrtg;
+                double pr[NPRIM] = {0};
+                pr[RHOL] = A(RHO_); pr[PRESSL] = A(PRESS_); pr[ZXL] = A(ZVECX_); pr[ZYL] = A(ZVECY_); pr[ZZL] = A(ZVECZ_);
+                pr[YEL] = A(YE_); pr[TEMPL] = A(TEMP_); pr[EPSL] = A(EPS_); pr[ENTL] = A(ENTROPY_);
+                pr[BXL] = A(BX_); pr[BYL] = A(BY_); pr[BZL] = A(BZ_);
+                double cons[NCONS];
+                p2c(pr, cons, m);
+                for (int v = 0; v < NHRSC; ++v) st[(q * NSTATE + v) * X.ncell + c] = cons[v];
+            }
+    (void)g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grmhd_setup_blast(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
+                      const double lo[3], const double hi[3], double rho_amb, double p_amb, double p_in, double r_in,
+                      const double Bvec[3], double noise, uint64_t seed, double* state, double* Bx, double* By,
+                      double* Bz, double* aux) {
+    Idx X(g);
+    double* B[3] = {Bx, By, Bz};
+    std::memset(aux, 0, sizeof(double) * NAUX * X.ncell * g->nq);
+    const double gm1 = gamma - 1.0;
+    int n[3] = {g->nx, g->ny, g->nz};
+    for (int64_t q = 0; q < g->nq; ++q) {
+        int a, b[3];
+        if (!find_block(inst, n_inst, q, a, b)) return 1;
+        const grmhd_instance& I = inst[a];
+        double dx[3];
+        int Ntot[3];
+        for (int d = 0; d < 3; ++d) { dx[d] = 1.0 / I.idx[d]; Ntot[d] = I.nb[d] * n[d]; }
+        for (int k = 0; k < X.N[2]; ++k)
+            for (int j = 0; j < X.N[1]; ++j)
+                for (int i = 0; i < X.N[0]; ++i) {
+                    int loc[3] = {i, j, k};
+                    int gl[3];
+                    double x[3];
+                    for (int d = 0; d < 3; ++d) {
+                        gl[d] = b[d] * n[d] + loc[d] - g->ngz;
+                        x[d] = lo[d] + (gl[d] + 0.5) * dx[d];
+                    }
+                    int64_t key = (int64_t)(gl[0] + g->ngz) + (int64_t)(Ntot[0] + 2 * g->ngz) *
+                                  ((gl[1] + g->ngz) + (int64_t)(Ntot[1] + 2 * g->ngz) * (gl[2] + g->ngz));
+                    double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                    double rho = rho_amb, p = r < r_in ? p_in : p_amb;
+                    rho *= 1.0 + noise * (2.0 * u01(seed, a, (uint64_t)key, 0) - 1.0);
+                    p *= 1.0 + noise * (2.0 * u01(seed, a, (uint64_t)key, 1) - 1.0);
+                    int64_t c = X.c(i, j, k);
+                    flat_metric(state, X, q, c);
+                    auto A = [&](int v) -> double& { return aux[(q * NAUX + v) * X.ncell + c]; };
+                    double eps = p / (gm1 * rho);
+                    A(RHO_) = rho; A(PRESS_) = p; A(EPS_) = eps; A(TEMP_) = eps * gm1;
+                    A(ENTROPY_) = std::log(eps) / gm1 - std::log(rho);
+                    A(ZVECX_) = A(ZVECY_) = A(ZVECZ_) = 0.0; A(YE_) = 0.0;
+                }
+        for (int d = 0; d < 3; ++d)
+            for (int64_t f = 0; f < X.nface[d]; ++f) B[d][q * X.nface[d] + f] = Bvec[d];
+        finish_cells(g, X, state, B, aux, I.idx, q);
+        (void)hi;
+    }
+    return 0;
+}
+
+int grmhd_setup_shocktube(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma, double x0,
+                          double x_jump, const double L[5], const double R[5], double* state, double* Bx, double* By,
+                          double* Bz, double* aux) {
+    // L/R = {rho, p, Bx, By, Bz}; discontinuity along x at x_jump (x <= x_jump is left)
+    Idx X(g);
+    double* B[3] = {Bx, By, Bz};
+    std::memset(aux, 0, sizeof(double) * NAUX * X.ncell * g->nq);
+    const double gm1 = gamma - 1.0;
+    for (int64_t q = 0; q < g->nq; ++q) {
+        int a, b[3];
+        if (!find_block(inst, n_inst, q, a, b)) return 1;
+        const grmhd_instance& I = inst[a];
+        double dx = 1.0 / I.idx[0];
+        for (int k = 0; k < X.N[2]; ++k)
+            for (int j = 0; j < X.N[1]; ++j)
+                for (int i = 0; i < X.N[0]; ++i) {
+                    int gl = b[0] * g->nx + i - g->ngz;
+                    double x = x0 + (gl + 0.5) * dx;
+                    const double* S = x <= x_jump ? L : R;
+                    int64_t c = X.c(i, j, k);
+                    flat_metric(state, X, q, c);
+                    auto A = [&](int v) -> double& { return aux[(q * NAUX + v) * X.ncell + c]; };
+                    double eps = S[1] / (gm1 * S[0]);
+                    A(RHO_) = S[0]; A(PRESS_) = S[1]; A(EPS_) = eps; A(TEMP_) = eps * gm1;
+                    A(ENTROPY_) = std::log(eps) / gm1 - std::log(S[0]);
+                }
+        // staggered B: face coordinate along x is the lower face of cell i
+        for (int d = 0; d < 3; ++d) {
+            int e0 = X.N[0] + (d == 0), e1 = X.N[1] + (d == 1), e2 = X.N[2] + (d == 2);
+            for (int k = 0; k < e2; ++k)
+                for (int j = 0; j < e1; ++j)
+                    for (int i = 0; i < e0; ++i) {
+                        int gl = b[0] * g->nx + i - g->ngz;
+                        double x = d == 0 ? x0 + gl * dx : x0 + (gl + 0.5) * dx;
+                        const double* S = x <= x_jump ? L : R;
+                        B[d][q * X.nface[d] + X.f(d, i, j, k)] = S[2 + d];
+                    }
+        }
+        finish_cells(g, X, state, B, aux, I.idx, q);
+    }
+    return 0;
+}
+
+// Kerr-Schild ADM quantities at Cartesian (x,y,z) for mass 1, spin a.
+void grmhd_kerr_schild(double a, double x, double y, double z, double g[6], double* alp, double beta[3]) {
+    double R2 = x * x + y * y + z * z;
+    double a2 = a * a;
+    double r2 = 0.5 * (R2 - a2 + std::sqrt((R2 - a2) * (R2 - a2) + 4.0 * a2 * z * z));
+    double r = std::sqrt(r2);
+    double H = r2 * r / (r2 * r2 + a2 * z * z);
+    double l[3] = {(r * x + a * y) / (r2 + a2), (r * y - a * x) / (r2 + a2), z / r};
+    const int sidx[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int s = 0; s < 6; ++s) g[s] = (sidx[s][0] == sidx[s][1] ? 1.0 : 0.0) + 2.0 * H * l[sidx[s][0]] * l[sidx[s][1]];
+    *alp = 1.0 / std::sqrt(1.0 + 2.0 * H);
+    for (int i = 0; i < 3; ++i) beta[i] = 2.0 * H * l[i] / (1.0 + 2.0 * H);
+}
+
+// Config-4 c2p states: rho log-U[1e-12,1e-2], eps log-U[1e-6,1], W U[1,10]
+// isotropic, b^2/rho log-U[1e-4,1e2], Kerr-Schild spin a at r U[1.5,50]
+// with isotropic angles; ideal gas.  Outputs SoA prims (13n), metric (10n),
+// densitised conserved (10n) and radius (n).
+int grmhd_setup_c2p_states(int64_t n, uint64_t seed, double gamma, double spin, const double rho_rng[2],
+                           const double eps_rng[2], const double W_rng[2], const double sig_rng[2],
+                           const double r_rng[2], double* prims, double* met, double* cons, double* rad) {
+    const double gm1 = gamma - 1.0;
+    for (int64_t s = 0; s < n; ++s) {
+        auto U = [&](int k) { return u01(seed, 0x4C32, (uint64_t)s, (uint64_t)k); };
+        double rho = std::exp(std::log(rho_rng[0]) + (std::log(rho_rng[1]) - std::log(rho_rng[0])) * U(0));
+        double eps = std::exp(std::log(eps_rng[0]) + (std::log(eps_rng[1]) - std::log(eps_rng[0])) * U(1));
+        double W = W_rng[0] + (W_rng[1] - W_rng[0]) * U(2);
+        double sig = std::exp(std::log(sig_rng[0]) + (std::log(sig_rng[1]) - std::log(sig_rng[0])) * U(3));
+        double r = r_rng[0] + (r_rng[1] - r_rng[0]) * U(4);
+        double ct = 2.0 * U(5) - 1.0, ph = 2.0 * M_PI * U(6);
+        double st = std::sqrt(std::fmax(0.0, 1.0 - ct * ct));
+        double rr = std::sqrt(r * r + spin * spin);
+        double x = rr * st * std::cos(ph), y = rr * st * std::sin(ph), z = r * ct;
+        Met m;
+        grmhd_kerr_schild(spin, x, y, z, m.g, &m.alp, m.beta);
+        m.init();
+        // velocity direction (isotropic) scaled to gamma_ij z^i z^j = W^2 - 1
+        double cv = 2.0 * U(7) - 1.0, pv = 2.0 * M_PI * U(8), sv = std::sqrt(std::fmax(0.0, 1 - cv * cv));
+        double nv[3] = {sv * std::cos(pv), sv * std::sin(pv), cv};
+        double zn = std::sqrt(m.sqv(nv[0], nv[1], nv[2]));
+        double zm = std::sqrt(W * W - 1.0) / zn;
+        double zv[3] = {nv[0] * zm, nv[1] * zm, nv[2] * zm};
+        // B direction (isotropic), magnitude from b^2 = sigma rho
+        double cb = 2.0 * U(9) - 1.0, pb = 2.0 * M_PI * U(10), sb = std::sqrt(std::fmax(0.0, 1 - cb * cb));
+        double nb[3] = {sb * std::cos(pb), sb * std::sin(pb), cb};
+        double Wc = lorentz(m.g, zv);
+        double bb[4], b2u;
+        smallb(m, nb, zv, Wc, bb, b2u);
+        double Bm = std::sqrt(sig * rho / b2u);
+        double pr[NPRIM] = {0};
+        pr[RHOL] = rho; pr[EPSL] = eps; pr[PRESSL] = gm1 * rho * eps; pr[TEMPL] = gm1 * eps;
+        pr[ENTL] = std::log(eps) / gm1 - std::log(rho); pr[YEL] = 0.0;
+        for (int i = 0; i < 3; ++i) { pr[ZXL + i] = zv[i]; pr[BXL + i] = Bm * nb[i]; }
+        double cs[NCONS];
+        p2c(pr, cs, m);
+        for (int i = 0; i < 3; ++i) cs[BSXL + i] = m.sqrtg * pr[BXL + i];
+        for (int v = 0; v < NPRIM; ++v) prims[v * n + s] = pr[v];
+        for (int v = 0; v < NCONS; ++v) cons[v * n + s] = cs[v];
+        for (int a2 = 0; a2 < 6; ++a2) met[a2 * n + s] = m.g[a2];
+        met[6 * n + s] = m.alp;
+        for (int a2 = 0; a2 < 3; ++a2) met[(7 + a2) * n + s] = m.beta[a2];
+        if (rad) rad[s] = r;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+"""Synthetic inputs of the benchmark configurations, built on host numpy arrays.
+
+Every generator fills arrays in the library's layout (see include/grmhd_abi.h)
+through the library's own host-side initial-data routines, so the GPU and any
+CPU checker receive bit-identical inputs.  Seed 20261202 unless stated.
+"""
+import numpy as np
+
+from . import (lib, Grid, make_instances, default_params, NSTATE, NAUX, BC_OUTFLOW, BC_PERIODIC,
+               EMF_GS, EMF_UCT)
+
+SEED = 20261202
+
+
+class Workload:
+    """Grid + instances + host arrays (state, B[3], aux) + parameters."""
+
+    def __init__(self, name, params, grid, inst_specs, state, B, aux, zones, extra=None):
+        self.name = name
+        self.params = params
+        self.grid = grid
+        self.inst_specs = inst_specs
+        self.instances = make_instances(inst_specs)
+        self.state = state
+        self.B = B
+        self.aux = aux
+        self.zones = zones
+        self.extra = extra or {}
+
+    def copy_arrays(self):
+        return self.state.copy(), [b.copy() for b in self.B], self.aux.copy()
+
+
+def alloc(grid):
+    Ni, Nj, Nk = grid.nx + 2 * grid.ngz, grid.ny + 2 * grid.ngz, grid.nz + 2 * grid.ngz
+    nq = grid.nq
+    state = np.zeros((nq, NSTATE, Nk, Nj, Ni))
+    aux = np.zeros((nq, NAUX, Nk, Nj, Ni))
+    B = [np.zeros((nq, 1, Nk, Nj, Ni + 1)), np.zeros((nq, 1, Nk, Nj + 1, Ni)), np.zeros((nq, 1, Nk + 1, Nj, Ni))]
+    return state, B, aux
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def blast(nb=(8, 8, 8), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01, emf=EMF_UCT, recon=0, riemann=0,
+          flat=True):
+    """Magnetised blast wave: 128^3 on [-0.5,0.5]^3 in 16^3 blocks by default.
+    Ambient rho=1e-2, p=1e-3; p=1.0 for r<0.1; B=(0.1,0,0); +-1% noise."""
+    nq = nb[0] * nb[1] * nb[2]
+    grid = Grid(n, n, n, ngz, nq)
+    N = [nb[d] * n for d in range(3)]
+    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(float(N[d]) for d in range(3)))]
+    inst = make_instances(spec)
+    state, B, aux = alloc(grid)
+    lo = np.array([-0.5, -0.5, -0.5])
+    hi = np.array([0.5, 0.5, 0.5])
+    Bv = np.array([0.1, 0.0, 0.0])
+    rc = lib().grmhd_setup_blast(grid, inst, 1, gamma, _p(lo), _p(hi), 1e-2, 1e-3, 1.0, 0.1, _p(Bv), noise, seed,
+                                 _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux))
+    assert rc == 0
+    params = default_params(gamma=gamma, emf=emf, recon=recon, riemann=riemann, flat_metric=1 if flat else 0)
+    return Workload("blast", params, grid, spec, state, B, aux, zones=nq * n ** 3,
+                    extra=dict(N=N, dx=1.0 / N[0]))
+
+
+def multi_blast(n_inst=64, sizes=(32, 48, 64, 80, 96), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01):
+    """Uneven instances: each a cubic blast-wave grid of side drawn from `sizes`,
+    cut into n^3 blocks; all instances share one block list."""
+    rng = np.random.default_rng(seed)
+    sides = rng.choice(np.array(sizes), size=n_inst)
+    specs, q0 = [], 0
+    for s in sides:
+        b = int(s) // n
+        specs.append(dict(nb=(b, b, b), q0=q0, bc=(BC_OUTFLOW,) * 6, idx=(float(s),) * 3))
+        q0 += b ** 3
+    grid = Grid(n, n, n, ngz, q0)
+    inst = make_instances(specs)
+    state, B, aux = alloc(grid)
+    lo = np.array([-0.5, -0.5, -0.5])
+    Bv = np.array([0.1, 0.0, 0.0])
+    rc = lib().grmhd_setup_blast(grid, inst, len(specs), gamma, _p(lo), _p(-lo), 1e-2, 1e-3, 1.0, 0.1, _p(Bv), noise,
+                                 seed, _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux))
+    assert rc == 0
+    params = default_params(gamma=gamma, flat_metric=1)
+    zones = int(sum(int(s) ** 3 for s in sides))
+    return Workload("multi_blast", params, grid, specs, state, B, aux, zones=zones, extra=dict(sides=sides))
+
+
+def balsara1(N=400, ngz=3, gamma=2.0):
+    """Balsara-1 relativistic MHD shock tube: N x 1 x 1 on x in [0,1], jump at x=0.5.
+    L: rho=1, p=1, B=(0.5,1,0); R: rho=0.125, p=0.1, B=(0.5,-1,0).  GS CT, WENO-Z, HLLE."""
+    grid = Grid(N, 1, 1, ngz, 1)
+    dx = 1.0 / N
+    spec = [dict(nb=(1, 1, 1), q0=0, bc=(BC_OUTFLOW, BC_OUTFLOW, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC, BC_PERIODIC),
+                 idx=(1.0 / dx,) * 3)]
+    inst = make_instances(spec)
+    state, B, aux = alloc(grid)
+    L = np.array([1.0, 1.0, 0.5, 1.0, 0.0])
+    R = np.array([0.125, 0.1, 0.5, -1.0, 0.0])
+    rc = lib().grmhd_setup_shocktube(grid, inst, 1, gamma, 0.0, 0.5, _p(L), _p(R), _p(state), _p(B[0]), _p(B[1]),
+                                     _p(B[2]), _p(aux))
+    assert rc == 0
+    params = default_params(gamma=gamma, emf=EMF_GS, flat_metric=1)
+    return Workload("balsara1", params, grid, spec, state, B, aux, zones=N, extra=dict(dx=dx))
+
+
+def c2p_states(n, seed=SEED, gamma=4.0 / 3.0, spin=0.9, rho=(1e-12, 1e-2), eps=(1e-6, 1.0), W=(1.0, 10.0),
+               sigma=(1e-4, 1e2), r=(1.5, 50.0)):
+    """Seeded c2p states in Kerr-Schild (spin a) at r in [1.5,50] M (SoA arrays)."""
+    prims = np.zeros((13, n))
+    met = np.zeros((10, n))
+    cons = np.zeros((10, n))
+    rad = np.zeros(n)
+    rr = [np.array(x, dtype=np.float64) for x in (rho, eps, W, sigma, r)]
+    rc = lib().grmhd_setup_c2p_states(n, seed, gamma, spin, *[_p(x) for x in rr], _p(prims), _p(met), _p(cons), _p(rad))
+    assert rc == 0
+    return dict(prims=prims, met=met, cons=cons, rad=rad, params=default_params(gamma=gamma))

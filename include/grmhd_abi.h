@@ -1,0 +1,180 @@
+/*
+ * grmhd_abi.h -- C ABI of the MI355X GRMHD RHS + con2prim library
+ * (libgrmhd_mi355x.so).
+ *
+ * The reference (GRACE) has no FFI layer: its hot path is a set of C++
+ * templates over the EOS type.  Each entry point below replaces one of those
+ * calls (reference file:line given per function) and keeps its argument
+ * meaning, its data layout and its error behaviour, with plain pointers and
+ * sizes instead of Kokkos views.
+ *
+ * Data layout (identical to the reference's Kokkos LayoutLeft views,
+ * include/grace/data_structures/variable_properties.hh:170-305):
+ *   cell arrays   a[q][v][k][j][i]  i fastest, extents Ni=nx+2ngz, Nj, Nk
+ *   face arrays   Bx[q][k][j][i] with extent Ni+1 along x (By: Nj+1, Bz: Nk+1)
+ *   state vars    0..6  DENS SX SY SZ TAU YESTAR ENTROPYSTAR (densitised)
+ *                 7..22 GXX GXY GXZ GYY GYZ GZZ ALP BETAX BETAY BETAZ
+ *                       KXX KXY KXZ KYY KYZ KZZ   (Cowling metric)
+ *   aux vars      RHO ZVECX ZVECY ZVECZ BX BY BZ YE TEMP ENTROPY EPS PRESS
+ *                 BDIV C2P_DENS_ERR C2P_ERR  (variable_indices.hh:162-331)
+ * All device pointers are caller-owned hipMalloc memory; the library never
+ * frees them.  Every call is asynchronous on the given HIP stream and returns
+ * a status code; nothing aborts across the ABI.  Per-cell c2p outcomes are
+ * bit sets identical to the reference's c2p_err_enum_t
+ * (include/grace/physics/eos/c2p.hh:95-125).
+ */
+#ifndef GRMHD_ABI_H
+#define GRMHD_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* grmhd_stream_t; /* hipStream_t, 0 = default stream */
+typedef struct grmhd_ctx grmhd_ctx;
+
+typedef enum {
+    GRMHD_OK = 0,
+    GRMHD_ERR_INVALID = 1,      /* bad argument / size / layout */
+    GRMHD_ERR_HIP = 2,          /* HIP runtime error (launch, memory) */
+    GRMHD_ERR_ALLOC = 3,        /* device allocation failed */
+    GRMHD_ERR_UNSUPPORTED = 4,  /* option combination not built */
+    GRMHD_ERR_C2P_FATAL = 5     /* a cell hit a branch where the reference aborts */
+} grmhd_status;
+
+enum { GRMHD_EOS_IDEAL_GAS = 0, GRMHD_EOS_HYBRID_PWPOLY = 1 };
+enum { GRMHD_RECON_WENOZ = 0, GRMHD_RECON_MC2 = 1, GRMHD_RECON_MINMOD = 2, GRMHD_RECON_DONOR = 3 };
+enum { GRMHD_RIEMANN_HLLE = 0, GRMHD_RIEMANN_LLF = 1 };
+enum { GRMHD_EMF_UCT = 0, GRMHD_EMF_GS = 1 };
+enum { GRMHD_THERMO_TEMP = 0, GRMHD_THERMO_PRESS = 1 };
+enum { GRMHD_BC_OUTFLOW = 0, GRMHD_BC_PERIODIC = 1 };
+
+/* Runtime + compile-time options of the reference's hot path.  Defaults
+ * (grmhd_default_params) are the reference's parameter-schema defaults,
+ * parameters/{grmhd,eos}.yaml; reference reader: grmhd_helpers.hh:165-292. */
+typedef struct {
+    /* EOS (src/physics/eos/eos_storage.cpp:160-220) */
+    int32_t eos_kind;
+    double gamma;             /* ideal gas Gamma */
+    double eps_max;           /* eos.eps_maximum */
+    double entropy_min;       /* eos.entropy_minimum (hybrid) */
+    double gamma_th;          /* hybrid thermal Gamma */
+    int32_t pwp_n;            /* hybrid: number of polytropic pieces (<= 8) */
+    double pwp_kappa0;
+    double pwp_gamma[8];
+    double pwp_rho[8];        /* n-1 dividing densities */
+    /* atmosphere (grmhd_helpers.hh:41-48) */
+    double rho_fl, temp_fl, ye_fl, atmo_tol;
+    /* c2p (grmhd_helpers.hh:52-59) */
+    double c2p_tol, max_lorentz, max_sigma, beta_fallback;
+    int32_t use_entropy_backup;
+    double bh_alp_thresh;
+    /* excision (grmhd_helpers.hh:89-96) */
+    int32_t excise_by_radius;
+    double excision_radius, excision_lapse;
+    /* numerical scheme (CMakeLists.txt:206-256, cmake/code_modules.cmake:14-40) */
+    int32_t recon, riemann, emf, thermo_recon, fd_order;
+    double rusanov_c_limit;
+    /* 1: the metric is Minkowski everywhere; kernels skip the 16 metric
+     * arrays and the source terms (identically zero there). */
+    int32_t flat_metric;
+} grmhd_params;
+
+/* Block grid: every block has the same interior extents. */
+typedef struct { int32_t nx, ny, nz, ngz; int64_t nq; } grmhd_grid;
+
+/* A uniform Cartesian instance cut into nb[0]*nb[1]*nb[2] blocks, stored at
+ * block indices q = q0 + bi + nb0*(bj + nb1*bk).  bc[] per face
+ * (-x,+x,-y,+y,-z,+z).  idx = inverse cell spacings. */
+typedef struct { int32_t nb[3]; int64_t q0; int32_t bc[6]; double idx[3]; } grmhd_instance;
+
+/* Per-launch c2p statistics (reduced on device, read with grmhd_c2p_stats). */
+typedef struct {
+    int64_t n_cells, n_fatal, n_atmo_reset, n_ent_backup, n_guess_used;
+    int64_t n_root_evals, n_bracket_evals, max_root_evals;
+} grmhd_c2p_stats_t;
+
+void grmhd_default_params(grmhd_params* p);
+
+/* Context: validated parameters + grid + device scratch (fluxes, vbar/Eface,
+ * edge EMFs, E_center, RK stage buffers).  Replaces the parameter readers and
+ * the variable_list singleton of the reference. */
+grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid* grid,
+                              const grmhd_instance* inst, int32_t n_inst, grmhd_ctx** out);
+void grmhd_ctx_destroy(grmhd_ctx* ctx);
+const char* grmhd_last_error(void);
+
+/* advance_substep<eos_t> (include/grace/evolution/evolve.hh:252-256,
+ * src/evolution/evolve.cpp:1671-1717): new_state += dt*dtfact*L(old_state)
+ * on interior cells, new_B += dt*dtfact*curl(E) on interior faces.  aux holds
+ * the primitives of old_state.  dt is read from device memory (dt_dev). */
+grmhd_status grmhd_advance_substep(grmhd_ctx* ctx, const double* dt_dev, double dtfact,
+                                   double* new_state, const double* old_state,
+                                   double* const new_B[3], const double* const old_B[3],
+                                   const double* aux, grmhd_stream_t stream);
+
+/* amr::apply_boundary_conditions (src/amr/boundary_conditions.cpp:55-81):
+ * same-level ghost copy between blocks + outflow/periodic physical BC, for the
+ * 7 hydro conserved variables and the 3 face-B arrays. */
+grmhd_status grmhd_fill_ghosts(grmhd_ctx* ctx, double* state, double* const B[3],
+                               grmhd_stream_t stream);
+
+/* compute_auxiliary_quantities<eos_t> (include/grace/evolution/auxiliaries.hh:49-54,
+ * src/evolution/auxiliaries.cpp:81-180): c2p on every cell incl. ghosts,
+ * conserved rewrite on reset bits, sticky-OR of C2P_ERR, BDIV, and (GS)
+ * the cell-centred E field kept in the context. */
+grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* ctx, double* state, const double* const B[3],
+                                       double* aux, grmhd_stream_t stream);
+
+/* conservs_to_prims<eos_t> over a flat batch (include/grace/physics/eos/c2p.hh:342-353).
+ * cons_soa[10*n]: DENS S_x S_y S_z TAU YE* S* B^x B^y B^z (densitised, SoA
+ *   blocks of n); metric_soa[10*n]: gxx gxy gxz gyy gyz gzz alp betax betay betaz.
+ * Outputs: prims_soa[13*n] (RHO PRESS ZX ZY ZZ YE TEMP EPS ENT BX BY BZ unused),
+ *   cons_soa overwritten with the re-densitised conserved, err_bits[n]
+ *   (c2p_err_t words), floored[n] (bool as uint8).  r_or_null: radii for
+ *   radius-based excision.  guess_or_null: per-state [mu_lo, mu_hi] bracket
+ *   (2*n, SoA) tried before the default [0, mu0] bracket.
+ * Returns GRMHD_OK; per-state fatal branches are reported in stats. */
+grmhd_status grmhd_c2p_batch(grmhd_ctx* ctx, int64_t n, double* cons_soa, const double* metric_soa,
+                             double* prims_soa, uint64_t* err_bits, uint8_t* floored, int32_t dry_run,
+                             const double* r_or_null, const double* guess_or_null,
+                             grmhd_stream_t stream);
+
+/* prims_to_conservs (src/physics/eos/c2p.cpp:416-455) over a flat batch. */
+grmhd_status grmhd_prims_to_conservs_batch(grmhd_ctx* ctx, int64_t n, const double* prims_soa,
+                                           const double* metric_soa, double* cons_soa,
+                                           grmhd_stream_t stream);
+
+/* Statistics of the last c2p launch (host copy; synchronises the stream). */
+grmhd_status grmhd_c2p_stats(grmhd_ctx* ctx, grmhd_c2p_stats_t* out, grmhd_stream_t stream);
+
+/* find_stable_timestep (src/evolution/find_stable_timestep.cpp:72-126):
+ * dt = min over interior cells of cfl*dx/c_max, written to dt_dev. */
+grmhd_status grmhd_compute_dt(grmhd_ctx* ctx, const double* state, const double* aux, double cfl,
+                              double* dt_dev, grmhd_stream_t stream);
+
+/* One full SSP-RK3 step (evolve_impl, src/evolution/evolve.cpp:147-225):
+ * reset of the per-step c2p diagnostics, three substeps each followed by the
+ * ghost fill and the c2p pass.  state/B/aux are updated in place; the two
+ * stage buffers live in the context.  The kernels of the step are fused
+ * (linear combination + sources + flux divergence + CT in one pass). */
+grmhd_status grmhd_rk3_step(grmhd_ctx* ctx, const double* dt_dev, double* state, double* const B[3],
+                            double* aux, grmhd_stream_t stream);
+
+/* Same step captured once into a hipGraph and replayed (pointers are bound
+ * at the first call; a call with different pointers re-captures). */
+grmhd_status grmhd_rk3_step_graph(grmhd_ctx* ctx, const double* dt_dev, double* state,
+                                  double* const B[3], double* aux, grmhd_stream_t stream);
+
+/* Duration in milliseconds of the last launch of the named kernel family,
+ * measured with HIP events on the launch stream ("flux", "emf", "update",
+ * "ghost", "c2p", "step"); enable with grmhd_set_timing(ctx, 1). */
+grmhd_status grmhd_set_timing(grmhd_ctx* ctx, int32_t on);
+grmhd_status grmhd_kernel_time_ms(grmhd_ctx* ctx, const char* family, double* ms_total, int64_t* n_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRMHD_ABI_H */

@@ -1,0 +1,41 @@
+/*
+ * grmhd_setup.h -- host-side input generators exported by libgrmhd_mi355x.so.
+ *
+ * Not part of the reference's hot-path interface: these build the synthetic
+ * inputs of the benchmark configurations (blast wave, shock tube, Kerr-Schild
+ * c2p states) on host arrays laid out as in grmhd_abi.h.  They follow the
+ * reference's initial-data path (src/physics/grmhd.cpp:359-527, 676-759) and
+ * its shock-tube data (include/grace/physics/id/shocktube.hh:46-100).
+ */
+#ifndef GRMHD_SETUP_H
+#define GRMHD_SETUP_H
+#include <stdint.h>
+#include "grmhd_abi.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Magnetised blast wave on every block of every instance (cells incl. ghosts).
+ * lo: lower domain corner; p = p_in for r < r_in, rho/p times (1 + noise*U[-1,1]). */
+int grmhd_setup_blast(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
+                      const double lo[3], const double hi[3], double rho_amb, double p_amb, double p_in,
+                      double r_in, const double Bvec[3], double noise, uint64_t seed, double* state,
+                      double* Bx, double* By, double* Bz, double* aux);
+
+/* Quasi-1D shock tube along x: L/R = {rho, p, Bx, By, Bz}, x <= x_jump is left. */
+int grmhd_setup_shocktube(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
+                          double x0, double x_jump, const double L[5], const double R[5], double* state,
+                          double* Bx, double* By, double* Bz, double* aux);
+
+/* Kerr-Schild (M=1, spin a) lapse, shift and 3-metric at a Cartesian point. */
+void grmhd_kerr_schild(double a, double x, double y, double z, double g[6], double* alp, double beta[3]);
+
+/* Seeded c2p test states (SoA outputs, see grmhd_abi.h for the layouts). */
+int grmhd_setup_c2p_states(int64_t n, uint64_t seed, double gamma, double spin, const double rho_rng[2],
+                           const double eps_rng[2], const double W_rng[2], const double sig_rng[2],
+                           const double r_rng[2], double* prims, double* met, double* cons, double* rad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
