@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""GRMHD zone-updates/s on one MI355X.
+
+Workload (default): 3-D magnetised blast wave, 128^3 cells in 512 blocks of
+16^3 with 3 ghost layers, ideal gas Gamma=4/3, WENO-Z + HLLE + UCT, FP64.
+A "step" is one full SSP-RK3 step of the whole block list: device CFL dt
+(cfl 0.25) + 3 x (fluxes, UCT EMFs, fused update + CT, ghost gather, c2p over
+all cells), replayed from a hipGraph.  value = interior zones x steps / time.
+
+Prints one JSON line.  Extra objects:
+  roofline      dominant kernel (largest measured time) vs the HBM peak, with
+                its measured FP64 instruction throughput when a counter
+                summary is present under profiles/
+  cpu_baseline  the CPU oracle (OpenMP, contracted FMAs) timing one RK3 step of
+                the same 128^3 workload on the host cores
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic bytes per interior zone for one launch of each kernel family
+# (flat-metric variant; compulsory unique traffic of the algorithm's data that
+# the kernel must move, counted once per zone).
+#  flux_d : reads 10 reconstructed primitives + the face-normal B (11 doubles),
+#           writes 7 fluxes + 4 UCT face values (11 doubles)            -> 176 B
+#  emf    : reads 2 face-B + 4 face values x 2 directions per edge family
+#           (3 edge comps x (2 B + 2 vbar + 4 speeds)), writes 3 EMFs   -> 216 B
+#  update : reads 7 cons x 2 levels + 21 fluxes + 3 EMF + 3 B x 2 levels,
+#           writes 7 cons + 3 B                                        -> 456 B
+#  c2p    : reads 7 cons + 3 face B + C2P_ERR, writes 12 prims + BDIV,
+#           C2P_DENS_ERR, C2P_ERR and rewrites <= 7 cons                -> 264 B
+ALG_BYTES = {"flux_x": 176, "flux_y": 176, "flux_z": 176, "emf": 216, "update": 456, "c2p": 264,
+             "ghost": 0}
+# Whole-step algorithmic bytes per zone-update (flat metric; RHS 3x(38..48
+# reads + 10 writes) + c2p 3x) -- 1552 B.
+STEP_ALG_BYTES = 1552
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_profile_counters(kernel_tag):
+    """FP64 VALU instruction counts and HBM traffic per launch from committed
+    rocprofv3 --pmc summaries (profiles/<kernel_tag>_pmc.md), if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.md")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("|") and "=" in line:
+                pass
+            if line.startswith("- ") and ":" in line:
+                k, v = line[2:].split(":", 1)
+                try:
+                    out[k.strip()] = float(v.split()[0])
+                except ValueError:
+                    pass
+    return out
+
+
+def cpu_baseline(wl, steps=1, threads=None):
+    """Times the oracle (liboracle_fast.so) on one RK3 step of the workload."""
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    P, d = C.c_void_p, C.c_double
+    L.orc_num_threads.argtypes = [C.c_int]
+    L.orc_num_threads.restype = C.c_int
+    L.orc_stepper_create.argtypes = [P, P, P, C.c_int]
+    L.orc_stepper_create.restype = P
+    L.orc_stepper_destroy.argtypes = [P]
+    L.orc_rk3_step.argtypes = [P, d, P, P, P, P, P]
+    L.orc_find_dt.argtypes = [P, P, P, d]
+    L.orc_find_dt.restype = d
+    nthr = threads or min(16, os.cpu_count() or 1)
+    nthr = L.orc_num_threads(nthr)
+    st, B, aux = wl.copy_arrays()
+    h = L.orc_stepper_create(C.byref(wl.params), C.byref(wl.grid), wl.instances, len(wl.inst_specs))
+    p = lambda a: C.c_void_p(a.ctypes.data)
+    # warm-up on the same arrays (first touch of the scratch)
+    t0 = time.perf_counter()
+    dt = L.orc_find_dt(h, p(st), p(aux), 0.25)
+    L.orc_rk3_step(h, dt, p(st), p(B[0]), p(B[1]), p(B[2]), p(aux))
+    t_warm = time.perf_counter() - t0
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        dt = L.orc_find_dt(h, p(st), p(aux), 0.25)
+        L.orc_rk3_step(h, dt, p(st), p(B[0]), p(B[1]), p(B[2]), p(aux))
+        times.append(time.perf_counter() - t0)
+    L.orc_stepper_destroy(h)
+    t = float(np.median(times))
+    return {"value": wl.zones / t, "unit": "zone-updates/s", "cores": nthr, "kind": "port",
+            "sample": f"{steps} timed RK3 step(s) (+1 warm-up, {t_warm:.1f}s) of the full {wl.name} workload "
+                      f"({wl.zones} zones), C++ oracle built -O3 -fopenmp, {nthr} threads, median step "
+                      f"{t:.2f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU halo exchange is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import grace_amd as ga
+    from grace_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    log("[bench] building 128^3 blast wave (512 blocks of 16^3)")
+    wl = W.blast(nb=(8, 8, 8), n=16)
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    st = torch.from_numpy(wl.state).to(dev)
+    B = [torch.from_numpy(b).to(dev) for b in wl.B]
+    aux = torch.from_numpy(wl.aux).to(dev)
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    use_graph = not args.no_graph
+
+    def step():
+        ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
+        ctx.rk3_step(dt, st, B, aux, stream=sp, graph=use_graph)
+
+    with torch.cuda.stream(stream):
+        log(f"[bench] warmup {args.warmup}")
+        for _ in range(args.warmup):
+            step()
+        stream.synchronize()
+        torch.cuda.synchronize()
+        log(f"[bench] timing {args.steps} steps")
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = 1000.0 * wall / args.steps
+    value = wl.zones * args.steps / wall
+
+    # per-kernel durations: one more eager step with HIP events around each launch
+    ctx.set_timing(1)
+    with torch.cuda.stream(stream):
+        for _ in range(2):
+            ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
+            ctx.rk3_step(dt, st, B, aux, stream=sp, graph=False)
+        stream.synchronize()
+    fam = {}
+    for f in ("flux_x", "flux_y", "flux_z", "emf", "update", "ghost", "c2p"):
+        ms, n = ctx.kernel_time_ms(f)
+        if n:
+            fam[f] = (ms / n, n)
+    ctx.set_timing(0)
+    dom = max((k for k in fam if k in ALG_BYTES), key=lambda k: fam[k][0] * fam[k][1])
+    avg_ms = fam[dom][0]
+    alg = ALG_BYTES[dom] * wl.zones
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    prof = read_profile_counters(f"{dom}_blast128")
+    traffic = prof.get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_zone": ALG_BYTES[dom],
+            "kernel_ms_per_step": {k: round(v[0] * v[1] / 2, 4) for k, v in fam.items()},
+            "step_hbm_frac": round(STEP_ALG_BYTES * value / 1e9 / HBM_PEAK_GBS, 5)}
+    if "fp64_flops_per_launch" in prof:
+        tf = prof["fp64_flops_per_launch"] / (avg_ms * 1e-3) / 1e12
+        roof["fp64"] = {"achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tf / FP64_PEAK_TFLOPS, 4), "source": "rocprofv3 SQ_INSTS_VALU_*_F64"}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        try:
+            cpu = cpu_baseline(wl, steps=args.cpu_steps)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+
+    out = {
+        "metric": "GRMHD zone-updates/s on 1 MI355X, learned vs plain c2p at <=1e-10 c2p err",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded blast-wave initial data, seed 20261202)",
+        "config": {"workload": "blast3d_128cube_512x16cube_blocks_ngz3_wenoz_hlle_uct_rk3_plain_c2p",
+                   "zones": wl.zones, "blocks": int(wl.grid.nq), "c2p": "plain Kastaun (FP64)",
+                   "graph": use_graph, "gpu_event_ms_per_step": gpu_ms / args.steps},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

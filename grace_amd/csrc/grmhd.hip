@@ -610,30 +610,107 @@ __global__ __launch_bounds__(256) void k_ghost(DevGrid G, const DevInst* __restr
     }
 }
 
+// A ghost cell whose position lies outside the instance across an outflow
+// face holds clamped copies of conserved and face B whose face average differs
+// from any interior cell: it needs its own c2p.  Every other ghost cell (same
+// level neighbour or periodic image) has the exact inputs of its source cell,
+// so its c2p outputs are copies of the source's.
+GHD bool cell_is_interior(const DevGrid& G, int i, int j, int k) {
+    return i >= G.ng && i < G.n[0] + G.ng && j >= G.ng && j < G.n[1] + G.ng && k >= G.ng && k < G.n[2] + G.ng;
+}
+GHD bool ghost_needs_own_c2p(const DevGrid& G, const DevInst& I, const DevBlk& b, int i, int j, int k) {
+    int loc[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) {
+        int Nt = I.nb[d] * G.n[d];
+        int g = b.b[d] * G.n[d] + loc[d] - G.ng;
+        if (g < 0 && I.bc[2 * d] == 0) return true;
+        if (g >= Nt && I.bc[2 * d + 1] == 0) return true;
+    }
+    return false;
+}
+
+// Copy c2p outputs (all aux variables, the 7 conserved and, for GS, E_center)
+// into ghost cells that do not need their own c2p.
+__global__ __launch_bounds__(256) void k_ghost_prims(DevGrid G, const DevInst* __restrict__ inst,
+                                                     const DevBlk* __restrict__ blk, double* st, int nv,
+                                                     double* aux, double* ecen) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G.ncell * G.nq) return;
+    int64_t q = t / G.ncell, c = t - q * G.ncell;
+    int i = (int)(c % G.N[0]), j = (int)((c / G.N[0]) % G.N[1]), k = (int)(c / ((int64_t)G.N[0] * G.N[1]));
+    if (cell_is_interior(G, i, j, k)) return;
+    const DevBlk b = blk[q];
+    const DevInst& I = inst[b.inst];
+    if (ghost_needs_own_c2p(G, I, b, i, j, k)) return;
+    int loc[3] = {i, j, k}, sb[3], sl[3];
+    for (int d = 0; d < 3; ++d) {
+        int Nt = I.nb[d] * G.n[d];
+        int m = map_cc(b.b[d] * G.n[d] + loc[d] - G.ng, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
+        sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng;
+    }
+    int64_t qs = I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]);
+    int64_t cs = cidx(G, sl[0], sl[1], sl[2]);
+    for (int v = 0; v < NAUX; ++v) aux[(q * NAUX + v) * G.ncell + c] = aux[(qs * NAUX + v) * G.ncell + cs];
+    for (int v = 0; v < NHRSC; ++v) st[(q * nv + v) * G.ncell + c] = st[(qs * nv + v) * G.ncell + cs];
+    if (ecen)
+        for (int v = 0; v < 3; ++v) ecen[(q * 3 + v) * G.ncell + c] = ecen[(qs * 3 + v) * G.ncell + cs];
+}
+
 // ---------------------------------------------------------------------------
-// c2p statistics (global counters, cleared by a kernel)
+// c2p statistics: one partial record per workgroup (no atomics), summed by a
+// single-workgroup reduction.
 // ---------------------------------------------------------------------------
 struct DevStats {
     unsigned long long n_cells, n_fatal, n_atmo, n_ent, n_guess, n_root, n_bracket, max_root;
 };
-__global__ void k_clear_stats(DevStats* s) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *s = DevStats{0, 0, 0, 0, 0, 0, 0, 0};
-}
-// Wave-level reduction of the per-lane counters, one atomic per wavefront.
-__device__ inline void stats_add(DevStats* s, bool fatal, uint64_t err, const C2PCount& cnt) {
-    unsigned long long a[7] = {1ull, (unsigned long long)fatal, (unsigned long long)((err >> E_ATMO_RESET) & 1),
+// Block-level reduction of the per-lane counters into part[blockIdx.x].
+// Every thread of the (256-thread) workgroup must call it.
+__device__ inline void stats_block(DevStats* part, bool active, bool fatal, uint64_t err, const C2PCount& cnt) {
+    __shared__ unsigned long long sh[4][8];
+    unsigned long long a[8] = {(unsigned long long)active, (unsigned long long)fatal,
+                               (unsigned long long)((err >> E_ATMO_RESET) & 1),
                                (unsigned long long)((err >> E_ENT_BACKUP_USED) & 1), (unsigned long long)cnt.guess_used,
-                               (unsigned long long)cnt.root, (unsigned long long)cnt.bracket};
-    unsigned long long mx = (unsigned long long)cnt.root;
+                               (unsigned long long)cnt.root, (unsigned long long)cnt.bracket,
+                               (unsigned long long)cnt.root};
     for (int off = 32; off > 0; off >>= 1) {
         for (int x = 0; x < 7; ++x) a[x] += __shfl_down(a[x], off, 64);
-        unsigned long long o = __shfl_down(mx, off, 64);
-        mx = o > mx ? o : mx;
+        unsigned long long o = __shfl_down(a[7], off, 64);
+        a[7] = o > a[7] ? o : a[7];
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&s->n_cells, a[0]); atomicAdd(&s->n_fatal, a[1]); atomicAdd(&s->n_atmo, a[2]);
-        atomicAdd(&s->n_ent, a[3]); atomicAdd(&s->n_guess, a[4]); atomicAdd(&s->n_root, a[5]);
-        atomicAdd(&s->n_bracket, a[6]); atomicMax(&s->max_root, mx);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int x = 0; x < 8; ++x) sh[w][x] = a[x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DevStats r{0, 0, 0, 0, 0, 0, 0, 0};
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int v = 0; v < nw; ++v) {
+            r.n_cells += sh[v][0]; r.n_fatal += sh[v][1]; r.n_atmo += sh[v][2]; r.n_ent += sh[v][3];
+            r.n_guess += sh[v][4]; r.n_root += sh[v][5]; r.n_bracket += sh[v][6];
+            r.max_root = sh[v][7] > r.max_root ? sh[v][7] : r.max_root;
+        }
+        part[blockIdx.x] = r;
+    }
+}
+__global__ __launch_bounds__(256) void k_stats_final(const DevStats* __restrict__ part, int64_t np, DevStats* out) {
+    __shared__ DevStats sh[256];
+    DevStats r{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t x = threadIdx.x; x < np; x += blockDim.x) {
+        const DevStats& p = part[x];
+        r.n_cells += p.n_cells; r.n_fatal += p.n_fatal; r.n_atmo += p.n_atmo; r.n_ent += p.n_ent;
+        r.n_guess += p.n_guess; r.n_root += p.n_root; r.n_bracket += p.n_bracket;
+        r.max_root = p.max_root > r.max_root ? p.max_root : r.max_root;
+    }
+    sh[threadIdx.x] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DevStats t{0, 0, 0, 0, 0, 0, 0, 0};
+        for (int v = 0; v < (int)blockDim.x; ++v) {
+            t.n_cells += sh[v].n_cells; t.n_fatal += sh[v].n_fatal; t.n_atmo += sh[v].n_atmo; t.n_ent += sh[v].n_ent;
+            t.n_guess += sh[v].n_guess; t.n_root += sh[v].n_root; t.n_bracket += sh[v].n_bracket;
+            t.max_root = sh[v].max_root > t.max_root ? sh[v].max_root : t.max_root;
+        }
+        *out = t;
     }
 }
 
@@ -646,12 +723,14 @@ __global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, c
                                                    const double* __restrict__ mstate, int mnv,
                                                    const double* __restrict__ Bx, const double* __restrict__ By,
                                                    const double* __restrict__ Bz, double* aux, double* ecen,
-                                                   DevStats* stats) {
+                                                   DevStats* stats, int selective) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool active = t < G.ncell * G.nq;
     int64_t tt = active ? t : 0;
     int64_t q = tt / G.ncell, c = tt - q * G.ncell;
     int i = (int)(c % G.N[0]), j = (int)((c / G.N[0]) % G.N[1]), k = (int)(c / ((int64_t)G.N[0] * G.N[1]));
+    if (selective && active && !cell_is_interior(G, i, j, k))
+        active = ghost_needs_own_c2p(G, inst[blk[q].inst], blk[q], i, j, k);
     const int64_t nc = G.ncell;
     Met m;
     if constexpr (FLAT) m.flat();
@@ -696,16 +775,9 @@ __global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, c
             Ec[2 * nc] = Bc0 * vt[1] - Bc1 * vt[0];
         }
     }
-    if (!active) { cnt = C2PCount(); }
-    if (stats) {
-        if (active) stats_add(stats, out.fatal, err, cnt);
-        else stats_add(stats, false, 0, cnt);
-    }
+    if (!active) { cnt = C2PCount(); err = 0; }
+    if (stats) stats_block(stats, active, active && out.fatal, err, cnt);
 }
-
-// Remove the per-lane "n_cells" count of inactive lanes: done on the host by
-// recording the launch size; the device count includes inactive lanes, so the
-// host overwrites n_cells with the true count.
 
 // ---------------------------------------------------------------------------
 // c2p over a flat batch
@@ -741,7 +813,7 @@ __global__ __launch_bounds__(256) void k_c2p_batch(int64_t n, EOS eos, Opts o, d
     } else {
         cnt = C2PCount();
     }
-    if (stats) stats_add(stats, active && out.fatal, active ? err : 0, cnt);
+    if (stats) stats_block(stats, active, active && out.fatal, active ? err : 0, cnt);
 }
 
 __global__ __launch_bounds__(256) void k_p2c_batch(int64_t n, const double* __restrict__ prims,
@@ -848,6 +920,8 @@ struct grmhd_ctx {
     double* B2[3] = {nullptr, nullptr, nullptr};
     double* dtpart = nullptr; int64_t n_dtpart = 0;
     DevStats* stats = nullptr;
+    DevStats* stats_part = nullptr;
+    int64_t n_part_cap = 0;
     int64_t last_c2p_n = 0;
     bool timing = false;
     std::vector<TimerRec> timers;
@@ -922,7 +996,7 @@ static void build_eos(grmhd_ctx* c) {
 
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats);
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1014,6 +1088,8 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
     if ((e = hipMalloc(&c->dtpart, sizeof(double) * c->n_dtpart)) != hipSuccess) return fail("dtpart", e);
     if ((e = hipMalloc(&c->stats, sizeof(DevStats))) != hipSuccess) return fail("stats", e);
     if ((e = hipMemset(c->stats, 0, sizeof(DevStats))) != hipSuccess) return fail("stats clear", e);
+    c->n_part_cap = nblk(G.ncell * G.nq);
+    if ((e = hipMalloc(&c->stats_part, sizeof(DevStats) * c->n_part_cap)) != hipSuccess) return fail("stats part", e);
     *out = c;
     return GRMHD_OK;
 }
@@ -1039,9 +1115,18 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
     int64_t nx = (int64_t)(G.n[0] + 1) * G.N[1] * G.N[2] * G.nq;
     int64_t ny = (int64_t)G.N[0] * (G.n[1] + 1) * G.N[2] * G.nq;
     int64_t nz = (int64_t)G.N[0] * G.N[1] * (G.n[2] + 1) * G.nq;
-    k_flux<EOS, REC, FLAT, 0><<<nblk(nx), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0]);
-    k_flux<EOS, REC, FLAT, 1><<<nblk(ny), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1]);
-    k_flux<EOS, REC, FLAT, 2><<<nblk(nz), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2]);
+    {
+        Scope sc(c, "flux_x", s);
+        k_flux<EOS, REC, FLAT, 0><<<nblk(nx), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0]);
+    }
+    {
+        Scope sc(c, "flux_y", s);
+        k_flux<EOS, REC, FLAT, 1><<<nblk(ny), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1]);
+    }
+    {
+        Scope sc(c, "flux_z", s);
+        k_flux<EOS, REC, FLAT, 2><<<nblk(nz), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2]);
+    }
 }
 template <typename EOS, bool FLAT>
 static void flux_rec(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
@@ -1055,7 +1140,6 @@ static void flux_rec(grmhd_ctx* c, const EOS& eos, const double* aux, const doub
 }
 static void launch_fluxes(grmhd_ctx* c, const double* aux, const double* ms, int mnv, const double* const B[3],
                           hipStream_t s) {
-    Scope sc(c, "flux", s);
     bool flat = c->P.flat_metric != 0;
     if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) {
         if (flat) flux_rec<EosIdeal, true>(c, c->ideal, aux, ms, mnv, B, s);
@@ -1100,18 +1184,25 @@ static void launch_ghost(grmhd_ctx* c, double* st, int nv, int v0, int vc, doubl
                                     B ? B[2] : nullptr, aux, a0, ac);
 }
 static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms, int mnv, const double* const B[3],
-                             double* aux, hipStream_t s) {
+                             double* aux, bool with_stats, bool selective, hipStream_t s) {
     Scope sc(c, "c2p", s);
     const DevGrid& G = c->G;
     int64_t n = G.ncell * G.nq;
     double* ec = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
-    k_clear_stats<<<1, 64, 0, s>>>(c->stats);
+    DevStats* part = with_stats ? c->stats_part : nullptr;
     bool flat = c->P.flat_metric != 0;
-#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<nblk(n), 256, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, c->stats)
+#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<nblk(n), 256, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, selective ? 1 : 0)
     if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
     else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
 #undef C2PL
-    c->last_c2p_n = n;
+    if (with_stats) {
+        k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
+        c->last_c2p_n = n;
+    }
+    if (selective) {
+        Scope sg(c, "ghost", s);
+        k_ghost_prims<<<nblk(n), 256, 0, s>>>(G, c->inst_d, c->blk_d, st, snv, aux, ec);
+    }
 }
 
 // ---- ABI entry points -------------------------------------------------------
@@ -1141,7 +1232,7 @@ extern "C" grmhd_status grmhd_fill_ghosts(grmhd_ctx* c, double* state, double* c
 extern "C" grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* c, double* state, const double* const B[3], double* aux,
                                                   grmhd_stream_t stream) {
     if (!c || !state || !B || !aux) { set_err("null argument"); return GRMHD_ERR_INVALID; }
-    launch_c2p_cells(c, state, NSTATE, state, NSTATE, B, aux, (hipStream_t)stream);
+    launch_c2p_cells(c, state, NSTATE, state, NSTATE, B, aux, true, false, (hipStream_t)stream);
     return launch_check("compute_auxiliaries");
 }
 
@@ -1152,15 +1243,22 @@ extern "C" grmhd_status grmhd_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_so
         set_err("invalid c2p batch arguments"); return GRMHD_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;
+    if (n > 0 && nblk(n) > c->n_part_cap) {
+        (void)hipFree(c->stats_part);
+        c->stats_part = nullptr;
+        c->n_part_cap = 0;
+        HIP_TRY(hipMalloc(&c->stats_part, sizeof(DevStats) * nblk(n)));
+        c->n_part_cap = nblk(n);
+    }
     Scope sc(c, "c2p", s);
-    k_clear_stats<<<1, 64, 0, s>>>(c->stats);
     if (n > 0) {
         if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
             k_c2p_batch<EosIdeal><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa, err_bits,
-                                                         floored, dry_run, r, guess, c->stats);
+                                                         floored, dry_run, r, guess, c->stats_part);
         else
             k_c2p_batch<EosHybrid><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa, prims_soa, err_bits,
-                                                          floored, dry_run, r, guess, c->stats);
+                                                          floored, dry_run, r, guess, c->stats_part);
+        k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
     }
     c->last_c2p_n = n;
     return launch_check("c2p_batch");
@@ -1250,7 +1348,7 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         launch_update(c, a, s);
         launch_ghost(c, S.nst, S.nnv, 0, NHRSC, S.nB, nullptr, 0, 0, s);
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
-        launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, s);
+        launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s);
     }
 }
 

@@ -168,9 +168,11 @@ grmhd_status grmhd_rk3_step(grmhd_ctx* ctx, const double* dt_dev, double* state,
 grmhd_status grmhd_rk3_step_graph(grmhd_ctx* ctx, const double* dt_dev, double* state,
                                   double* const B[3], double* aux, grmhd_stream_t stream);
 
-/* Duration in milliseconds of the last launch of the named kernel family,
- * measured with HIP events on the launch stream ("flux", "emf", "update",
- * "ghost", "c2p", "step"); enable with grmhd_set_timing(ctx, 1). */
+/* Total duration (ms) and number of launches of a kernel family since the
+ * last query, measured with HIP events recorded on the launch stream around
+ * each launch ("flux_x", "flux_y", "flux_z", "emf", "update", "ghost", "c2p",
+ * "step").  Enable with grmhd_set_timing(ctx, 1); graph replays record only
+ * "step". */
 grmhd_status grmhd_set_timing(grmhd_ctx* ctx, int32_t on);
 grmhd_status grmhd_kernel_time_ms(grmhd_ctx* ctx, const char* family, double* ms_total, int64_t* n_launch);
 
