@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported in this build "
@@ -130,6 +132,14 @@ def main():
     from grace_amd import workloads as W
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8, "launches": 3}), flush=True)
+        return
     log("[bench] building 128^3 blast wave (512 blocks of 16^3)")
     wl = W.blast(nb=(8, 8, 8), n=16)
     ctx = ga.Context(wl.params, wl.grid, wl.instances)

@@ -9,7 +9,7 @@
 //   k_update       fused RK linear combination + geometric sources + flux
 //                  divergence + CT face update (tstep_utils.hh:140-220,
 //                  evolve.cpp:1342-1479)
-//   k_ghost        block ghost-zone gather (boundary_conditions.cpp:55-81)
+//   k_gather       block ghost-zone gather (boundary_conditions.cpp:55-81)
 //   k_c2p_cells    c2p over cells + conserved rewrite + BDIV + E_center
 //                  (auxiliaries.cpp:81-180, grmhd.hh:306-434)
 //   k_c2p_batch    c2p over a flat batch (c2p.cpp:188-414)
@@ -560,56 +560,6 @@ GHD int map_fc(int f, int N, int lo, int hi) {
     return f;
 }
 
-__global__ __launch_bounds__(256) void k_ghost(DevGrid G, const DevInst* __restrict__ inst, const DevBlk* __restrict__ blk,
-                                               double* st, int nv, int v0, int vcount, double* Bx, double* By,
-                                               double* Bz, double* aux, int a0, int acount) {
-    int64_t per = G.ncell + G.nface[0] + G.nface[1] + G.nface[2];
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * G.nq) return;
-    int64_t q = t / per, r = t - q * per;
-    int fd = -1;
-    int64_t off = r;
-    if (off >= G.ncell) { off -= G.ncell; fd = 0;
-        if (off >= G.nface[0]) { off -= G.nface[0]; fd = 1;
-            if (off >= G.nface[1]) { off -= G.nface[1]; fd = 2; } } }
-    if (fd >= 0 && Bx == nullptr) return;
-    int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
-    int loc[3] = {(int)(off % E0), (int)((off / E0) % E1), (int)(off / ((int64_t)E0 * E1))};
-    bool ghost = false;
-    for (int d = 0; d < 3; ++d) {
-        int hi = G.n[d] + G.ng + (d == fd ? 1 : 0);
-        ghost |= (loc[d] < G.ng || loc[d] >= hi);
-    }
-    if (!ghost) return;
-    const DevBlk b = blk[q];
-    const DevInst& I = inst[b.inst];
-    int sb[3], sl[3];
-    for (int d = 0; d < 3; ++d) {
-        int Nt = I.nb[d] * G.n[d];
-        int g = b.b[d] * G.n[d] + loc[d] - G.ng;
-        if (d == fd) {
-            int m = map_fc(g, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
-            if (m == Nt) { sb[d] = I.nb[d] - 1; sl[d] = G.n[d] + G.ng; }
-            else { sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng; }
-        } else {
-            int m = map_cc(g, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
-            sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng;
-        }
-    }
-    int64_t qs = I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]);
-    if (fd < 0) {
-        int64_t cd = cidx(G, loc[0], loc[1], loc[2]), cs = cidx(G, sl[0], sl[1], sl[2]);
-        if (st)
-            for (int v = v0; v < v0 + vcount; ++v) st[(q * nv + v) * G.ncell + cd] = st[(qs * nv + v) * G.ncell + cs];
-        if (aux)
-            for (int v = a0; v < a0 + acount; ++v) aux[(q * NAUX + v) * G.ncell + cd] = aux[(qs * NAUX + v) * G.ncell + cs];
-    } else {
-        double* Bd = fd == 0 ? Bx : (fd == 1 ? By : Bz);
-        int64_t nf = G.nface[fd];
-        Bd[q * nf + fidx_d(G, fd, loc[0], loc[1], loc[2])] = Bd[qs * nf + fidx_d(G, fd, sl[0], sl[1], sl[2])];
-    }
-}
-
 // A ghost cell whose position lies outside the instance across an outflow
 // face holds clamped copies of conserved and face B whose face average differs
 // from any interior cell: it needs its own c2p.  Every other ghost cell (same
@@ -629,31 +579,70 @@ GHD bool ghost_needs_own_c2p(const DevGrid& G, const DevInst& I, const DevBlk& b
     return false;
 }
 
-// Copy c2p outputs (all aux variables, the 7 conserved and, for GS, E_center)
-// into ghost cells that do not need their own c2p.
-__global__ __launch_bounds__(256) void k_ghost_prims(DevGrid G, const DevInst* __restrict__ inst,
-                                                     const DevBlk* __restrict__ blk, double* st, int nv,
-                                                     double* aux, double* ecen) {
+// Ghost gather over precomputed ghost-shell tables (local offsets shared by
+// all blocks): entries [0,ncg) are cells, then x, y, z faces.
+// cell_mode: 0 skip cells, 1 every ghost cell, 2 ghost cells that need their
+// own c2p, 3 ghost cells that take copies of c2p outputs.
+struct GatherArgs {
+    double* st; int nv; int v0, vc;          // state vars [v0, v0+vc)
+    double* aux; uint32_t aux_mask;          // aux vars selected by bit mask
+    double* ecen;                            // GS cell-centred E (3 vars) or null
+    double* B[3];                            // face arrays (null: skip faces)
+    int cell_mode;
+};
+__global__ __launch_bounds__(256) void k_gather(DevGrid G, const DevInst* __restrict__ inst,
+                                                const DevBlk* __restrict__ blk, const int32_t* __restrict__ list,
+                                                int64_t ncg, int64_t nfg0, int64_t nfg1, int64_t nfg2, GatherArgs a) {
+    const int64_t per = ncg + (a.B[0] ? nfg0 + nfg1 + nfg2 : 0);
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= G.ncell * G.nq) return;
-    int64_t q = t / G.ncell, c = t - q * G.ncell;
-    int i = (int)(c % G.N[0]), j = (int)((c / G.N[0]) % G.N[1]), k = (int)(c / ((int64_t)G.N[0] * G.N[1]));
-    if (cell_is_interior(G, i, j, k)) return;
+    if (t >= per * G.nq) return;
+    int64_t q = t / per, e = t - q * per;
+    int fd = -1;
+    int64_t off;
+    if (e < ncg) { off = list[e]; }
+    else {
+        int64_t r = e - ncg;
+        if (r < nfg0) { fd = 0; } else if (r < nfg0 + nfg1) { fd = 1; r -= nfg0; } else { fd = 2; r -= nfg0 + nfg1; }
+        off = list[ncg + (fd >= 1 ? nfg0 : 0) + (fd == 2 ? nfg1 : 0) + r];
+    }
+    if (fd < 0 && a.cell_mode == 0) return;
+    int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
+    int loc[3] = {(int)(off % E0), (int)((off / E0) % E1), (int)(off / ((int64_t)E0 * E1))};
     const DevBlk b = blk[q];
     const DevInst& I = inst[b.inst];
-    if (ghost_needs_own_c2p(G, I, b, i, j, k)) return;
-    int loc[3] = {i, j, k}, sb[3], sl[3];
+    if (fd < 0 && a.cell_mode >= 2) {
+        bool own = ghost_needs_own_c2p(G, I, b, loc[0], loc[1], loc[2]);
+        if ((a.cell_mode == 2) != own) return;
+    }
+    int sb[3], sl[3];
     for (int d = 0; d < 3; ++d) {
         int Nt = I.nb[d] * G.n[d];
-        int m = map_cc(b.b[d] * G.n[d] + loc[d] - G.ng, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
-        sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng;
+        int g = b.b[d] * G.n[d] + loc[d] - G.ng;
+        if (d == fd) {
+            int m = map_fc(g, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
+            if (m == Nt) { sb[d] = I.nb[d] - 1; sl[d] = G.n[d] + G.ng; }
+            else { sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng; }
+        } else {
+            int m = map_cc(g, Nt, I.bc[2 * d], I.bc[2 * d + 1]);
+            sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng;
+        }
     }
     int64_t qs = I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]);
-    int64_t cs = cidx(G, sl[0], sl[1], sl[2]);
-    for (int v = 0; v < NAUX; ++v) aux[(q * NAUX + v) * G.ncell + c] = aux[(qs * NAUX + v) * G.ncell + cs];
-    for (int v = 0; v < NHRSC; ++v) st[(q * nv + v) * G.ncell + c] = st[(qs * nv + v) * G.ncell + cs];
-    if (ecen)
-        for (int v = 0; v < 3; ++v) ecen[(q * 3 + v) * G.ncell + c] = ecen[(qs * 3 + v) * G.ncell + cs];
+    if (fd < 0) {
+        const int64_t cs = cidx(G, sl[0], sl[1], sl[2]);
+        const int64_t nc = G.ncell;
+        if (a.st)
+            for (int v = a.v0; v < a.v0 + a.vc; ++v) a.st[(q * a.nv + v) * nc + off] = a.st[(qs * a.nv + v) * nc + cs];
+        if (a.aux)
+            for (int v = 0; v < NAUX; ++v)
+                if (a.aux_mask & (1u << v)) a.aux[(q * NAUX + v) * nc + off] = a.aux[(qs * NAUX + v) * nc + cs];
+        if (a.ecen)
+            for (int v = 0; v < 3; ++v) a.ecen[(q * 3 + v) * nc + off] = a.ecen[(qs * 3 + v) * nc + cs];
+    } else {
+        double* Bd = a.B[fd];
+        int64_t nf = G.nface[fd];
+        Bd[q * nf + off] = Bd[qs * nf + fidx_d(G, fd, sl[0], sl[1], sl[2])];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -922,6 +911,8 @@ struct grmhd_ctx {
     DevStats* stats = nullptr;
     DevStats* stats_part = nullptr;
     int64_t n_part_cap = 0;
+    int32_t* ghost_list = nullptr;
+    int64_t ncg = 0, nfg[3] = {0, 0, 0};
     int64_t last_c2p_n = 0;
     bool timing = false;
     std::vector<TimerRec> timers;
@@ -996,7 +987,7 @@ static void build_eos(grmhd_ctx* c) {
 
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part);
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1090,6 +1081,27 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
     if ((e = hipMemset(c->stats, 0, sizeof(DevStats))) != hipSuccess) return fail("stats clear", e);
     c->n_part_cap = nblk(G.ncell * G.nq);
     if ((e = hipMalloc(&c->stats_part, sizeof(DevStats) * c->n_part_cap)) != hipSuccess) return fail("stats part", e);
+    {   // ghost-shell tables (local offsets), cells then x/y/z faces
+        std::vector<int32_t> lst;
+        for (int fd = -1; fd < 3; ++fd) {
+            int E[3] = {G.N[0] + (fd == 0), G.N[1] + (fd == 1), G.N[2] + (fd == 2)};
+            int64_t cnt = 0;
+            for (int k = 0; k < E[2]; ++k)
+                for (int j = 0; j < E[1]; ++j)
+                    for (int i = 0; i < E[0]; ++i) {
+                        int loc[3] = {i, j, k};
+                        bool ghost = false;
+                        for (int d = 0; d < 3; ++d) ghost |= loc[d] < G.ng || loc[d] >= G.n[d] + G.ng + (d == fd ? 1 : 0);
+                        if (!ghost) continue;
+                        lst.push_back((int32_t)(i + (int64_t)E[0] * (j + (int64_t)E[1] * k)));
+                        cnt++;
+                    }
+            if (fd < 0) c->ncg = cnt; else c->nfg[fd] = cnt;
+        }
+        if ((e = hipMalloc(&c->ghost_list, sizeof(int32_t) * lst.size())) != hipSuccess) return fail("ghost list", e);
+        if ((e = hipMemcpy(c->ghost_list, lst.data(), sizeof(int32_t) * lst.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail("ghost list copy", e);
+    }
     *out = c;
     return GRMHD_OK;
 }
@@ -1175,14 +1187,14 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     if (c->P.flat_metric) k_update<true><<<nblk(n), 256, 0, s>>>(G, a, c->P.fd_order);
     else k_update<false><<<nblk(n), 256, 0, s>>>(G, a, c->P.fd_order);
 }
-static void launch_ghost(grmhd_ctx* c, double* st, int nv, int v0, int vc, double* const B[3], double* aux, int a0,
-                         int ac, hipStream_t s) {
+static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     Scope sc(c, "ghost", s);
     const DevGrid& G = c->G;
-    int64_t n = (G.ncell + G.nface[0] + G.nface[1] + G.nface[2]) * G.nq;
-    k_ghost<<<nblk(n), 256, 0, s>>>(G, c->inst_d, c->blk_d, st, nv, v0, vc, B ? B[0] : nullptr, B ? B[1] : nullptr,
-                                    B ? B[2] : nullptr, aux, a0, ac);
+    int64_t per = c->ncg + (a.B[0] ? c->nfg[0] + c->nfg[1] + c->nfg[2] : 0);
+    k_gather<<<nblk(per * G.nq), 256, 0, s>>>(G, c->inst_d, c->blk_d, c->ghost_list, c->ncg, c->nfg[0], c->nfg[1],
+                                              c->nfg[2], a);
 }
+
 static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms, int mnv, const double* const B[3],
                              double* aux, bool with_stats, bool selective, hipStream_t s) {
     Scope sc(c, "c2p", s);
@@ -1198,10 +1210,6 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     if (with_stats) {
         k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
         c->last_c2p_n = n;
-    }
-    if (selective) {
-        Scope sg(c, "ghost", s);
-        k_ghost_prims<<<nblk(n), 256, 0, s>>>(G, c->inst_d, c->blk_d, st, snv, aux, ec);
     }
 }
 
@@ -1225,7 +1233,10 @@ extern "C" grmhd_status grmhd_advance_substep(grmhd_ctx* c, const double* dt_dev
 
 extern "C" grmhd_status grmhd_fill_ghosts(grmhd_ctx* c, double* state, double* const B[3], grmhd_stream_t stream) {
     if (!c || !state || !B) { set_err("null argument"); return GRMHD_ERR_INVALID; }
-    launch_ghost(c, state, NSTATE, 0, NHRSC, B, nullptr, 0, 0, (hipStream_t)stream);
+    GatherArgs a{};
+    a.st = state; a.nv = NSTATE; a.v0 = 0; a.vc = NHRSC; a.cell_mode = 1;
+    for (int d = 0; d < 3; ++d) a.B[d] = B[d];
+    launch_gather(c, a, (hipStream_t)stream);
     return launch_check("fill_ghosts");
 }
 
@@ -1346,9 +1357,25 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         for (int d = 0; d < 3; ++d) { a.nB[d] = S.nB[d]; a.aB[d] = S.aB[d]; a.bB[d] = S.bB ? S.bB[d] : nullptr; }
         a.dt = dt_dev; a.dtfact = S.dtf;
         launch_update(c, a, s);
-        launch_ghost(c, S.nst, S.nnv, 0, NHRSC, S.nB, nullptr, 0, 0, s);
+        {   // ghost faces everywhere; conserved only where a ghost runs its own c2p
+            GatherArgs g{};
+            g.st = S.nst; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC; g.cell_mode = 2;
+            for (int d = 0; d < 3; ++d) g.B[d] = S.nB[d];
+            launch_gather(c, g, s);
+        }
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
         launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s);
+        {   // c2p outputs into the other ghost cells: the reconstructed
+            // primitives between stages, everything after the last stage
+            GatherArgs g{};
+            const bool last = sidx == 2;
+            uint32_t mask = 0x3FFu | (c->P.thermo_recon ? (1u << PRESS_) : 0u);
+            g.aux = aux; g.aux_mask = last ? 0x7FFFu : mask;
+            g.st = last ? S.nst : nullptr; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC;
+            g.ecen = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
+            g.cell_mode = 3;
+            launch_gather(c, g, s);
+        }
     }
 }
 

@@ -245,6 +245,11 @@ def test_gpu_rk3_step_blast(dev):
     say("[rk3] blast one full step")
     ref, got, _ = run_rk3_pair(wl, dev, steps=1)
     check_close(wl, ref, got, 1e-12)
+    # primitives in every cell, ghosts included (ghosts take copies of the
+    # neighbour's c2p outputs, outflow ghosts run their own c2p)
+    for v in (ga.RHO, ga.PRESS, ga.EPS, ga.ZVECX, ga.ZVECY, ga.ZVECZ, ga.BX, ga.BY, ga.BZ, ga.TEMP):
+        assert linf_rel(got[2][:, v], ref[2][:, v]) < 1e-12, v
+    assert np.array_equal(got[2][:, ga.C2P_ERR], ref[2][:, ga.C2P_ERR])
     # div B of the evolved field stays at round-off
     bdiv = interior(wl.grid, got[2][:, ga.BDIV])
     assert np.max(np.abs(bdiv)) < 1e-12
