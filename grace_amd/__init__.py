@@ -70,7 +70,27 @@ ABI_SYMBOLS = [
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
     "grmhd_setup_blast", "grmhd_setup_shocktube", "grmhd_kerr_schild", "grmhd_setup_c2p_states",
+    # include/grmhd_mlp.h
+    "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
+    "grmhd_mlp_get_params", "grmhd_mlp_set_scaling", "grmhd_mlp_get_scaling", "grmhd_mlp_set_bracket",
+    "grmhd_mlp_get_bracket", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
+    "grmhd_mlp_train_step", "grmhd_mlp_default_fit_opts", "grmhd_mlp_fit", "grmhd_c2p_batch_learned",
 ]
+
+MLP_NPARAM = 5378
+MLP_NSCALE = 24
+
+
+class MlpFitOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("val_frac", C.c_float),
+                ("delta_quantile", C.c_double), ("delta_scale", C.c_double), ("nexp", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+class MlpFitReport(C.Structure):
+    _fields_ = [("n_train", C.c_int64), ("n_val", C.c_int64), ("steps", C.c_int64)] + \
+               [(n, C.c_double) for n in ("loss_first", "loss_last", "val_mae_e", "val_p50_e", "val_p99_e",
+                                           "val_max_e", "val_mae_s", "val_p99_s", "delta_e", "delta_s", "seconds")]
 
 
 def lib():
@@ -91,7 +111,7 @@ def lib():
     L.grmhd_advance_substep.argtypes = [P, P, d, P, P, P * 3, P * 3, P, P]
     L.grmhd_fill_ghosts.argtypes = [P, P, P * 3, P]
     L.grmhd_compute_auxiliaries.argtypes = [P, P, P * 3, P, P]
-    L.grmhd_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P]
+    L.grmhd_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P, P]
     L.grmhd_prims_to_conservs_batch.argtypes = [P, i64, P, P, P, P]
     L.grmhd_c2p_stats.argtypes = [P, C.POINTER(C2PStats), P]
     L.grmhd_compute_dt.argtypes = [P, P, P, d, P, P]
@@ -104,6 +124,26 @@ def lib():
     L.grmhd_setup_shocktube.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, d, d, P, P, P, P, P, P, P]
     L.grmhd_kerr_schild.argtypes = [d, d, d, d, P, P, P]
     L.grmhd_setup_c2p_states.argtypes = [i64, C.c_uint64, d, d, P, P, P, P, P, P, P, P, P]
+    f32 = C.c_float
+    L.grmhd_mlp_init_params.argtypes = [C.c_uint64, P]
+    L.grmhd_mlp_create.argtypes = [C.c_uint64, C.POINTER(P)]
+    L.grmhd_mlp_destroy.argtypes = [P]
+    L.grmhd_mlp_destroy.restype = None
+    L.grmhd_mlp_set_params.argtypes = [P, P, P]
+    L.grmhd_mlp_get_params.argtypes = [P, P, P]
+    L.grmhd_mlp_set_scaling.argtypes = [P, P, P]
+    L.grmhd_mlp_get_scaling.argtypes = [P, P]
+    L.grmhd_mlp_set_bracket.argtypes = [P, d, d, i32]
+    L.grmhd_mlp_get_bracket.argtypes = [P, C.POINTER(d), C.POINTER(d), C.POINTER(i32)]
+    L.grmhd_mlp_features.argtypes = [P, i64, P, P, P, P]
+    L.grmhd_mlp_targets.argtypes = [i64, P, P, P, P]
+    L.grmhd_mlp_forward.argtypes = [P, i64, P, P, P]
+    L.grmhd_mlp_grad.argtypes = [P, i64, P, P, P, P]
+    L.grmhd_mlp_train_step.argtypes = [P, i64, P, P, f32, P, P]
+    L.grmhd_mlp_default_fit_opts.argtypes = [C.POINTER(MlpFitOpts)]
+    L.grmhd_mlp_default_fit_opts.restype = None
+    L.grmhd_mlp_fit.argtypes = [P, P, i64, P, P, P, C.POINTER(MlpFitOpts), C.POINTER(MlpFitReport), P]
+    L.grmhd_c2p_batch_learned.argtypes = [P, P, i64, P, P, P, P, P, P, P, P]
     for name in ABI_SYMBOLS:
         getattr(L, name)  # raises AttributeError if an exported symbol is missing
     _lib = L
@@ -194,9 +234,9 @@ class Context:
         check(self.L.grmhd_compute_auxiliaries(self.h, _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)),
               "compute_auxiliaries")
 
-    def c2p_batch(self, n, cons, met, prims, err, floored, dry_run=0, r=None, guess=None, stream=None):
+    def c2p_batch(self, n, cons, met, prims, err, floored, dry_run=0, r=None, guess=None, roots=None, stream=None):
         check(self.L.grmhd_c2p_batch(self.h, n, _ptr(cons), _ptr(met), _ptr(prims), _ptr(err), _ptr(floored),
-                                     int(dry_run), _ptr(r), _ptr(guess), _ptr(stream)), "c2p_batch")
+                                     int(dry_run), _ptr(r), _ptr(guess), _ptr(roots), _ptr(stream)), "c2p_batch")
 
     def p2c_batch(self, n, prims, met, cons, stream=None):
         check(self.L.grmhd_prims_to_conservs_batch(self.h, n, _ptr(prims), _ptr(met), _ptr(cons), _ptr(stream)),
@@ -223,3 +263,95 @@ class Context:
         n = C.c_int64()
         check(self.L.grmhd_kernel_time_ms(self.h, fam.encode(), C.byref(ms), C.byref(n)), "kernel_time_ms")
         return ms.value, n.value
+
+
+class Mlp:
+    """Handle of the learned Kastaun bracket network (include/grmhd_mlp.h)."""
+
+    def __init__(self, seed=20261202):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.grmhd_mlp_create(C.c_uint64(seed), C.byref(h)), "mlp_create")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.grmhd_mlp_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def init_params(seed):
+        import numpy as np
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        check(lib().grmhd_mlp_init_params(C.c_uint64(seed), _ptr(p)), "mlp_init_params")
+        return p
+
+    def set_params(self, p, stream=None):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == MLP_NPARAM
+        check(self.L.grmhd_mlp_set_params(self.h, _ptr(p), _ptr(stream)), "mlp_set_params")
+
+    def get_params(self, stream=None):
+        import numpy as np
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        check(self.L.grmhd_mlp_get_params(self.h, _ptr(p), _ptr(stream)), "mlp_get_params")
+        return p
+
+    def set_scaling(self, sc, stream=None):
+        import numpy as np
+        sc = np.ascontiguousarray(sc, dtype=np.float32)
+        assert sc.size == MLP_NSCALE
+        check(self.L.grmhd_mlp_set_scaling(self.h, _ptr(sc), _ptr(stream)), "mlp_set_scaling")
+
+    def get_scaling(self):
+        import numpy as np
+        sc = np.zeros(MLP_NSCALE, dtype=np.float32)
+        check(self.L.grmhd_mlp_get_scaling(self.h, _ptr(sc)), "mlp_get_scaling")
+        return sc
+
+    def set_bracket(self, delta_e, delta_s, nexp):
+        check(self.L.grmhd_mlp_set_bracket(self.h, float(delta_e), float(delta_s), int(nexp)), "mlp_set_bracket")
+
+    def get_bracket(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_int32()
+        check(self.L.grmhd_mlp_get_bracket(self.h, C.byref(a), C.byref(b), C.byref(c)), "mlp_get_bracket")
+        return a.value, b.value, c.value
+
+    def forward(self, n, feat, out, stream=None):
+        check(self.L.grmhd_mlp_forward(self.h, n, _ptr(feat), _ptr(out), _ptr(stream)), "mlp_forward")
+
+    def grad(self, n, feat, target, grad_out, stream=None):
+        check(self.L.grmhd_mlp_grad(self.h, n, _ptr(feat), _ptr(target), _ptr(grad_out), _ptr(stream)), "mlp_grad")
+
+    def train_step(self, n, feat, target, lr, loss_out=None, stream=None):
+        check(self.L.grmhd_mlp_train_step(self.h, n, _ptr(feat), _ptr(target), float(lr), _ptr(loss_out),
+                                          _ptr(stream)), "mlp_train_step")
+
+    @staticmethod
+    def default_fit_opts(**kw):
+        o = MlpFitOpts()
+        lib().grmhd_mlp_default_fit_opts(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def fit(self, ctx, n, cons, met, r=None, opts=None, stream=None):
+        rep = MlpFitReport()
+        o = opts or self.default_fit_opts()
+        check(self.L.grmhd_mlp_fit(ctx.h, self.h, n, _ptr(cons), _ptr(met), _ptr(r), C.byref(o), C.byref(rep),
+                                   _ptr(stream)), "mlp_fit")
+        return {k: getattr(rep, k) for k, _ in MlpFitReport._fields_}
+
+
+def mlp_features(ctx, n, cons, met, feat, stream=None):
+    check(lib().grmhd_mlp_features(ctx.h, n, _ptr(cons), _ptr(met), _ptr(feat), _ptr(stream)), "mlp_features")
+
+
+def mlp_targets(n, roots, feat, target, stream=None):
+    check(lib().grmhd_mlp_targets(n, _ptr(roots), _ptr(feat), _ptr(target), _ptr(stream)), "mlp_targets")
+
+
+def c2p_batch_learned(ctx, mlp, n, cons, met, prims, err, floored, r=None, roots=None, stream=None):
+    check(lib().grmhd_c2p_batch_learned(ctx.h, mlp.h, n, _ptr(cons), _ptr(met), _ptr(prims), _ptr(err),
+                                        _ptr(floored), _ptr(r), _ptr(roots), _ptr(stream)), "c2p_batch_learned")

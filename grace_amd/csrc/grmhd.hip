@@ -22,7 +22,9 @@
 #include <vector>
 #include <map>
 #include "phys.hh"
+#include "mlp.hh"
 #include "../../include/grmhd_abi.h"
+#include "../../include/grmhd_mlp.h"
 
 using namespace gm;
 
@@ -33,6 +35,7 @@ static void set_err(const char* fmt, ...) {
     va_end(ap);
 }
 extern "C" const char* grmhd_last_error(void) { return g_errbuf; }
+void grmhd_set_error_text(const char* t) { set_err("%s", t); }
 
 #define HIP_TRY(x)                                                                  \
     do {                                                                            \
@@ -658,7 +661,7 @@ __device__ inline void stats_block(DevStats* part, bool active, bool fatal, uint
     __shared__ unsigned long long sh[4][8];
     unsigned long long a[8] = {(unsigned long long)active, (unsigned long long)fatal,
                                (unsigned long long)((err >> E_ATMO_RESET) & 1),
-                               (unsigned long long)((err >> E_ENT_BACKUP_USED) & 1), (unsigned long long)cnt.guess_used,
+                               (unsigned long long)((err >> E_ENT_BACKUP_USED) & 1), (unsigned long long)__popc((unsigned)cnt.guess_used),
                                (unsigned long long)cnt.root, (unsigned long long)cnt.bracket,
                                (unsigned long long)cnt.root};
     for (int off = 32; off > 0; off >>= 1) {
@@ -741,7 +744,9 @@ __global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, c
     C2POut out{false, false};
     if (active) {
         A[BX_ * nc] = Bc0 / m.sqrtg; A[BY_ * nc] = Bc1 / m.sqrtg; A[BZ_ * nc] = Bc2 / m.sqrtg;
-        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, 1e300, err, false, cnt, 0.0, 0.0);
+        C2PGuess gs{0, 0, 0, 0, 0, 0};
+        C2PRoots rt{-1, -1, -1, -1};
+        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, 1e300, err, false, cnt, gs, rt);
         A[RHO_ * nc] = pr[RHOL]; A[EPS_ * nc] = pr[EPSL]; A[PRESS_ * nc] = pr[PRESSL];
         A[TEMP_ * nc] = pr[TEMPL]; A[ENTROPY_ * nc] = pr[ENTL]; A[YE_ * nc] = pr[YEL];
         A[ZVECX_ * nc] = pr[ZXL]; A[ZVECY_ * nc] = pr[ZYL]; A[ZVECZ_ * nc] = pr[ZZL];
@@ -771,12 +776,14 @@ __global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, c
 // ---------------------------------------------------------------------------
 // c2p over a flat batch
 // ---------------------------------------------------------------------------
-template <typename EOS>
-__global__ __launch_bounds__(256) void k_c2p_batch(int64_t n, EOS eos, Opts o, double* cons_soa,
+// LEARNED: the brackets come from the MLP (features -> BF16 MFMA forward in
+// LDS/registers, one pass per wave) instead of the guess array.
+template <typename EOS, bool LEARNED>
+__global__ __launch_bounds__(256, 2) void k_c2p_batch(int64_t n, EOS eos, Opts o, double* cons_soa,
                                                    const double* __restrict__ met, double* prims,
                                                    uint64_t* errb, uint8_t* floored, int dry_run,
                                                    const double* __restrict__ rad, const double* __restrict__ guess,
-                                                   DevStats* stats) {
+                                                   double* __restrict__ roots, DevStats* stats, MlpDev mlp) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool active = t < n;
     int64_t s = active ? t : 0;
@@ -788,13 +795,37 @@ __global__ __launch_bounds__(256) void k_c2p_batch(int64_t n, EOS eos, Opts o, d
     double cons[NCONS], pr[NPRIM];
     for (int v = 0; v < NCONS; ++v) cons[v] = cons_soa[v * n + s];
     for (int v = 0; v < NPRIM; ++v) pr[v] = 0.0;
-    double glo = 0, ghi = 0;
-    if (guess) { glo = guess[s]; ghi = guess[n + s]; }
+    C2PGuess gs{0, 0, 0, 0, 0, 0};
+    if (LEARNED) {
+        __shared__ __align__(16) uint16_t xs[4][64 * MLP_KP];
+        __shared__ __align__(16) float ys[4][64 * 2];
+        float f[MLP_NF];
+        double lxhi = 0.0;
+        mlp_features(eos, m, cons, f, lxhi);
+        if (!active) for (int k = 0; k < MLP_NF; ++k) f[k] = mlp.norm[k];
+        float y0, y1;
+        mlp_block_forward(mlp, f, xs[threadIdx.x >> 6], ys[threadIdx.x >> 6], y0, y1);
+        // energy root: FP32 exp is ample for a bracket of half-width >= 1e-6;
+        // the entropy/energy ratio is close to 1, so its bracket is formed in FP64
+        const double xe = (double)__expf((float)lxhi + y0);
+        gs.e_lo = xe * mlp.fe_lo;
+        gs.e_hi = fmin(xe * mlp.fe_hi, 1.0);
+        const double z = (double)y1;
+        const double rs = (fabs(z) < 1e-3) ? 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0))) : exp(z);
+        gs.s_lo = rs * mlp.fs_lo;
+        gs.s_hi = rs * mlp.fs_hi;
+        gs.nexp = mlp.nexp;
+        gs.s_rel = 1;
+    } else if (guess) {
+        gs.e_lo = guess[s]; gs.e_hi = guess[n + s]; gs.s_lo = guess[2 * n + s]; gs.s_hi = guess[3 * n + s];
+    }
+    C2PRoots rt{-1, -1, -1, -1};
     uint64_t err = 0;
     C2PCount cnt;
     C2POut out{false, false};
     if (active) {
-        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, rad ? rad[s] : 1e300, err, dry_run != 0, cnt, glo, ghi);
+        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, rad ? rad[s] : 1e300, err, dry_run != 0, cnt, gs, rt);
+        if (roots) { roots[s] = rt.x_e; roots[n + s] = rt.x_s; roots[2 * n + s] = rt.m_e; roots[3 * n + s] = rt.m_s; }
         for (int v = 0; v < NPRIM; ++v) prims[v * n + s] = pr[v];
         for (int v = 0; v < NCONS; ++v) cons_soa[v * n + s] = cons[v];
         errb[s] = err;
@@ -1249,7 +1280,7 @@ extern "C" grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* c, double* state, c
 
 extern "C" grmhd_status grmhd_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_soa, const double* metric_soa,
                                         double* prims_soa, uint64_t* err_bits, uint8_t* floored, int32_t dry_run,
-                                        const double* r, const double* guess, grmhd_stream_t stream) {
+                                        const double* r, const double* guess, double* roots, grmhd_stream_t stream) {
     if (!c || n < 0 || (n > 0 && (!cons_soa || !metric_soa || !prims_soa || !err_bits || !floored))) {
         set_err("invalid c2p batch arguments"); return GRMHD_ERR_INVALID;
     }
@@ -1263,16 +1294,88 @@ extern "C" grmhd_status grmhd_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_so
     }
     Scope sc(c, "c2p", s);
     if (n > 0) {
+        MlpDev none{};
         if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
-            k_c2p_batch<EosIdeal><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa, err_bits,
-                                                         floored, dry_run, r, guess, c->stats_part);
+            k_c2p_batch<EosIdeal, false><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa,
+                                                                err_bits, floored, dry_run, r, guess, roots,
+                                                                c->stats_part, none);
         else
-            k_c2p_batch<EosHybrid><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa, prims_soa, err_bits,
-                                                          floored, dry_run, r, guess, c->stats_part);
+            k_c2p_batch<EosHybrid, false><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa, prims_soa,
+                                                                 err_bits, floored, dry_run, r, guess, roots,
+                                                                 c->stats_part, none);
         k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
     }
     c->last_c2p_n = n;
     return launch_check("c2p_batch");
+}
+
+static grmhd_status ensure_stats_part(grmhd_ctx* c, int64_t n) {
+    if (n > 0 && nblk(n) > c->n_part_cap) {
+        (void)hipFree(c->stats_part);
+        c->stats_part = nullptr;
+        c->n_part_cap = 0;
+        HIP_TRY(hipMalloc(&c->stats_part, sizeof(DevStats) * nblk(n)));
+        c->n_part_cap = nblk(n);
+    }
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_c2p_batch_learned(grmhd_ctx* c, grmhd_mlp* mlp, int64_t n, double* cons_soa,
+                                                const double* metric_soa, double* prims_soa, uint64_t* err_bits,
+                                                uint8_t* floored, const double* r, double* roots,
+                                                grmhd_stream_t stream) {
+    if (!c || !mlp || n < 0 || (n > 0 && (!cons_soa || !metric_soa || !prims_soa || !err_bits || !floored))) {
+        set_err("invalid learned c2p batch arguments"); return GRMHD_ERR_INVALID;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    grmhd_status st = ensure_stats_part(c, n);
+    if (st != GRMHD_OK) return st;
+    Scope sc(c, "c2p", s);
+    if (n > 0) {
+        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
+            k_c2p_batch<EosIdeal, true><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa,
+                                                               err_bits, floored, 0, r, nullptr, roots,
+                                                               c->stats_part, mlp->dev());
+        else
+            k_c2p_batch<EosHybrid, true><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa,
+                                                                prims_soa, err_bits, floored, 0, r, nullptr, roots,
+                                                                c->stats_part, mlp->dev());
+        k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
+    }
+    c->last_c2p_n = n;
+    return launch_check("c2p_batch_learned");
+}
+
+template <typename EOS>
+__global__ __launch_bounds__(256) void k_mlp_features(int64_t n, EOS eos, const double* __restrict__ cons_soa,
+                                                      const double* __restrict__ met, float* __restrict__ feat) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    Met m;
+    for (int a = 0; a < 6; ++a) m.g[a] = met[a * n + s];
+    m.alp = met[6 * n + s];
+    for (int a = 0; a < 3; ++a) m.beta[a] = met[(7 + a) * n + s];
+    m.init();
+    double cons[NCONS];
+    for (int v = 0; v < NCONS; ++v) cons[v] = cons_soa[v * n + s];
+    float f[MLP_NF];
+    double lxhi;
+    mlp_features(eos, m, cons, f, lxhi);
+    for (int k = 0; k < MLP_NF; ++k) feat[k * n + s] = f[k];
+}
+
+extern "C" grmhd_status grmhd_mlp_features(grmhd_ctx* c, int64_t n, const double* cons_soa, const double* metric_soa,
+                                           float* feat_soa, grmhd_stream_t stream) {
+    if (!c || n < 0 || (n > 0 && (!cons_soa || !metric_soa || !feat_soa))) {
+        set_err("invalid feature arguments"); return GRMHD_ERR_INVALID;
+    }
+    if (n == 0) return GRMHD_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
+        k_mlp_features<EosIdeal><<<nblk(n), 256, 0, s>>>(n, c->ideal, cons_soa, metric_soa, feat_soa);
+    else
+        k_mlp_features<EosHybrid><<<nblk(n), 256, 0, s>>>(n, c->hybrid, cons_soa, metric_soa, feat_soa);
+    return launch_check("mlp_features");
 }
 
 extern "C" grmhd_status grmhd_prims_to_conservs_batch(grmhd_ctx* c, int64_t n, const double* prims_soa,

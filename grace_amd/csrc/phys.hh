@@ -330,8 +330,8 @@ GHD void conserved(const Met& m, double rho, double press, double eps, const dou
 // ---- roots --------------------------------------------------------------------
 // Brent's zero-in (Brent 1973, ch. 4): same step sequence as rootfinding.hh:73-231.
 template <typename F>
-__host__ __device__ inline double brent(F& f, double a, double b, double t) {
-    double sa = a, sb = b, fa = f(sa), fb = f(sb);
+__host__ __device__ inline double brent_fab(F& f, double a, double b, double fa, double fb, double t) {
+    double sa = a, sb = b;
     double c = sa, fc = fa, e = sb - sa, d = e;
     const double macheps = 2.220446049250313e-16;
     for (;;) {
@@ -359,6 +359,11 @@ __host__ __device__ inline double brent(F& f, double a, double b, double t) {
         if ((0.0 < fb && 0.0 < fc) || (fb <= 0.0 && fc <= 0.0)) { c = sa; fc = fa; e = sb - sa; d = e; }
     }
     return sb;
+}
+template <typename F>
+__host__ __device__ inline double brent(F& f, double a, double b, double t) {
+    double fa = f(a), fb = f(b);
+    return brent_fab(f, a, b, fa, fb, t);
 }
 template <typename F>
 __host__ __device__ inline double newton_br(double sa, double sb, F& f, int maxiter, double tol, int& err) {
@@ -530,26 +535,72 @@ GHD void c2p_setup(const EOS& eos, const Met& m, double cons[NCONS], C2PIn& c) {
 }
 
 struct C2PCount { int root = 0, bracket = 0; int guess_used = 0; };
+// Optional brackets for the two root solves, as fractions of 1/h0 (the
+// master functions have one root on (0, 1/h0]): energy [e_lo, e_hi], entropy
+// [s_lo, s_hi].  A bracket is used when 0 <= lo < hi <= 1 and the master
+// function changes sign on it; the Newton solve for the upper bound mu+ and
+// the default bracket [0, mu+] are then skipped.  Otherwise the plain path
+// runs unchanged.
+// nexp > 0 lets a bracket that misses the root be moved towards it (see
+// guess_bracket); s_rel makes [s_lo, s_hi] factors on the energy root.
+struct C2PGuess { double e_lo, e_hi, s_lo, s_hi; int nexp, s_rel; };
+// Diagnostics of the root solves, all scaled by h0: the roots (x_e, x_s) and
+// the default bracket ends mu+ h0 (m_e, m_s); -1 when not computed.
+struct C2PRoots { double x_e, x_s, m_e, m_s; };
+
+GHD bool guess_ok(double lo, double hi) { return lo < hi && lo >= 0.0 && hi <= 1.0; }
+
+// Sign change of a master function (negative left of its one root on
+// (0, umax]) starting from [lo, hi].  While the bracket misses the root, up
+// to nexp times, it is replaced by the adjacent interval on the root's side,
+// with its log-width squared-and-doubled each time (factor g -> g^4), reusing
+// the end already evaluated.  On success a, b, fa, fb hold the bracket.
+template <typename F>
+GHD bool guess_bracket(F& f, double lo, double hi, double umax, int nexp, double& a, double& b, double& fa,
+                       double& fb) {
+    fa = f(lo); fb = f(hi);
+    double g = (lo > 0.0) ? hi / lo : 2.0;
+    for (int k = 0;; ++k) {
+        if (fa * fb < 0) { a = lo; b = hi; return true; }
+        if (k == nexp || fa == 0.0 || fb == 0.0) return false;
+        g = g * g; g = g * g;
+        if (fb < 0) {                 // both ends left of the root
+            if (hi >= umax) return false;
+            lo = hi; fa = fb; hi = fmin(hi * g, umax); fb = f(hi);
+        } else {                      // both ends right of the root
+            hi = lo; fb = fa; lo = lo / g; fa = f(lo);
+        }
+    }
+}
 
 template <typename EOS>
 GHD double kastaun(const EOS& eos, const Met& m, double cons[NCONS], double pr[NPRIM], unsigned& sig,
-                   C2PCount& cnt, double glo, double ghi) {
+                   C2PCount& cnt, const C2PGuess& gs, C2PRoots& rt) {
     C2PIn c; c2p_setup(eos, m, cons, c);
-    double h0 = c.h0, mu0 = 1 / h0;
-    if (c.r2 >= h0 * h0) {
-        FBr g{c.r2, c.Bt2, c.rBt2, h0 * h0, &cnt.bracket};
-        double lo, hi; g.bracket(lo, hi);
-        int rerr;
-        mu0 = newton_br(lo, hi, g, 30, 1e-10, rerr);
-        if (rerr == 0) mu0 *= 1 + 1e-10; else mu0 = 1.0 / h0;
-    }
+    double h0 = c.h0;
     FMu<EOS> f(eos, c.D, c.q, c.r2, c.rBt2, c.Bt2, h0, &cnt.root);
-    double a = 0, b = mu0;
-    if (glo < ghi && glo >= 0 && ghi <= mu0) {
-        double fl = f(glo), fh = f(ghi);
-        if (fl * fh < 0) { a = glo; b = ghi; cnt.guess_used = 1; }
+    double mu = -1.0;
+    if (guess_ok(gs.e_lo, gs.e_hi)) {
+        double a, b, fa, fb;
+        if (guess_bracket(f, gs.e_lo / h0, gs.e_hi / h0, 1.0 / h0, gs.nexp, a, b, fa, fb)) {
+            mu = brent_fab(f, a, b, fa, fb, 1e-15);
+            cnt.guess_used |= 1;
+        }
     }
-    double mu = brent(f, a, b, 1e-15);
+    rt.m_e = -1.0;
+    if (mu < 0.0) {
+        double mu0 = 1 / h0;
+        if (c.r2 >= h0 * h0) {
+            FBr g{c.r2, c.Bt2, c.rBt2, h0 * h0, &cnt.bracket};
+            double lo, hi; g.bracket(lo, hi);
+            int rerr;
+            mu0 = newton_br(lo, hi, g, 30, 1e-10, rerr);
+            if (rerr == 0) mu0 *= 1 + 1e-10; else mu0 = 1.0 / h0;
+        }
+        mu = brent(f, 0., mu0, 1e-15);
+        rt.m_e = mu0 * h0;
+    }
+    rt.x_e = mu * h0;
     double x, w, eps, rho, press, temp, ent;
     double res = f.prims(mu, sig, x, w, rho, press, eps, temp, ent);
     pr[EPSL] = eps; pr[RHOL] = rho; pr[PRESSL] = press; pr[TEMPL] = temp; pr[ENTL] = ent;
@@ -560,17 +611,31 @@ GHD double kastaun(const EOS& eos, const Met& m, double cons[NCONS], double pr[N
 
 template <typename EOS>
 GHD double kastaun_ent(const EOS& eos, const Met& m, double cons[NCONS], double pr[NPRIM], unsigned& sig,
-                       C2PCount& cnt) {
+                       C2PCount& cnt, const C2PGuess& gs, C2PRoots& rt) {
     C2PIn c; c2p_setup(eos, m, cons, c);
-    double h0 = c.h0, mu0 = 1 / h0;
-    if (c.r2 >= h0 * h0) {
-        FBr g{c.r2, c.Bt2, c.rBt2, h0 * h0, &cnt.bracket};
-        int rerr;
-        mu0 = newton_br(0., 1. / h0, g, 30, 1e-10, rerr);
-        if (rerr == 0) mu0 *= 1 + 1e-10; else mu0 = brent(g, 0., 1. / h0, 1e-10) * (1 + 1e-10);
-    }
+    double h0 = c.h0;
     FMuEnt<EOS> f(eos, c.D, c.r2, c.rBt2, c.Bt2, c.s, h0, &cnt.root);
-    double mu = brent(f, 0., mu0, 1e-15);
+    double mu = -1.0;
+    if (guess_ok(gs.s_lo, gs.s_hi)) {
+        double a, b, fa, fb;
+        if (guess_bracket(f, gs.s_lo / h0, gs.s_hi / h0, 1.0 / h0, gs.nexp, a, b, fa, fb)) {
+            mu = brent_fab(f, a, b, fa, fb, 1e-15);
+            cnt.guess_used |= 2;
+        }
+    }
+    rt.m_s = -1.0;
+    if (mu < 0.0) {
+        double mu0 = 1 / h0;
+        if (c.r2 >= h0 * h0) {
+            FBr g{c.r2, c.Bt2, c.rBt2, h0 * h0, &cnt.bracket};
+            int rerr;
+            mu0 = newton_br(0., 1. / h0, g, 30, 1e-10, rerr);
+            if (rerr == 0) mu0 *= 1 + 1e-10; else mu0 = brent(g, 0., 1. / h0, 1e-10) * (1 + 1e-10);
+        }
+        mu = brent(f, 0., mu0, 1e-15);
+        rt.m_s = mu0 * h0;
+    }
+    rt.x_s = mu * h0;
     double x, w, eps, rho, press, temp, ent;
     double res = f.prims(mu, sig, x, w, rho, press, eps, temp, ent);
     pr[EPSL] = eps; pr[RHOL] = rho; pr[PRESSL] = press; pr[TEMPL] = temp; pr[ENTL] = ent;
@@ -655,7 +720,7 @@ struct C2POut { bool floored, fatal; };
 template <typename EOS>
 GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eos, const Atmo& at,
                const ExcPar& ex, const C2PPar& cp, double radius, uint64_t& err, bool dry_run, C2PCount& cnt,
-               double glo, double ghi) {
+               const C2PGuess& gs, C2PRoots& rt) {
     C2POut out{false, false};
     bool failed = false, floored = false;
     unsigned ret = 0;
@@ -686,7 +751,7 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
     }
     if (floored && dry_run) { out.floored = true; return out; }
     if (cons[DENSL] >= at.rho_fl * (1 + at.atmo_tol)) {
-        double res = kastaun(eos, m, cons, pr, ret, cnt, glo, ghi);
+        double res = kastaun(eos, m, cons, pr, ret, cnt, gs, rt);
         failed = (fabs(res) > cp.tol) || (ret & BIT(SIG_RHO_TOO_HIGH)) || (ret & BIT(SIG_EPS_TOO_HIGH)) || !isfinite(res);
         double beta;
         {
@@ -699,7 +764,9 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
             double save[NPRIM];
             for (int i = 0; i < NPRIM; ++i) save[i] = pr[i];
             unsigned bret = 0;
-            double rb = kastaun_ent(eos, m, cons, pr, bret, cnt);
+            C2PGuess gse = gs;
+            if (gs.s_rel) { gse.s_lo = gs.s_lo * rt.x_e; gse.s_hi = fmin(gs.s_hi * rt.x_e, 1.0); }
+            double rb = kastaun_ent(eos, m, cons, pr, bret, cnt, gse, rt);
             bool ok = (fabs(rb) <= cp.tol) && isfinite(rb) && (pr[EPSL] < eos.eps_max);
             if (ok) { ret = bret; failed = false; err |= BIT(E_ENT_BACKUP_USED); }
             else if (!failed) { for (int i = 0; i < NPRIM; ++i) pr[i] = save[i]; }

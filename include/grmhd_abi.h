@@ -134,12 +134,19 @@ grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* ctx, double* state, const doub
  * Outputs: prims_soa[13*n] (RHO PRESS ZX ZY ZZ YE TEMP EPS ENT BX BY BZ unused),
  *   cons_soa overwritten with the re-densitised conserved, err_bits[n]
  *   (c2p_err_t words), floored[n] (bool as uint8).  r_or_null: radii for
- *   radius-based excision.  guess_or_null: per-state [mu_lo, mu_hi] bracket
- *   (2*n, SoA) tried before the default [0, mu0] bracket.
+ *   radius-based excision.
+ * guess_or_null (4*n, SoA): brackets [e_lo, e_hi] of the energy root and
+ *   [s_lo, s_hi] of the entropy-backup root, in units of 1/h0 (mu h0; the
+ *   master functions of kastaun_c2p.hh:121-301 and ent_based_c2p.hh:44-160
+ *   have one root on (0, 1/h0]).  A bracket that changes sign replaces the
+ *   Newton solve for mu+ and the default bracket [0, mu+]
+ *   (kastaun_c2p.hh:355-372); any other bracket leaves the plain path as is.
+ * roots_or_null (4*n, SoA): roots x_e = mu_e h0, x_s = mu_s h0, then the
+ *   default bracket ends mu+ h0 of each solve; -1 where not computed.
  * Returns GRMHD_OK; per-state fatal branches are reported in stats. */
 grmhd_status grmhd_c2p_batch(grmhd_ctx* ctx, int64_t n, double* cons_soa, const double* metric_soa,
                              double* prims_soa, uint64_t* err_bits, uint8_t* floored, int32_t dry_run,
-                             const double* r_or_null, const double* guess_or_null,
+                             const double* r_or_null, const double* guess_or_null, double* roots_or_null,
                              grmhd_stream_t stream);
 
 /* prims_to_conservs (src/physics/eos/c2p.cpp:416-455) over a flat batch. */

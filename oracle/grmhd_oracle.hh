@@ -588,9 +588,8 @@ inline void get_geom_sources(double alp, const double beta[3], const double g[6]
 // Pinned by: closed-form roots and agreement with bisection (tests).
 // ---------------------------------------------------------------------------
 template <typename F>
-inline double brent(F& f, double a, double b, double t) {
+inline double brent_with_ends(F& f, double a, double b, double fa, double fb, double t) {
     double sa = a, sb = b;
-    double fa = f(sa), fb = f(sb);
     double c = sa, fc = fa;
     double e = sb - sa, d = e;
     const double macheps = std::numeric_limits<double>::epsilon();
@@ -632,6 +631,11 @@ inline double brent(F& f, double a, double b, double t) {
         }
     }
     return sb;
+}
+template <typename F>
+inline double brent(F& f, double a, double b, double t) {
+    double fa = f(a), fb = f(b);
+    return brent_with_ends(f, a, b, fa, fb, t);
 }
 
 template <typename F>
@@ -851,34 +855,59 @@ struct C2PSetup {
     }
 };
 
-// Optional learned/explicit initial bracket for the Kastaun root:
-// if lo < hi and f(lo) f(hi) < 0 the bracket [lo, hi] replaces [0, mu0].
-struct MuGuess { double lo = 0, hi = 0; bool valid = false; bool used = false; };
+// Optional narrowed brackets for the energy root (e_lo, e_hi) and the
+// entropy-backup root (s_lo, s_hi), in units of 1/h0: the master functions
+// have one root on (0, 1/h0] (Kastaun, Kalinani & Ciolfi 2021, PRD 103,
+// 023018, sec. III; checked on a dense grid in the tests).  A bracket with
+// 0 <= lo < hi <= 1 on which f changes sign is searched directly, skipping
+// the Newton solve for mu+ and the bracket [0, mu+]; otherwise the plain
+// path runs.  The roots x = mu h0 and the bracket ends mu+ h0 are written
+// back (-1 where not computed).
+struct MuGuess {
+    double e_lo = 0, e_hi = 0, s_lo = 0, s_hi = 0;
+    int used = 0;                    // bit 0: energy bracket, bit 1: entropy bracket
+    double x_e = -1, x_s = -1, m_e = -1, m_s = -1;
+};
+
+// Brent on the guessed bracket [lo h0^-1, hi h0^-1] when it changes sign;
+// returns -1 when the bracket is not usable.
+template <typename F>
+inline double solve_in_guess(F& f, double h0, double lo_frac, double hi_frac, bool& used) {
+    used = false;
+    if (lo_frac < hi_frac && lo_frac >= 0.0 && hi_frac <= 1.0) {
+        double lo = lo_frac / h0, hi = hi_frac / h0;
+        double flo = f(lo), fhi = f(hi);
+        if (flo * fhi < 0) { used = true; return brent_with_ends(f, lo, hi, flo, fhi, 1e-15); }
+    }
+    return -1.0;
+}
 
 template <typename EOS>
 inline double kastaun_invert(const EOS& eos, const Metric& m, double cons[NCONS], double prims[NPRIM],
                              Bits& sig, C2PStats* st = nullptr, MuGuess* guess = nullptr) {
     C2PSetup cs(eos, m, cons);
     double h0 = cs.h0;
-    double mu0 = 1 / h0;
-    if (cs.r2 >= h0 * h0) {
-        FBrack g(cs.r2, cs.Bt2, cs.rBt2, h0);
-        g.st = st;
-        double lo, hi;
-        g.bracket(lo, hi);
-        int rerr;
-        mu0 = newton_bracketed(lo, hi, g, 30, 1e-10, rerr);
-        if (rerr == 0) mu0 *= 1 + 1e-10;
-        else mu0 = 1.0 / h0;
-    }
     FRoot<EOS> fmu(eos, cs.D, cs.q, cs.r2, cs.rBt2, cs.Bt2, cs.ye, h0);
     fmu.st = st;
-    double a = 0, b = mu0;
-    if (guess && guess->valid && guess->lo < guess->hi && guess->lo >= 0 && guess->hi <= mu0) {
-        double flo = fmu(guess->lo), fhi = fmu(guess->hi);
-        if (flo * fhi < 0) { a = guess->lo; b = guess->hi; guess->used = true; }
+    bool used = false;
+    double mu = guess ? solve_in_guess(fmu, h0, guess->e_lo, guess->e_hi, used) : -1.0;
+    if (guess) { guess->used |= used ? 1 : 0; guess->m_e = -1; }
+    if (!used) {
+        double mu0 = 1 / h0;
+        if (cs.r2 >= h0 * h0) {
+            FBrack g(cs.r2, cs.Bt2, cs.rBt2, h0);
+            g.st = st;
+            double lo, hi;
+            g.bracket(lo, hi);
+            int rerr;
+            mu0 = newton_bracketed(lo, hi, g, 30, 1e-10, rerr);
+            if (rerr == 0) mu0 *= 1 + 1e-10;
+            else mu0 = 1.0 / h0;
+        }
+        mu = brent(fmu, 0, mu0, 1e-15);
+        if (guess) guess->m_e = mu0 * h0;
     }
-    double mu = brent(fmu, a, b, 1e-15);
+    if (guess) guess->x_e = mu * h0;
     double x, w, eps, rho, press, temp, entropy;
     double residual = fmu.compute_primitives(mu, sig, x, w, rho, press, eps, temp, entropy);
     prims[EPSL] = eps; prims[RHOL] = rho; prims[PRESSL] = press; prims[TEMPL] = temp; prims[ENTL] = entropy;
@@ -889,21 +918,28 @@ inline double kastaun_invert(const EOS& eos, const Metric& m, double cons[NCONS]
 
 template <typename EOS>
 inline double entropy_invert(const EOS& eos, const Metric& m, double cons[NCONS], double prims[NPRIM],
-                             Bits& sig, C2PStats* st = nullptr) {
+                             Bits& sig, C2PStats* st = nullptr, MuGuess* guess = nullptr) {
     C2PSetup cs(eos, m, cons);
     double h0 = cs.h0;
-    double mu0 = 1 / h0;
-    if (cs.r2 >= h0 * h0) {
-        FBrack g(cs.r2, cs.Bt2, cs.rBt2, h0);
-        g.st = st;
-        int rerr;
-        mu0 = newton_bracketed(0, 1. / h0, g, 30, 1e-10, rerr);
-        if (rerr == 0) mu0 *= 1 + 1e-10;
-        else mu0 = brent(g, 0, 1. / h0, 1e-10) * (1 + 1e-10);
-    }
     EntFRoot<EOS> fmu(eos, cs.D, cs.r2, cs.rBt2, cs.Bt2, cs.s, cs.ye, h0);
     fmu.st = st;
-    double mu = brent(fmu, 0, mu0, 1e-15);
+    bool used = false;
+    double mu = guess ? solve_in_guess(fmu, h0, guess->s_lo, guess->s_hi, used) : -1.0;
+    if (guess) { guess->used |= used ? 2 : 0; guess->m_s = -1; }
+    if (!used) {
+        double mu0 = 1 / h0;
+        if (cs.r2 >= h0 * h0) {
+            FBrack g(cs.r2, cs.Bt2, cs.rBt2, h0);
+            g.st = st;
+            int rerr;
+            mu0 = newton_bracketed(0, 1. / h0, g, 30, 1e-10, rerr);
+            if (rerr == 0) mu0 *= 1 + 1e-10;
+            else mu0 = brent(g, 0, 1. / h0, 1e-10) * (1 + 1e-10);
+        }
+        mu = brent(fmu, 0, mu0, 1e-15);
+        if (guess) guess->m_s = mu0 * h0;
+    }
+    if (guess) guess->x_s = mu * h0;
     double x, w, eps, rho, press, temp, entropy;
     double residual = fmu.compute_primitives(mu, sig, x, w, rho, press, eps, temp, entropy);
     prims[EPSL] = eps; prims[RHOL] = rho; prims[PRESSL] = press; prims[TEMPL] = temp; prims[ENTL] = entropy;
@@ -1117,7 +1153,7 @@ inline C2PResult conservs_to_prims(double cons[NCONS], double prims[NPRIM], cons
             double save[NPRIM];
             std::memcpy(save, prims, sizeof(save));
             Bits bret;
-            double rb = entropy_invert(eos, m, cons, prims, bret, st);
+            double rb = entropy_invert(eos, m, cons, prims, bret, st, guess);
             bool ok = (std::fabs(rb) <= cp.tol) && std::isfinite(rb) && (prims[EPSL] < eos.c2p_eps_max());
             if (ok) {
                 ret = bret;

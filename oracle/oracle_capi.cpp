@@ -61,7 +61,7 @@ Metric met_from_soa(const double* met, int64_t n, int64_t s) {
 template <typename EOS>
 void c2p_batch_impl(const grmhd_params* p, const EOS& eos, int64_t n, double* cons_soa, const double* met,
                     double* prims, uint64_t* errb, uint8_t* floored, int dry_run, const double* r,
-                    const double* guess, int64_t* evals, int64_t* fatal_out) {
+                    const double* guess, double* roots, int64_t* evals, int64_t* fatal_out) {
     AtmoParams atmo = make_atmo(p);
     C2PParams cp = make_c2p(p);
     ExcisionParams ex = make_exc(p);
@@ -74,10 +74,11 @@ void c2p_batch_impl(const grmhd_params* p, const EOS& eos, int64_t n, double* co
         Bits err;
         C2PStats st;
         MuGuess g;
-        if (guess) { g.lo = guess[s]; g.hi = guess[n + s]; g.valid = true; }
+        if (guess) { g.e_lo = guess[s]; g.e_hi = guess[n + s]; g.s_lo = guess[2 * n + s]; g.s_hi = guess[3 * n + s]; }
         double rr = r ? r[s] : 1e300;
         C2PResult res = conservs_to_prims(cons, pr, m, eos, atmo, ex, cp, &rr, err, dry_run != 0, &st,
-                                          guess ? &g : nullptr);
+                                          (guess || roots) ? &g : nullptr);
+        if (roots) { roots[s] = g.x_e; roots[n + s] = g.x_s; roots[2 * n + s] = g.m_e; roots[3 * n + s] = g.m_s; }
         if (res.fatal) fatal++;
         for (int v = 0; v < NPRIM; ++v) prims[v * n + s] = pr[v];
         for (int v = 0; v < NCONS; ++v) cons_soa[v * n + s] = cons[v];
@@ -148,11 +149,11 @@ int orc_num_threads(int n) {
 
 int orc_c2p_batch(const grmhd_params* p, int64_t n, double* cons_soa, const double* met, double* prims,
                   uint64_t* errb, uint8_t* floored, int dry_run, const double* r, const double* guess,
-                  int64_t* evals, int64_t* fatal) {
+                  double* roots, int64_t* evals, int64_t* fatal) {
     if (p->eos_kind == GRMHD_EOS_IDEAL_GAS)
-        c2p_batch_impl(p, make_ideal(p), n, cons_soa, met, prims, errb, floored, dry_run, r, guess, evals, fatal);
+        c2p_batch_impl(p, make_ideal(p), n, cons_soa, met, prims, errb, floored, dry_run, r, guess, roots, evals, fatal);
     else
-        c2p_batch_impl(p, make_hybrid(p), n, cons_soa, met, prims, errb, floored, dry_run, r, guess, evals, fatal);
+        c2p_batch_impl(p, make_hybrid(p), n, cons_soa, met, prims, errb, floored, dry_run, r, guess, roots, evals, fatal);
     return 0;
 }
 
@@ -238,6 +239,28 @@ double orc_kastaun_f(const grmhd_params* p, const double cons_in[10], const doub
     if (mu0_out) *mu0_out = mu0;
     FRoot<IdealGas> f(eos, cs.D, cs.q, cs.r2, cs.rBt2, cs.Bt2, cs.ye, h0);
     return f(mu);
+}
+
+// Master function of the energy (which=0) or entropy (which=1) solve on a
+// grid of mu values, for one densitised state (ideal gas).
+void orc_master_grid(const grmhd_params* p, const double cons_in[10], const double met[10], int which,
+                     int64_t nmu, const double* mu, double* out, double* h0_out) {
+    double g[6], b[3];
+    for (int a = 0; a < 6; ++a) g[a] = met[a];
+    for (int a = 0; a < 3; ++a) b[a] = met[7 + a];
+    Metric m(g, b, met[6]);
+    double cons[NCONS];
+    for (int v = 0; v < NCONS; ++v) cons[v] = cons_in[v] / m.sqrtg;
+    IdealGas eos = make_ideal(p);
+    C2PSetup cs(eos, m, cons);
+    if (h0_out) *h0_out = cs.h0;
+    if (which == 0) {
+        FRoot<IdealGas> f(eos, cs.D, cs.q, cs.r2, cs.rBt2, cs.Bt2, cs.ye, cs.h0);
+        for (int64_t i = 0; i < nmu; ++i) out[i] = f(mu[i]);
+    } else {
+        EntFRoot<IdealGas> f(eos, cs.D, cs.r2, cs.rBt2, cs.Bt2, cs.s, cs.ye, cs.h0);
+        for (int64_t i = 0; i < nmu; ++i) out[i] = f(mu[i]);
+    }
 }
 
 // Block stepper ----------------------------------------------------------------

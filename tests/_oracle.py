@@ -19,7 +19,7 @@ def olib(fast=False):
     P, d, i32, i64 = C.c_void_p, C.c_double, C.c_int, C.c_int64
     L.orc_num_threads.argtypes = [i32]
     L.orc_num_threads.restype = i32
-    L.orc_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P, P]
+    L.orc_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P, P, P]
     L.orc_p2c_batch.argtypes = [P, i64, P, P, P]
     L.orc_eos_from_temp.argtypes = [P, d, d, P, P, P, P]
     L.orc_eos_from_temp.restype = d
@@ -59,8 +59,10 @@ def arr(*vals):
     return np.array(vals, dtype=np.float64)
 
 
-def c2p_batch(params, cons, met, dry_run=0, r=None, guess=None, fast=False):
-    """cons (10,n) densitised, met (10,n).  Returns prims, cons_out, err, floored, evals, fatal."""
+def c2p_batch(params, cons, met, dry_run=0, r=None, guess=None, fast=False, roots=None):
+    """cons (10,n) densitised, met (10,n).  Returns prims, cons_out, err, floored, evals, fatal.
+    guess: (4,n) energy/entropy brackets as fractions of mu0; roots: (4,n)
+    array filled with mu0_e, t_e, mu0_s, t_s."""
     L = olib(fast)
     n = cons.shape[1]
     cons = np.ascontiguousarray(cons, dtype=np.float64).copy()
@@ -71,7 +73,7 @@ def c2p_batch(params, cons, met, dry_run=0, r=None, guess=None, fast=False):
     ev = np.zeros((n, 2), dtype=np.int64)
     fatal = np.zeros(1, dtype=np.int64)
     L.orc_c2p_batch(C.byref(params), n, p(cons), p(met), p(prims), p(err), p(fl), int(dry_run),
-                    p(r), p(guess), p(ev), p(fatal))
+                    p(r), p(guess), p(roots), p(ev), p(fatal))
     return prims, cons, err, fl, ev, int(fatal[0])
 
 

@@ -1,0 +1,193 @@
+"""GPU tests of the learned Kastaun bracket (include/grmhd_mlp.h) against the
+numpy restatement (oracle/mlp_oracle.py) and the CPU c2p oracle.
+
+Tolerances: features within 4 FP32 ulp (+1e-6 absolute; device logf vs a correctly
+rounded log);
+forward outputs within 3e-2 (max) / 1e-3 (median) in normalised units --
+the GPU accumulates in a different order and its tanh differs by an ulp,
+which can flip single bf16 roundings of hidden activations (a wrong MFMA
+operand layout or a transposed weight gives O(1)); gradients within 2e-2
+relative L2 per parameter block; learned c2p primitives within 1e-10 of the
+FP64 oracle (state-normalised, test/test_c2p.cpp:87-103).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import grace_amd as ga
+from grace_amd import workloads as W
+import _oracle as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def to_dev(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def to_host(t):
+    return t.detach().cpu().numpy()
+
+
+def single_ctx(params):
+    grid = ga.Grid(1, 1, 1, 3, 1)
+    inst = ga.make_instances([dict(nb=(1, 1, 1), q0=0, bc=(0,) * 6, idx=(1.0, 1.0, 1.0))])
+    return ga.Context(params, grid, inst)
+
+
+def problem(n, seed):
+    rng = np.random.default_rng(seed)
+    st = W.c2p_states(n, seed=seed)
+    feat, _, _, _ = M.features(st["cons"], st["met"], 1 + 1e-15)
+    p = M.init_params(seed)
+    p[M.B1:M.W2] = rng.normal(0, 0.3, M.H)
+    p[M.B2:M.W3] = rng.normal(0, 0.3, M.H)
+    p[M.B3:M.NP] = rng.normal(0, 0.3, M.NO)
+    scale = np.zeros(M.NSCALE, dtype=np.float32)
+    scale[:10] = feat.mean(1)
+    scale[10:20] = 1.0 / feat.std(1)
+    scale[20:24] = [0.2, -0.1, 1.5, 0.7]
+    target = rng.normal(0, 1, (2, n)).astype(np.float32)
+    target[1, ::5] = np.nan
+    return st, feat, p, scale, target
+
+
+def test_gpu_mlp_features(dev):
+    import torch
+    n = 5000
+    say(f"[mlp features] n={n}")
+    st = W.c2p_states(n, seed=31)
+    ctx = single_ctx(st["params"])
+    f = torch.zeros((10, n), dtype=torch.float32, device=dev)
+    ga.mlp_features(ctx, n, to_dev(st["cons"], dev), to_dev(st["met"], dev), f)
+    torch.cuda.synchronize()
+    ref = M.features(st["cons"], st["met"], 1 + 1e-15)[0]
+    got = to_host(f)
+    # FP32 logf on the device vs a correctly rounded log: a few ulp
+    ulp = np.spacing(np.abs(ref).astype(np.float32))
+    d = np.abs(got.astype(np.float64) - ref)
+    say(f"  max |diff|/ulp {np.max(d / ulp):.2f}")
+    assert np.all(d <= 4 * ulp + 1e-6)
+
+
+def test_gpu_mlp_forward_matches_oracle(dev):
+    import torch
+    n = 4099
+    say(f"[mlp forward] n={n}")
+    st, feat, p, scale, _ = problem(n, 5)
+    mlp = ga.Mlp()
+    mlp.set_params(p)
+    mlp.set_scaling(scale)
+    out = torch.zeros((2, n), dtype=torch.float32, device=dev)
+    mlp.forward(n, to_dev(feat, dev), out)
+    torch.cuda.synchronize()
+    y, _ = M.forward(p, scale, feat, quant=True)
+    ref = M.denormalise(y, scale)
+    got = to_host(out).astype(np.float64)
+    d = np.abs(got - ref) / scale[22:24, None]
+    say(f"  normalised |diff| max {d.max():.3e} median {np.median(d):.3e}; output spread {y.std():.3f}")
+    assert y.std() > 0.1
+    assert np.median(d) < 1e-3 and d.max() < 3e-2
+
+
+def test_gpu_mlp_grad_matches_oracle(dev):
+    import torch
+    n = 1000
+    say(f"[mlp grad] n={n}")
+    st, feat, p, scale, target = problem(n, 9)
+    mlp = ga.Mlp()
+    mlp.set_params(p)
+    mlp.set_scaling(scale)
+    g = torch.zeros(ga.MLP_NPARAM + 1, dtype=torch.float32, device=dev)
+    mlp.grad(n, to_dev(feat, dev), to_dev(target, dev), g)
+    torch.cuda.synchronize()
+    gg = to_host(g).astype(np.float64)
+    ref, loss = M.grad(p, scale, feat, target, quant=True)
+    say(f"  loss gpu {gg[-1]:.6e} oracle {loss:.6e}")
+    assert abs(gg[-1] - loss) < 1e-3 * loss
+    for name, a, b in (("W1", M.W1, M.B1), ("b1", M.B1, M.W2), ("W2", M.W2, M.B2), ("b2", M.B2, M.W3),
+                       ("W3", M.W3, M.B3), ("b3", M.B3, M.NP)):
+        e = np.linalg.norm(gg[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
+        say(f"  {name}: rel L2 {e:.3e}")
+        assert e < 2e-2, name
+    assert np.all(gg[M.W1 + 10 * 64:M.B1] == 0)
+
+
+def test_gpu_mlp_adam_step(dev):
+    """One train_step equals Adam (0.9, 0.999, 1e-8, bias-corrected) applied
+    to the gradient the library reports for the same batch."""
+    import torch
+    n = 512
+    st, feat, p, scale, target = problem(n, 13)
+    mlp = ga.Mlp()
+    mlp.set_params(p)
+    mlp.set_scaling(scale)
+    fd, td = to_dev(feat, dev), to_dev(target, dev)
+    g = torch.zeros(ga.MLP_NPARAM + 1, dtype=torch.float32, device=dev)
+    mlp.grad(n, fd, td, g)
+    mlp.train_step(n, fd, td, 1e-3)
+    torch.cuda.synchronize()
+    gh = to_host(g)[:-1].astype(np.float64)
+    p1 = mlp.get_params().astype(np.float64)
+    m = 0.1 * gh
+    v = 0.001 * gh * gh
+    exp = p.astype(np.float64) - 1e-3 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    exp[M.W1 + 10 * 64:M.B1] = 0.0
+    assert np.max(np.abs(p1 - exp)) < 1e-6
+
+
+def test_gpu_learned_c2p_trained(dev):
+    """Train on states the plain c2p solves (seed A), then run the fused
+    learned c2p on other states (seed B): same error bits, primitives within
+    1e-10 of the FP64 oracle, fewer master-function evaluations than plain."""
+    import torch
+    ntr, n = 1 << 17, 20000
+    say(f"[learned c2p] fit on {ntr} states")
+    tr = W.c2p_states(ntr, seed=W.SEED + 7)
+    ctx = single_ctx(tr["params"])
+    mlp = ga.Mlp()
+    rep = mlp.fit(ctx, ntr, to_dev(tr["cons"], dev), to_dev(tr["met"], dev), to_dev(tr["rad"], dev),
+                  opts=ga.Mlp.default_fit_opts(epochs=60))
+    say(f"  fit report {rep}")
+    assert rep["loss_last"] < 0.1 * rep["loss_first"]
+    assert rep["val_p50_e"] < 0.05
+    te = W.c2p_states(n, seed=W.SEED + 8)
+    res = {}
+    for mode in ("plain", "learned"):
+        cons = to_dev(te["cons"], dev)
+        met = to_dev(te["met"], dev)
+        prims = torch.zeros((13, n), dtype=torch.float64, device=dev)
+        err = torch.zeros(n, dtype=torch.int64, device=dev)
+        fl = torch.zeros(n, dtype=torch.uint8, device=dev)
+        if mode == "plain":
+            ctx.c2p_batch(n, cons, met, prims, err, fl, r=to_dev(te["rad"], dev))
+        else:
+            ga.c2p_batch_learned(ctx, mlp, n, cons, met, prims, err, fl, r=to_dev(te["rad"], dev))
+        torch.cuda.synchronize()
+        res[mode] = (to_host(prims), to_host(err).astype(np.uint64), ctx.c2p_stats())
+        say(f"  {mode}: {res[mode][2]}")
+    op, oc, oe, of, ev, fatal = O.c2p_batch(te["params"], te["cons"], te["met"], r=te["rad"])
+    from test_oracle_c2p import prim_residual
+    lp, le, ls = res["learned"]
+    pp, pe, ps = res["plain"]
+    same = le == oe
+    assert same.mean() > 0.999
+    r = prim_residual(lp, op)[same]
+    say(f"  learned vs oracle: max state-normalised diff {r.max():.3e}")
+    assert r.max() < 1e-10
+    assert ls["n_fatal"] == fatal == 0
+    assert ls["n_guess_used"] > 0.5 * n
+    plain_evals = ps["n_root_evals"] + ps["n_bracket_evals"]
+    learned_evals = ls["n_root_evals"] + ls["n_bracket_evals"]
+    say(f"  evals/state plain {plain_evals / n:.2f} learned {learned_evals / n:.2f}")
+    assert learned_evals < plain_evals

@@ -100,6 +100,50 @@ def test_gpu_c2p_batch_kerr_schild(dev):
     assert stats["n_fatal"] == fatal
 
 
+def test_gpu_c2p_guess_brackets(dev):
+    """Guessed brackets (units of 1/h0) on the GPU: the oracle's roots
+    +-1e-3 are used by > 99% of states, skip the mu+ Newton, and give the
+    plain primitives to 1e-11; the GPU roots match the oracle's."""
+    import torch
+    n = 20000
+    say(f"[c2p guess] Kerr-Schild states n={n}")
+    st = W.c2p_states(n, seed=W.SEED + 1)
+    params = st["params"]
+    grid = ga.Grid(1, 1, 1, 3, 1)
+    inst = ga.make_instances([dict(nb=(1, 1, 1), q0=0, bc=(0,) * 6, idx=(1.0, 1.0, 1.0))])
+    ctx = ga.Context(params, grid, inst)
+    oroots = np.zeros((4, n))
+    op, oc, oe, of, ev, fatal = O.c2p_batch(params, st["cons"], st["met"], roots=oroots)
+    xe, xs = oroots[0], oroots[1]
+    g = np.zeros((4, n))
+    g[0], g[1] = xe * (1 - 1e-3), np.minimum(xe * (1 + 1e-3), 1.0)
+    g[2] = np.where(xs > 0, xs * (1 - 1e-3), 0.0)
+    g[3] = np.where(xs > 0, np.minimum(xs * (1 + 1e-3), 1.0), 0.0)
+    outs = []
+    for guess in (None, g):
+        cons = to_dev(st["cons"], dev)
+        met = to_dev(st["met"], dev)
+        prims = torch.zeros((13, n), dtype=torch.float64, device=dev)
+        err = torch.zeros(n, dtype=torch.int64, device=dev)
+        fl = torch.zeros(n, dtype=torch.uint8, device=dev)
+        roots = torch.zeros((4, n), dtype=torch.float64, device=dev)
+        gd = None if guess is None else to_dev(guess, dev)
+        ctx.c2p_batch(n, cons, met, prims, err, fl, guess=gd, roots=roots)
+        torch.cuda.synchronize()
+        outs.append((to_host(prims), to_host(err).astype(np.uint64), to_host(roots), ctx.c2p_stats()))
+    (p0, e0, r0, s0), (p1, e1, r1, s1) = outs
+    say(f"  plain stats {s0}")
+    say(f"  guess stats {s1}")
+    assert np.all(np.abs(r0[0] - xe) <= 1e-11 * xe)
+    used = r1[2] == -1
+    assert used.mean() > 0.99 and s1["n_guess_used"] >= used.sum()
+    assert s1["n_bracket_evals"] < s0["n_bracket_evals"] // 50
+    from test_oracle_c2p import prim_residual
+    same = e0 == e1
+    assert same.mean() > 0.999
+    assert prim_residual(p1, p0)[same].max() < 1e-11
+
+
 def test_gpu_c2p_reference_sweeps(dev):
     """test/test_c2p.cpp sweeps through the GPU c2p (hybrid pwpoly EOS)."""
     import torch

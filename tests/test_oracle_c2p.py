@@ -153,3 +153,76 @@ def test_kastaun_root_matches_bisection():
         assert abs(f(mu_bis)) < 1e-13
         nchk += 1
     assert nchk > 150
+
+
+def _c4_states(n, seed):
+    from grace_amd.workloads import c2p_states
+    return c2p_states(n, seed=seed)
+
+
+def test_master_functions_have_one_root_below_inv_h0():
+    """The energy and entropy master functions change sign exactly once on
+    (0, 1/h0] (dense grid, log- and linear-spaced, refined near 1/h0), so a
+    sign-changing sub-bracket holds the same root as the plain solve."""
+    L = olib()
+    L.orc_master_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]
+    st = _c4_states(150, seed=11)
+    params = st["params"]
+    roots = np.zeros((4, 150))
+    c2p_batch(params, st["cons"], st["met"], r=st["rad"], roots=roots)
+    grid = np.unique(np.concatenate([np.logspace(-10, 0, 700), np.linspace(1e-4, 1, 700),
+                                     1 - np.logspace(-13, -1, 300)]))
+    out = np.zeros_like(grid)
+    h0 = C.c_double()
+    checked = 0
+    for s in range(150):
+        cons = np.ascontiguousarray(st["cons"][:, s])
+        met = np.ascontiguousarray(st["met"][:, s])
+        for which in (0, 1):
+            L.orc_master_grid(C.byref(params), p(cons), p(met), which, grid.size, p(grid), p(out), C.byref(h0))
+            mus = grid / h0.value
+            sg = np.sign(out)
+            ch = np.nonzero(sg[1:] != sg[:-1])[0]
+            assert ch.size == 1, (s, which, ch.size)
+            x = roots[which, s]
+            if x > 0:   # the plain root lies in the one sign-change cell
+                assert mus[ch[0]] * h0.value <= x * (1 + 1e-12) and x <= mus[ch[0] + 1] * h0.value * (1 + 1e-12)
+                checked += 1
+    assert checked > 150
+
+
+def test_guess_bracket_semantics():
+    """A sign-changing guessed bracket (units of 1/h0) gives the plain root to
+    the c2p tolerance and skips the mu+ Newton; a bracket that does not change
+    sign, or an invalid one, leaves the plain path bit-identical."""
+    n = 4000
+    st = _c4_states(n, seed=5)
+    params = st["params"]
+    roots = np.zeros((4, n))
+    p0, c0, e0, f0, ev0, _ = c2p_batch(params, st["cons"], st["met"], r=st["rad"], roots=roots)
+    xe, xs = roots[0], roots[1]
+    assert np.all(xe > 0) and np.all(roots[2] > 0)
+    # good brackets around the plain roots
+    g = np.zeros((4, n))
+    g[0], g[1] = xe * (1 - 1e-3), np.minimum(xe * (1 + 1e-3), 1.0)
+    ran_s = xs > 0
+    g[2] = np.where(ran_s, xs * (1 - 1e-3), 0.0)
+    g[3] = np.where(ran_s, np.minimum(xs * (1 + 1e-3), 1.0), 0.0)
+    r1 = np.zeros((4, n))
+    p1, c1, e1, f1, ev1, _ = c2p_batch(params, st["cons"], st["met"], r=st["rad"], guess=g, roots=r1)
+    used_e = r1[2] == -1
+    assert used_e.mean() > 0.99
+    assert np.all(ev1[used_e, 1] <= ev0[used_e, 1])
+    # both roots sit inside the Brent tolerance of a flat master function
+    assert np.all(np.abs(r1[0] - xe) <= 1e-11 * xe)
+    assert np.array_equal(e1, e0)
+    assert prim_residual(p1, p0).max() < 1e-11
+    # a bracket beside the root (no sign change) and an inverted one
+    for bad in (np.stack([np.minimum(xe * 1.5, 0.999), np.full(n, 1.0), np.zeros(n), np.zeros(n)]),
+                np.stack([xe * 1.001, xe * 0.999, xs * 1.001, xs * 0.999])):
+        r2 = np.zeros((4, n))
+        p2, c2, e2, f2, ev2, _ = c2p_batch(params, st["cons"], st["met"], r=st["rad"], guess=bad, roots=r2)
+        same = (bad[0] >= bad[1]) | (bad[0] > xe)
+        assert np.array_equal(p2[:, same], p0[:, same]) and np.array_equal(e2[same], e0[same])
+        assert np.all(r2[2][same] == roots[2][same])
