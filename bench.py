@@ -111,12 +111,106 @@ def cpu_baseline(wl, steps=1, threads=None):
                       f"{t:.2f}s"}
 
 
+def state_residual(a, b):
+    """max |a - b| over RHO, EPS, Z^i per state, over max(|rho|, |eps|, |z|)
+    of the reference state (the norm of test/test_c2p.cpp:87-103)."""
+    import torch
+    z = torch.sqrt(b[2] ** 2 + b[3] ** 2 + b[4] ** 2)
+    scale = torch.maximum(torch.maximum(b[0].abs(), b[7].abs()), torch.clamp(z, min=1e-30))
+    e = torch.zeros_like(scale)
+    for i in (0, 7, 2, 3, 4):
+        e = torch.maximum(e, (a[i] - b[i]).abs())
+    return e / scale
+
+
+def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
+    """Kastaun c2p alone on the 4M Kerr-Schild states: plain FP64 solve vs the
+    learned bracket (MLP trained on the GPU on 1M states of another seed);
+    bracket width and moves autotuned by timing on a 1M-state slice."""
+    log(f"[bench] c2p leg: {n} states, training on {n_train}")
+    st = W.c2p_states(n)
+    tr = W.c2p_states(n_train, seed=W.SEED + 100)
+    ctx = ga.Context(st["params"], ga.Grid(1, 1, 1, 3, 1),
+                     ga.make_instances([dict(nb=(1, 1, 1), q0=0, bc=(0,) * 6, idx=(1.0, 1.0, 1.0))]))
+    mlp = ga.Mlp()
+    t0 = time.perf_counter()
+    rep = mlp.fit(ctx, n_train, torch.from_numpy(tr["cons"]).to(dev), torch.from_numpy(tr["met"]).to(dev),
+                  torch.from_numpy(tr["rad"]).to(dev))
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    del tr
+    full = (torch.from_numpy(st["cons"]).to(dev), torch.from_numpy(st["met"]).to(dev),
+            torch.from_numpy(st["rad"]).to(dev))
+    m = n // 4
+    part = tuple(x[..., :m].contiguous() for x in full)
+    cons = torch.empty_like(full[0])
+    prims = torch.zeros((13, n), dtype=torch.float64, device=dev)
+    err = torch.zeros(n, dtype=torch.int64, device=dev)
+    fl = torch.zeros(n, dtype=torch.uint8, device=dev)
+
+    def run(learned, data, k):
+        c0, met, rad = data
+        cons_k = cons[..., :k] if k == n else torch.empty_like(c0)
+        ts = []
+        for i in range(reps + 1):
+            cons_k.copy_(c0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            if learned:
+                ga.c2p_batch_learned(ctx, mlp, k, cons_k, met, prims, err, fl, r=rad)
+            else:
+                ctx.c2p_batch(k, cons_k, met, prims, err, fl, r=rad)
+            e1.record()
+            torch.cuda.synchronize()
+            if i:
+                ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts)), ctx.c2p_stats()
+
+    de0, ds0, _ = mlp.get_bracket()
+    best = None
+    for fac in (0.5, 1.0, 2.0, 4.0):
+        for nexp in (2, 3, 4):
+            mlp.set_bracket(de0 * fac, ds0, nexp)
+            t, _ = run(True, part, m)
+            if best is None or t < best[0]:
+                best = (t, de0 * fac, nexp)
+    mlp.set_bracket(best[1], ds0, best[2])
+    t_pl, s_pl = run(False, full, n)
+    p_pl, e_pl = prims.clone(), err.clone()
+    t_ln, s_ln = run(True, full, n)
+    same = e_pl == err
+    res = state_residual(prims, p_pl)[same]
+    solves = s_ln["n_cells"] + s_ln["n_ent_backup"]
+    return {
+        "workload": "c2p_4M_kerr_schild_a0.9_r1.5-50_ideal_gamma4/3",
+        "states": n,
+        "plain": {"ms": round(t_pl, 4), "states_per_s": n / (t_pl * 1e-3),
+                  "root_evals_per_state": round(s_pl["n_root_evals"] / n, 3),
+                  "bracket_evals_per_state": round(s_pl["n_bracket_evals"] / n, 3),
+                  "max_root_evals": s_pl["max_root_evals"]},
+        "learned": {"ms": round(t_ln, 4), "states_per_s": n / (t_ln * 1e-3),
+                    "root_evals_per_state": round(s_ln["n_root_evals"] / n, 3),
+                    "bracket_evals_per_state": round(s_ln["n_bracket_evals"] / n, 3),
+                    "max_root_evals": s_ln["max_root_evals"],
+                    "fallback_rate": round(1.0 - s_ln["n_guess_used"] / max(solves, 1), 5)},
+        "speedup": round(t_pl / t_ln, 4),
+        "error_vs_plain": {"max_state_rel": float(res.max()), "median_state_rel": float(res.median()),
+                           "same_error_bits": float(same.double().mean())},
+        "bracket": {"delta_e": best[1], "delta_s": ds0, "nexp": best[2]},
+        "train": {"states": n_train, "seconds": round(fit_s, 3), "steps": rep["steps"],
+                  "val_p50_log_err": rep["val_p50_e"], "val_p99_log_err": rep["val_p99_e"],
+                  "loss_first": rep["loss_first"], "loss_last": rep["loss_last"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-c2p", action="store_true", help="skip the learned-vs-plain c2p leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -205,6 +299,10 @@ def main():
         roof["fp64"] = {"achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(tf / FP64_PEAK_TFLOPS, 4), "source": "rocprofv3 SQ_INSTS_VALU_*_F64"}
 
+    c2p_leg = None
+    if not args.no_c2p:
+        c2p_leg = c4_leg(ga, W, torch, dev)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -231,6 +329,7 @@ def main():
                    "graph": use_graph, "gpu_event_ms_per_step": gpu_ms / args.steps},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "c2p_learned_vs_plain": c2p_leg,
     }
     print(json.dumps(out), flush=True)
 

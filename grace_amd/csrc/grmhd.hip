@@ -15,6 +15,7 @@
 //   k_c2p_batch    c2p over a flat batch (c2p.cpp:188-414)
 //   k_dt_*         CFL reduction (find_stable_timestep.cpp:72-126)
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <cstdarg>
@@ -109,6 +110,18 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
     int j = (int)((r / R[0]) % R[1]);
     int k = (int)(r / ((int64_t)R[0] * R[1]));
     if (D == 0) i += G.ng; else if (D == 1) j += G.ng; else k += G.ng;
+    // UCT needs the full fluxes only on faces with both transverse indices
+    // interior; faces with one transverse index in the ghost layers feed the
+    // transverse reconstructions of the edge EMFs (vbar, wave speeds) only,
+    // and faces with both in the ghost layers are not read at all.
+    bool ring = false;
+    if (o.emf == 0) {
+        const int a0 = D == 0 ? j : i, a1 = D == 2 ? j : k;
+        const int n0 = D == 0 ? G.n[1] : G.n[0], n1 = D == 2 ? G.n[1] : G.n[2];
+        const bool g0 = a0 < G.ng || a0 >= n0 + G.ng, g1 = a1 < G.ng || a1 >= n1 + G.ng;
+        if (g0 && g1) return;
+        ring = g0 || g1;
+    }
     const int64_t nc = G.ncell;
     const double* A = aux + q * NAUX * nc;
     const int64_t c0 = cidx(G, i, j, k);
@@ -192,6 +205,7 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
         V[0] = hll(vtl[J0], vtr[J0], 0, 0, cmin, cmax);
         V[nf] = hll(vtl[J1], vtr[J1], 0, 0, cmin, cmax);
         V[2 * nf] = cmin; V[3 * nf] = cmax;
+        if (ring) return;
     }
     double Dl, taul, Sl[3], Esl, Dr, taur, Sr[3], Esr;
     conserved(mf, pL[RHOL], pL[PRESSL], pL[EPSL], zl, pL[ENTL], wl, b2l, bl, Dl, taul, Sl, Esl);
@@ -582,8 +596,11 @@ GHD bool ghost_needs_own_c2p(const DevGrid& G, const DevInst& I, const DevBlk& b
     return false;
 }
 
-// Ghost gather over precomputed ghost-shell tables (local offsets shared by
-// all blocks): entries [0,ncg) are cells, then x, y, z faces.
+// Ghost gather over precomputed tables.  The ghost shell of a block is a
+// list of local offsets shared by all blocks (entries [0,ncg) cells, then x,
+// y, z faces); src[q][e] holds the source block and local offset of entry e
+// of block q, found once on the host by the same global-index mapping.  Cell
+// entries whose ghost needs its own c2p carry GSRC_OWN in the block field.
 // cell_mode: 0 skip cells, 1 every ghost cell, 2 ghost cells that need their
 // own c2p, 3 ghost cells that take copies of c2p outputs.
 struct GatherArgs {
@@ -593,30 +610,11 @@ struct GatherArgs {
     double* B[3];                            // face arrays (null: skip faces)
     int cell_mode;
 };
-__global__ __launch_bounds__(256) void k_gather(DevGrid G, const DevInst* __restrict__ inst,
-                                                const DevBlk* __restrict__ blk, const int32_t* __restrict__ list,
-                                                int64_t ncg, int64_t nfg0, int64_t nfg1, int64_t nfg2, GatherArgs a) {
-    const int64_t per = ncg + (a.B[0] ? nfg0 + nfg1 + nfg2 : 0);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * G.nq) return;
-    int64_t q = t / per, e = t - q * per;
-    int fd = -1;
-    int64_t off;
-    if (e < ncg) { off = list[e]; }
-    else {
-        int64_t r = e - ncg;
-        if (r < nfg0) { fd = 0; } else if (r < nfg0 + nfg1) { fd = 1; r -= nfg0; } else { fd = 2; r -= nfg0 + nfg1; }
-        off = list[ncg + (fd >= 1 ? nfg0 : 0) + (fd == 2 ? nfg1 : 0) + r];
-    }
-    if (fd < 0 && a.cell_mode == 0) return;
-    int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
-    int loc[3] = {(int)(off % E0), (int)((off / E0) % E1), (int)(off / ((int64_t)E0 * E1))};
-    const DevBlk b = blk[q];
-    const DevInst& I = inst[b.inst];
-    if (fd < 0 && a.cell_mode >= 2) {
-        bool own = ghost_needs_own_c2p(G, I, b, loc[0], loc[1], loc[2]);
-        if ((a.cell_mode == 2) != own) return;
-    }
+constexpr int GSRC_OWN = 1 << 30;
+
+// Source of ghost element `loc` (face direction fd, -1 for cells) of block b.
+static void ghost_source(const DevGrid& G, const DevInst& I, const DevBlk& b, int fd, const int loc[3], int& qs,
+                         int& off) {
     int sb[3], sl[3];
     for (int d = 0; d < 3; ++d) {
         int Nt = I.nb[d] * G.n[d];
@@ -630,21 +628,40 @@ __global__ __launch_bounds__(256) void k_gather(DevGrid G, const DevInst* __rest
             sb[d] = m / G.n[d]; sl[d] = m % G.n[d] + G.ng;
         }
     }
-    int64_t qs = I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]);
-    if (fd < 0) {
-        const int64_t cs = cidx(G, sl[0], sl[1], sl[2]);
+    qs = (int)(I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]));
+    off = (int)(fd < 0 ? cidx(G, sl[0], sl[1], sl[2]) : fidx_d(G, fd, sl[0], sl[1], sl[2]));
+}
+
+// grid: x over the entries of one block (cells, or all faces), y over blocks
+__global__ __launch_bounds__(256) void k_gather(DevGrid G, const int32_t* __restrict__ list,
+                                                const int2* __restrict__ src, int per, int e0, int ne, int ncg,
+                                                int nfg0, int nfg1, int qbase, GatherArgs a) {
+    const int e = e0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= e0 + ne) return;
+    const int q = qbase + (int)blockIdx.y;
+    const int off = list[e];
+    const int2 sr = src[(int64_t)q * per + e];
+    if (e < ncg) {
+        const bool own = (sr.x & GSRC_OWN) != 0;
+        if (a.cell_mode >= 2 && (a.cell_mode == 2) != own) return;
+        const int64_t qs = sr.x & (GSRC_OWN - 1);
         const int64_t nc = G.ncell;
+        const int cs = sr.y;
         if (a.st)
-            for (int v = a.v0; v < a.v0 + a.vc; ++v) a.st[(q * a.nv + v) * nc + off] = a.st[(qs * a.nv + v) * nc + cs];
+            for (int v = a.v0; v < a.v0 + a.vc; ++v)
+                a.st[((int64_t)q * a.nv + v) * nc + off] = a.st[(qs * a.nv + v) * nc + cs];
         if (a.aux)
-            for (int v = 0; v < NAUX; ++v)
-                if (a.aux_mask & (1u << v)) a.aux[(q * NAUX + v) * nc + off] = a.aux[(qs * NAUX + v) * nc + cs];
+            for (uint32_t m = a.aux_mask; m; m &= m - 1) {
+                const int v = __builtin_ctz(m);
+                a.aux[((int64_t)q * NAUX + v) * nc + off] = a.aux[(qs * NAUX + v) * nc + cs];
+            }
         if (a.ecen)
-            for (int v = 0; v < 3; ++v) a.ecen[(q * 3 + v) * nc + off] = a.ecen[(qs * 3 + v) * nc + cs];
+            for (int v = 0; v < 3; ++v) a.ecen[((int64_t)q * 3 + v) * nc + off] = a.ecen[(qs * 3 + v) * nc + cs];
     } else {
+        const int fd = e < ncg + nfg0 ? 0 : (e < ncg + nfg0 + nfg1 ? 1 : 2);
         double* Bd = a.B[fd];
-        int64_t nf = G.nface[fd];
-        Bd[q * nf + off] = Bd[qs * nf + fidx_d(G, fd, sl[0], sl[1], sl[2])];
+        const int64_t nf = G.nface[fd];
+        Bd[(int64_t)q * nf + off] = Bd[(int64_t)sr.x * nf + sr.y];
     }
 }
 
@@ -709,20 +726,45 @@ __global__ __launch_bounds__(256) void k_stats_final(const DevStats* __restrict_
 // ---------------------------------------------------------------------------
 // c2p over cells (compute_auxiliaries)
 // ---------------------------------------------------------------------------
+// Minimum resident 256-thread blocks per CU for the c2p kernels (caps their
+// registers: 2 -> 256 VGPRs, 3 -> 168, 4 -> 128 per lane).
+#ifndef GRMHD_C2P_MINB
+#define GRMHD_C2P_MINB 3
+#endif
+// Cell sets: CS_INTERIOR (grid x over the interior cells of a block, y over
+// blocks), CS_SHELL (x over the ghost-shell list, y over blocks), CS_LIST
+// (x over explicit {block, cell} pairs).
+enum { CS_INTERIOR = 0, CS_SHELL = 1, CS_LIST = 2 };
 template <typename EOS, bool FLAT>
-__global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, const DevInst* __restrict__ inst,
+__global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EOS eos, Opts o, const DevInst* __restrict__ inst,
                                                    const DevBlk* __restrict__ blk, double* st, int snv,
                                                    const double* __restrict__ mstate, int mnv,
                                                    const double* __restrict__ Bx, const double* __restrict__ By,
                                                    const double* __restrict__ Bz, double* aux, double* ecen,
-                                                   DevStats* stats, int selective) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = t < G.ncell * G.nq;
-    int64_t tt = active ? t : 0;
-    int64_t q = tt / G.ncell, c = tt - q * G.ncell;
-    int i = (int)(c % G.N[0]), j = (int)((c / G.N[0]) % G.N[1]), k = (int)(c / ((int64_t)G.N[0] * G.N[1]));
-    if (selective && active && !cell_is_interior(G, i, j, k))
-        active = ghost_needs_own_c2p(G, inst[blk[q].inst], blk[q], i, j, k);
+                                                   DevStats* stats, int cset, const int32_t* __restrict__ shell,
+                                                   const int2* __restrict__ cl, int ne, int qbase) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    bool active = e < ne;
+    const int ee = active ? e : 0;
+    int64_t q;
+    int64_t c;
+    int i, j, k;
+    if (cset == CS_INTERIOR) {
+        q = qbase + (int)blockIdx.y;
+        const int n0 = G.n[0], n01 = G.n[0] * G.n[1];
+        k = ee / n01; const int r = ee - k * n01;
+        j = r / n0; i = r - j * n0;
+        i += G.ng; j += G.ng; k += G.ng;
+        c = cidx(G, i, j, k);
+    } else {
+        int off;
+        if (cset == CS_SHELL) { q = qbase + (int)blockIdx.y; off = shell[ee]; }
+        else { const int2 p = cl[ee]; q = p.x; off = p.y; }
+        c = off;
+        const int N0 = G.N[0], N01 = G.N[0] * G.N[1];
+        k = off / N01; const int r = off - k * N01;
+        j = r / N0; i = r - j * N0;
+    }
     const int64_t nc = G.ncell;
     Met m;
     if constexpr (FLAT) m.flat();
@@ -770,7 +812,7 @@ __global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, c
         }
     }
     if (!active) { cnt = C2PCount(); err = 0; }
-    if (stats) stats_block(stats, active, active && out.fatal, err, cnt);
+    if (stats) stats_block(stats + (int64_t)blockIdx.y * gridDim.x, active, active && out.fatal, err, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -779,7 +821,7 @@ __global__ __launch_bounds__(256) void k_c2p_cells(DevGrid G, EOS eos, Opts o, c
 // LEARNED: the brackets come from the MLP (features -> BF16 MFMA forward in
 // LDS/registers, one pass per wave) instead of the guess array.
 template <typename EOS, bool LEARNED>
-__global__ __launch_bounds__(256, 2) void k_c2p_batch(int64_t n, EOS eos, Opts o, double* cons_soa,
+__global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_batch(int64_t n, EOS eos, Opts o, double* cons_soa,
                                                    const double* __restrict__ met, double* prims,
                                                    uint64_t* errb, uint8_t* floored, int dry_run,
                                                    const double* __restrict__ rad, const double* __restrict__ guess,
@@ -943,6 +985,9 @@ struct grmhd_ctx {
     DevStats* stats_part = nullptr;
     int64_t n_part_cap = 0;
     int32_t* ghost_list = nullptr;
+    int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
+    int2* own_cells = nullptr;                // {block, cell} of ghosts that run their own c2p
+    int64_t n_own = 0;
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
     int64_t last_c2p_n = 0;
     bool timing = false;
@@ -1018,7 +1063,7 @@ static void build_eos(grmhd_ctx* c) {
 
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list);
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1110,7 +1155,7 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
     if ((e = hipMalloc(&c->dtpart, sizeof(double) * c->n_dtpart)) != hipSuccess) return fail("dtpart", e);
     if ((e = hipMalloc(&c->stats, sizeof(DevStats))) != hipSuccess) return fail("stats", e);
     if ((e = hipMemset(c->stats, 0, sizeof(DevStats))) != hipSuccess) return fail("stats clear", e);
-    c->n_part_cap = nblk(G.ncell * G.nq);
+    c->n_part_cap = nblk(G.ncell * G.nq) + 2 * G.nq + 2;   // per-launch rounding of the 2-D c2p grids
     if ((e = hipMalloc(&c->stats_part, sizeof(DevStats) * c->n_part_cap)) != hipSuccess) return fail("stats part", e);
     {   // ghost-shell tables (local offsets), cells then x/y/z faces
         std::vector<int32_t> lst;
@@ -1132,6 +1177,36 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
         if ((e = hipMalloc(&c->ghost_list, sizeof(int32_t) * lst.size())) != hipSuccess) return fail("ghost list", e);
         if ((e = hipMemcpy(c->ghost_list, lst.data(), sizeof(int32_t) * lst.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail("ghost list copy", e);
+        // per-block sources of every ghost element
+        const int64_t per = (int64_t)lst.size();
+        std::vector<int2> src((size_t)(per * G.nq));
+        for (int64_t q = 0; q < G.nq; ++q) {
+            const DevBlk& b = blk[q];
+            const DevInst& I = c->inst_h[b.inst];
+            for (int64_t x = 0; x < per; ++x) {
+                int fd = x < c->ncg ? -1 : (x < c->ncg + c->nfg[0] ? 0 : (x < c->ncg + c->nfg[0] + c->nfg[1] ? 1 : 2));
+                int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
+                int off = lst[x];
+                int loc[3] = {off % E0, (off / E0) % E1, off / (E0 * E1)};
+                int qs, so;
+                ghost_source(G, I, b, fd, loc, qs, so);
+                if (fd < 0 && ghost_needs_own_c2p(G, I, b, loc[0], loc[1], loc[2])) qs |= GSRC_OWN;
+                src[(size_t)(q * per + x)] = make_int2(qs, so);
+            }
+        }
+        std::vector<int2> own;
+        for (int64_t q = 0; q < G.nq; ++q)
+            for (int64_t x = 0; x < c->ncg; ++x)
+                if (src[(size_t)(q * per + x)].x & GSRC_OWN) own.push_back(make_int2((int)q, lst[x]));
+        c->n_own = (int64_t)own.size();
+        if (!own.empty()) {
+            if ((e = hipMalloc(&c->own_cells, sizeof(int2) * own.size())) != hipSuccess) return fail("own cells", e);
+            if ((e = hipMemcpy(c->own_cells, own.data(), sizeof(int2) * own.size(), hipMemcpyHostToDevice)) != hipSuccess)
+                return fail("own cells copy", e);
+        }
+        if ((e = hipMalloc(&c->ghost_src, sizeof(int2) * src.size())) != hipSuccess) return fail("ghost sources", e);
+        if ((e = hipMemcpy(c->ghost_src, src.data(), sizeof(int2) * src.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail("ghost sources copy", e);
     }
     *out = c;
     return GRMHD_OK;
@@ -1221,26 +1296,46 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
 static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     Scope sc(c, "ghost", s);
     const DevGrid& G = c->G;
-    int64_t per = c->ncg + (a.B[0] ? c->nfg[0] + c->nfg[1] + c->nfg[2] : 0);
-    k_gather<<<nblk(per * G.nq), 256, 0, s>>>(G, c->inst_d, c->blk_d, c->ghost_list, c->ncg, c->nfg[0], c->nfg[1],
-                                              c->nfg[2], a);
+    const int per = (int)(c->ncg + c->nfg[0] + c->nfg[1] + c->nfg[2]);
+    const bool cells = a.cell_mode != 0;
+    const bool faces = a.B[0] != nullptr;
+    if (!cells && !faces) return;
+    const int e0 = cells ? 0 : (int)c->ncg;
+    const int ne = (faces ? per : (int)c->ncg) - e0;
+    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {   // grid.y is limited to 65535 blocks per launch
+        dim3 grid(nblk(ne), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+        k_gather<<<grid, 256, 0, s>>>(G, c->ghost_list, c->ghost_src, per, e0, ne, (int)c->ncg, (int)c->nfg[0],
+                                      (int)c->nfg[1], (int)q0, a);
+    }
 }
 
+// c2p over the cells of every block: the interior cells, plus either every
+// ghost cell (all) or only the ghost cells that need their own c2p.
 static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms, int mnv, const double* const B[3],
                              double* aux, bool with_stats, bool selective, hipStream_t s) {
     Scope sc(c, "c2p", s);
     const DevGrid& G = c->G;
-    int64_t n = G.ncell * G.nq;
     double* ec = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
-    DevStats* part = with_stats ? c->stats_part : nullptr;
-    bool flat = c->P.flat_metric != 0;
-#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<nblk(n), 256, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, selective ? 1 : 0)
-    if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
-    else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
+    const bool flat = c->P.flat_metric != 0;
+    const int ni = G.n[0] * G.n[1] * G.n[2];
+    int64_t npart = 0;
+    auto launch = [&](int cset, int ne, dim3 grid, int qbase) {
+        DevStats* part = with_stats ? c->stats_part + npart : nullptr;
+        npart += (int64_t)grid.x * grid.y;
+#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<grid, 256, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, qbase)
+        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
+        else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
 #undef C2PL
+    };
+    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
+        const unsigned nqq = (unsigned)std::min<int64_t>(65535, G.nq - q0);
+        launch(CS_INTERIOR, ni, dim3(nblk(ni), nqq), (int)q0);
+        if (!selective) launch(CS_SHELL, (int)c->ncg, dim3(nblk(c->ncg), nqq), (int)q0);
+    }
+    if (selective && c->n_own > 0) launch(CS_LIST, (int)c->n_own, dim3(nblk(c->n_own), 1), 0);
     if (with_stats) {
-        k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
-        c->last_c2p_n = n;
+        k_stats_final<<<1, 256, 0, s>>>(c->stats_part, npart, c->stats);
+        c->last_c2p_n = G.ncell * G.nq;
     }
 }
 
