@@ -97,30 +97,59 @@ template <typename EOS, int REC, bool FLAT, int D>
 __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
                                               const double* __restrict__ mstate, int mnv,
                                               const double* __restrict__ Bd, double* __restrict__ flux,
-                                              double* __restrict__ vbar) {
-    // face range: normal [ng, n+ng], transverse [0, N)
-    int R[3] = {G.N[0], G.N[1], G.N[2]};
-    R[D] = G.n[D] + 1;
-    int64_t per = (int64_t)R[0] * R[1] * R[2];
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * G.nq) return;
-    int64_t q = t / per;
-    int64_t r = t - q * per;
-    int i = (int)(r % R[0]);
-    int j = (int)((r / R[0]) % R[1]);
-    int k = (int)(r / ((int64_t)R[0] * R[1]));
-    if (D == 0) i += G.ng; else if (D == 1) j += G.ng; else k += G.ng;
-    // UCT needs the full fluxes only on faces with both transverse indices
-    // interior; faces with one transverse index in the ghost layers feed the
-    // transverse reconstructions of the edge EMFs (vbar, wave speeds) only,
-    // and faces with both in the ghost layers are not read at all.
+                                              double* __restrict__ vbar, int qbase) {
+    // One workgroup row per block (q = blockIdx.y).  UCT: the first
+    // (n_D+1) n_T0 n_T1 threads take the faces whose transverse indices are
+    // both interior (full fluxes), the next ones the faces with exactly one
+    // transverse index in the ghost layers ("ring": they feed only the
+    // transverse reconstructions of the edge EMFs: vbar and wave speeds);
+    // faces with both transverse indices in the ghost layers are never read
+    // and get no thread.  GS reads fluxes and face EMFs one layer into the
+    // transverse ghosts, so it takes every face (normal [ng, n+ng],
+    // transverse [0, N)).
+    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int i, j, k;
     bool ring = false;
-    if (o.emf == 0) {
-        const int a0 = D == 0 ? j : i, a1 = D == 2 ? j : k;
-        const int n0 = D == 0 ? G.n[1] : G.n[0], n1 = D == 2 ? G.n[1] : G.n[2];
-        const bool g0 = a0 < G.ng || a0 >= n0 + G.ng, g1 = a1 < G.ng || a1 >= n1 + G.ng;
-        if (g0 && g1) return;
-        ring = g0 || g1;
+    {
+        const int ng = G.ng, Rn = G.n[D] + 1;
+        constexpr int T0 = D == 0 ? 1 : 0, T1 = D == 2 ? 1 : 2;
+        const int n0 = G.n[T0], n1 = G.n[T1];
+        int a = 0, b = 0, c = 0;   // (normal, t0, t1)
+        if (o.emf != 0) {
+            const int N0 = G.N[T0], N1 = G.N[T1];
+            if (e >= Rn * N0 * N1) return;
+            if (D == 0) { a = e % Rn; const int r = e / Rn; b = r % N0; c = r / N0; }
+            else if (D == 1) { b = e % N0; const int r = e / N0; a = r % Rn; c = r / Rn; }
+            else { b = e % N0; const int r = e / N0; c = r % N1; a = r / N1; }
+        } else {
+            const int nint = Rn * n0 * n1, g2 = 2 * ng;
+            const int nA = Rn * g2 * n1, nB = Rn * n0 * g2;
+            auto ghost = [&](int g, int nn) { return g < ng ? g : nn + g; };
+            if (e < nint) {
+                if (D == 0) { a = e % Rn; const int r = e / Rn; b = ng + r % n0; c = ng + r / n0; }
+                else if (D == 1) { b = ng + e % n0; const int r = e / n0; a = r % Rn; c = ng + r / Rn; }
+                else { b = ng + e % n0; const int r = e / n0; c = ng + r % n1; a = r / n1; }
+            } else if (e < nint + nA) {          // t0 ghost, t1 interior
+                const int x = e - nint;
+                ring = true;
+                if (D == 0) { a = x % Rn; const int r = x / Rn; b = ghost(r % g2, n0); c = ng + r / g2; }
+                else if (D == 1) { b = ghost(x % g2, n0); const int r = x / g2; a = r % Rn; c = ng + r / Rn; }
+                else { b = ghost(x % g2, n0); const int r = x / g2; c = ng + r % n1; a = r / n1; }
+            } else if (e < nint + nA + nB) {     // t0 interior, t1 ghost
+                const int x = e - nint - nA;
+                ring = true;
+                if (D == 0) { a = x % Rn; const int r = x / Rn; b = ng + r % n0; c = ghost(r / n0, n1); }
+                else if (D == 1) { b = ng + x % n0; const int r = x / n0; a = r % Rn; c = ghost(r / Rn, n1); }
+                else { b = ng + x % n0; const int r = x / n0; c = ghost(r % g2, n1); a = r / g2; }
+            } else {
+                return;
+            }
+        }
+        a += ng;
+        if (D == 0) { i = a; j = b; k = c; }
+        else if (D == 1) { i = b; j = a; k = c; }
+        else { i = b; j = c; k = a; }
     }
     const int64_t nc = G.ncell;
     const double* A = aux + q * NAUX * nc;
@@ -285,13 +314,14 @@ template <int REC>
 __global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __restrict__ Bx,
                                                  const double* __restrict__ By, const double* __restrict__ Bz,
                                                  const double* __restrict__ vx, const double* __restrict__ vy,
-                                                 const double* __restrict__ vz, double* __restrict__ emf) {
-    int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
-    int64_t per = (int64_t)R0 * R1 * R2;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * G.nq) return;
-    int64_t q = t / per, r = t - q * per;
-    int i = G.ng + (int)(r % R0), j = G.ng + (int)((r / R0) % R1), k = G.ng + (int)(r / ((int64_t)R0 * R1));
+                                                 const double* __restrict__ vz, double* __restrict__ emf, int qbase) {
+    // x: the (n+1)^3 sites of one block, y: blocks
+    const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= R0 * R1 * R2) return;
+    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
+    const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2];
     const double* bx = Bx + q * nfx; const double* by = By + q * nfy; const double* bz = Bz + q * nfz;
     const double* Vx = vx + q * 4 * nfx; const double* Vy = vy + q * 4 * nfy; const double* Vz = vz + q * 4 * nfz;
@@ -347,13 +377,14 @@ __global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __rest
 __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restrict__ ef0, const double* __restrict__ ef1,
                                                 const double* __restrict__ ef2, const double* __restrict__ ec,
                                                 const double* __restrict__ f0, const double* __restrict__ f1,
-                                                const double* __restrict__ f2, double* __restrict__ emf) {
-    int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
-    int64_t per = (int64_t)R0 * R1 * R2;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * G.nq) return;
-    int64_t q = t / per, r = t - q * per;
-    int i = G.ng + (int)(r % R0), j = G.ng + (int)((r / R0) % R1), k = G.ng + (int)(r / ((int64_t)R0 * R1));
+                                                const double* __restrict__ f2, double* __restrict__ emf, int qbase) {
+    // x: the (n+1)^3 sites of one block, y: blocks
+    const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= R0 * R1 * R2) return;
+    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
+    const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2], nc = G.ncell;
     auto EF = [&](int d, int c, int ii, int jj, int kk) {
         const double* base = d == 0 ? ef0 : (d == 1 ? ef1 : ef2);
@@ -479,13 +510,14 @@ GHD void geom_sources(const Met& m, const double K[6], const double da[3], const
 }
 
 template <bool FLAT>
-__global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order) {
-    int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
-    int64_t per = (int64_t)R0 * R1 * R2;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= per * G.nq) return;
-    int64_t q = t / per, r = t - q * per;
-    int i = G.ng + (int)(r % R0), j = G.ng + (int)((r / R0) % R1), k = G.ng + (int)(r / ((int64_t)R0 * R1));
+__global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order, int qbase) {
+    // x: the (n+1)^3 sites of one block, y: blocks
+    const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= R0 * R1 * R2) return;
+    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
+    const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
     const DevInst& I = a.inst[a.blk[q].inst];
     const double ix = I.idx[0], iy = I.idx[1], iz = I.idx[2];
     const double dtf = (*a.dt) * a.dtfact;
@@ -1230,20 +1262,22 @@ template <typename EOS, int REC, bool FLAT>
 static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
                          const double* const B[3], hipStream_t s) {
     const DevGrid& G = c->G;
-    int64_t nx = (int64_t)(G.n[0] + 1) * G.N[1] * G.N[2] * G.nq;
-    int64_t ny = (int64_t)G.N[0] * (G.n[1] + 1) * G.N[2] * G.nq;
-    int64_t nz = (int64_t)G.N[0] * G.N[1] * (G.n[2] + 1) * G.nq;
-    {
-        Scope sc(c, "flux_x", s);
-        k_flux<EOS, REC, FLAT, 0><<<nblk(nx), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0]);
+    const bool uct = c->P.emf != GRMHD_EMF_GS;
+    int nf[3];
+    for (int d = 0; d < 3; ++d) {
+        const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+        const int Rn = G.n[d] + 1, n0 = G.n[t0], n1 = G.n[t1];
+        nf[d] = uct ? Rn * (n0 * n1 + 2 * G.ng * (n0 + n1)) : Rn * G.N[t0] * G.N[t1];
     }
-    {
-        Scope sc(c, "flux_y", s);
-        k_flux<EOS, REC, FLAT, 1><<<nblk(ny), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1]);
-    }
-    {
-        Scope sc(c, "flux_z", s);
-        k_flux<EOS, REC, FLAT, 2><<<nblk(nz), 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2]);
+    static const char* fam[3] = {"flux_x", "flux_y", "flux_z"};
+    for (int d = 0; d < 3; ++d) {
+        Scope sc(c, fam[d], s);
+        for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
+            dim3 grid(nblk(nf[d]), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+            if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], (int)q0);
+            else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], (int)q0);
+            else k_flux<EOS, REC, FLAT, 2><<<grid, 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], (int)q0);
+        }
     }
 }
 template <typename EOS, bool FLAT>
@@ -1267,21 +1301,32 @@ static void launch_fluxes(grmhd_ctx* c, const double* aux, const double* ms, int
         else flux_rec<EosHybrid, false>(c, c->hybrid, aux, ms, mnv, B, s);
     }
 }
+// grids of the per-site kernels: x over the (n+1)^3 sites of a block, y over
+// blocks (chunks of 65535)
+template <typename F>
+static void site_launch(const DevGrid& G, F&& f) {
+    const int sites = (G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1);
+    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535)
+        f(dim3(nblk(sites), (unsigned)std::min<int64_t>(65535, G.nq - q0)), (int)q0);
+}
 static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
     Scope sc(c, "emf", s);
     const DevGrid& G = c->G;
-    int64_t n = (int64_t)(G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1) * G.nq;
     if (c->P.emf == GRMHD_EMF_GS) {
-        k_emf_gs<<<nblk(n), 256, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
-                                         c->flux[2], c->emf);
+        site_launch(G, [&](dim3 g, int q0) {
+            k_emf_gs<<<g, 256, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
+                                       c->flux[2], c->emf, q0);
+        });
         return;
     }
-    switch (c->P.recon) {
-        case 0: k_emf_uct<0><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
-        case 1: k_emf_uct<1><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
-        case 2: k_emf_uct<2><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
-        default: k_emf_uct<3><<<nblk(n), 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf); break;
-    }
+    site_launch(G, [&](dim3 g, int q0) {
+        switch (c->P.recon) {
+            case 0: k_emf_uct<0><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            case 1: k_emf_uct<1><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            case 2: k_emf_uct<2><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            default: k_emf_uct<3><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+        }
+    });
 }
 static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     Scope sc(c, "update", s);
@@ -1289,9 +1334,10 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     a.inst = c->inst_d; a.blk = c->blk_d;
     for (int d = 0; d < 3; ++d) a.flux[d] = c->flux[d];
     a.emf = c->emf;
-    int64_t n = (int64_t)(G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1) * G.nq;
-    if (c->P.flat_metric) k_update<true><<<nblk(n), 256, 0, s>>>(G, a, c->P.fd_order);
-    else k_update<false><<<nblk(n), 256, 0, s>>>(G, a, c->P.fd_order);
+    site_launch(G, [&](dim3 g, int q0) {
+        if (c->P.flat_metric) k_update<true><<<g, 256, 0, s>>>(G, a, c->P.fd_order, q0);
+        else k_update<false><<<g, 256, 0, s>>>(G, a, c->P.fd_order, q0);
+    });
 }
 static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     Scope sc(c, "ghost", s);

@@ -816,27 +816,36 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
 
 // ---- reconstruction ----------------------------------------------------------
 // u0..u5 = u(i-3..i+2); uL at i-1/2-eps, uR at i-1/2+eps
+// WENO-Z (weno_reconstruction.hh:172-278).  The three ratios t5/(b_k+eps)
+// of each side come from one division, t5/(d0 d1 d2) times the other two
+// d's (d_k >= 1e-42, so the product cannot underflow); this differs from the
+// reference's three divisions by a few ulp.
+GHD void wenoz_side(double b0, double b1, double b2, double p0, double p1, double p2, double w0, double w1,
+                    double w2, double& u) {
+    const double eps = 1.0e-42;
+    const double d0 = b0 + eps, d1 = b1 + eps, d2 = b2 + eps;
+    const double d01 = d0 * d1;
+    const double r = fabs(b0 - b2) / (d01 * d2);
+    const double q0 = d1 * d2 * r, q1 = d0 * d2 * r, q2 = d01 * r;
+    const double a0 = w0 * (1.0 + q0 * q0), a1 = w1 * (1.0 + q1 * q1), a2 = w2 * (1.0 + q2 * q2);
+    u = (p0 * a0 + p1 * a1 + p2 * a2) / (6.0 * (a0 + a1 + a2));
+}
 GHD void wenoz(double um3, double um2, double um1, double u0, double up1, double up2, double& uL, double& uR) {
-    const double c0 = 13.0 / 12.0, c1 = 0.25, eps = 1.0e-42;
+    const double c0 = 13.0 / 12.0, c1 = 0.25;
     {
         double b0 = c0 * sq(um2 + u0 - 2.0 * um1) + c1 * sq(um2 + 3.0 * u0 - 4.0 * um1);
         double b1 = c0 * sq(um1 + up1 - 2.0 * u0) + c1 * sq(um1 - up1);
         double b2 = c0 * sq(up2 + u0 - 2.0 * up1) + c1 * sq(up2 + 3.0 * u0 - 4.0 * up1);
-        double t5 = fabs(b0 - b2);
-        double i0 = sq(t5 / (b0 + eps)), i1 = sq(t5 / (b1 + eps)), i2 = sq(t5 / (b2 + eps));
-        double a0 = 0.1 * (1.0 + i2), a1 = 0.6 * (1.0 + i1), a2 = 0.3 * (1.0 + i0);
-        uR = ((2.0 * up2 - 7.0 * up1 + 11.0 * u0) * a0 + (-1.0 * up1 + 5.0 * u0 + 2.0 * um1) * a1
-              + (2.0 * u0 + 5.0 * um1 - um2) * a2) / (6.0 * (a0 + a1 + a2));
+        // a0 pairs with i2, a2 with i0 on this side
+        wenoz_side(b2, b1, b0, (2.0 * up2 - 7.0 * up1 + 11.0 * u0), (-1.0 * up1 + 5.0 * u0 + 2.0 * um1),
+                   (2.0 * u0 + 5.0 * um1 - um2), 0.1, 0.6, 0.3, uR);
     }
     {
         double b0 = c0 * sq(um3 + um1 - 2.0 * um2) + c1 * sq(um3 + 3.0 * um1 - 4.0 * um2);
         double b1 = c0 * sq(um2 + u0 - 2.0 * um1) + c1 * sq(um2 - u0);
         double b2 = c0 * sq(up1 + um1 - 2.0 * u0) + c1 * sq(up1 + 3.0 * um1 - 4.0 * u0);
-        double t5 = fabs(b0 - b2);
-        double i0 = sq(t5 / (b0 + eps)), i1 = sq(t5 / (b1 + eps)), i2 = sq(t5 / (b2 + eps));
-        double a0 = 0.1 * (1.0 + i0), a1 = 0.6 * (1.0 + i1), a2 = 0.3 * (1.0 + i2);
-        uL = ((2.0 * um3 - 7.0 * um2 + 11.0 * um1) * a0 + (-1.0 * um2 + 5.0 * um1 + 2.0 * u0) * a1
-              + (2.0 * um1 + 5.0 * u0 - up1) * a2) / (6.0 * (a0 + a1 + a2));
+        wenoz_side(b0, b1, b2, (2.0 * um3 - 7.0 * um2 + 11.0 * um1), (-1.0 * um2 + 5.0 * um1 + 2.0 * u0),
+                   (2.0 * um1 + 5.0 * u0 - up1), 0.1, 0.6, 0.3, uL);
     }
 }
 GHD double mc2(double sL, double sR) {
