@@ -41,7 +41,7 @@ FP64_PEAK_TFLOPS = 78.6
 #  c2p    : reads 7 cons + 3 face B + C2P_ERR, writes 12 prims + BDIV,
 #           C2P_DENS_ERR, C2P_ERR and rewrites <= 7 cons                -> 264 B
 ALG_BYTES = {"flux_x": 176, "flux_y": 176, "flux_z": 176, "emf": 216, "update": 456, "c2p": 264,
-             "ghost": 0}
+             "ghost": 0, "ghost_faces": 0}
 # Whole-step algorithmic bytes per zone-update (flat metric; RHS 3x(38..48
 # reads + 10 writes) + c2p 3x) -- 1552 B.
 STEP_ALG_BYTES = 1552
@@ -278,7 +278,7 @@ def main():
             ctx.rk3_step(dt, st, B, aux, stream=sp, graph=False)
         stream.synchronize()
     fam = {}
-    for f in ("flux_x", "flux_y", "flux_z", "emf", "update", "ghost", "c2p"):
+    for f in ("flux_x", "flux_y", "flux_z", "emf", "update", "ghost", "ghost_faces", "c2p"):
         ms, n = ctx.kernel_time_ms(f)
         if n:
             fam[f] = (ms / n, n)

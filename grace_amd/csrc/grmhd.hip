@@ -664,37 +664,69 @@ static void ghost_source(const DevGrid& G, const DevInst& I, const DevBlk& b, in
     off = (int)(fd < 0 ? cidx(G, sl[0], sl[1], sl[2]) : fidx_d(G, fd, sl[0], sl[1], sl[2]));
 }
 
-// grid: x over the entries of one block (cells, or all faces), y over blocks
-__global__ __launch_bounds__(256) void k_gather(DevGrid G, const int32_t* __restrict__ list,
-                                                const int2* __restrict__ src, int per, int e0, int ne, int ncg,
-                                                int nfg0, int nfg1, int qbase, GatherArgs a) {
-    const int e = e0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (e >= e0 + ne) return;
-    const int q = qbase + (int)blockIdx.y;
+// One copy per thread.  A channel is one variable of one array
+// (base, vars per block, var, elements per var per block).
+struct GatherChan { double* base; int nv, v; int64_t n; };
+constexpr int GATHER_MAXCH = 26;
+struct GatherChans { GatherChan c[GATHER_MAXCH]; int nch; };
+
+// Cell ghosts of every block: x over the shell cells, y over blocks; each
+// thread copies every channel of its cell, loads of four channels in flight
+// before their stores.  mode: 1 every ghost cell, 2 only those that run their
+// own c2p, 3 only those that take copies of c2p outputs.
+__global__ __launch_bounds__(256) void k_gather_cells(const int32_t* __restrict__ list, const int2* __restrict__ src,
+                                                      int per, int ncg, int qbase, int mode, GatherChans ch) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= ncg) return;
+    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int2 sr = src[q * per + e];
+    const bool own = (sr.x & GSRC_OWN) != 0;
+    if (mode >= 2 && (mode == 2) != own) return;
+    const int64_t qs = sr.x & (GSRC_OWN - 1);
     const int off = list[e];
-    const int2 sr = src[(int64_t)q * per + e];
-    if (e < ncg) {
-        const bool own = (sr.x & GSRC_OWN) != 0;
-        if (a.cell_mode >= 2 && (a.cell_mode == 2) != own) return;
-        const int64_t qs = sr.x & (GSRC_OWN - 1);
-        const int64_t nc = G.ncell;
-        const int cs = sr.y;
-        if (a.st)
-            for (int v = a.v0; v < a.v0 + a.vc; ++v)
-                a.st[((int64_t)q * a.nv + v) * nc + off] = a.st[(qs * a.nv + v) * nc + cs];
-        if (a.aux)
-            for (uint32_t m = a.aux_mask; m; m &= m - 1) {
-                const int v = __builtin_ctz(m);
-                a.aux[((int64_t)q * NAUX + v) * nc + off] = a.aux[(qs * NAUX + v) * nc + cs];
+    for (int c0 = 0; c0 < ch.nch; c0 += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (c0 + u < ch.nch) {
+                const GatherChan& C = ch.c[c0 + u];
+                v[u] = C.base[(qs * C.nv + C.v) * C.n + sr.y];
             }
-        if (a.ecen)
-            for (int v = 0; v < 3; ++v) a.ecen[((int64_t)q * 3 + v) * nc + off] = a.ecen[(qs * 3 + v) * nc + cs];
-    } else {
-        const int fd = e < ncg + nfg0 ? 0 : (e < ncg + nfg0 + nfg1 ? 1 : 2);
-        double* Bd = a.B[fd];
-        const int64_t nf = G.nface[fd];
-        Bd[(int64_t)q * nf + off] = Bd[(int64_t)sr.x * nf + sr.y];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (c0 + u < ch.nch) {
+                const GatherChan& C = ch.c[c0 + u];
+                C.base[(q * C.nv + C.v) * C.n + off] = v[u];
+            }
+        }
     }
+}
+
+// Explicit {block, cell, source block, source cell} entries (the ghosts
+// that run their own c2p), z over channels.
+__global__ __launch_bounds__(256) void k_gather_list(const int4* __restrict__ ent, int n, GatherChans ch) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n) return;
+    const int4 t = ent[e];
+    const GatherChan C = ch.c[blockIdx.z];
+    C.base[((int64_t)t.x * C.nv + C.v) * C.n + t.y] = C.base[((int64_t)t.z * C.nv + C.v) * C.n + t.w];
+}
+
+// Face ghosts of the three face-B arrays: x over the shell faces, y over blocks.
+__global__ __launch_bounds__(256) void k_gather_faces(const int32_t* __restrict__ list, const int2* __restrict__ src,
+                                                      int per, int ncg, int nfg0, int nfg1, int nfg2, int qbase,
+                                                      double* B0, double* B1, double* B2, int64_t nf0, int64_t nf1,
+                                                      int64_t nf2) {
+    const int e = ncg + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= ncg + nfg0 + nfg1 + nfg2) return;
+    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int2 sr = src[q * per + e];
+    double* B; int64_t nf;
+    if (e < ncg + nfg0) { B = B0; nf = nf0; }
+    else if (e < ncg + nfg0 + nfg1) { B = B1; nf = nf1; }
+    else { B = B2; nf = nf2; }
+    B[q * nf + list[e]] = B[(int64_t)sr.x * nf + sr.y];
 }
 
 // ---------------------------------------------------------------------------
@@ -1019,6 +1051,7 @@ struct grmhd_ctx {
     int32_t* ghost_list = nullptr;
     int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
     int2* own_cells = nullptr;                // {block, cell} of ghosts that run their own c2p
+    int4* own_gather = nullptr;               // the same with their sources
     int64_t n_own = 0;
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
     int64_t last_c2p_n = 0;
@@ -1095,7 +1128,7 @@ static void build_eos(grmhd_ctx* c) {
 
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells);
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1227,14 +1260,23 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
             }
         }
         std::vector<int2> own;
+        std::vector<int4> owng;
         for (int64_t q = 0; q < G.nq; ++q)
-            for (int64_t x = 0; x < c->ncg; ++x)
-                if (src[(size_t)(q * per + x)].x & GSRC_OWN) own.push_back(make_int2((int)q, lst[x]));
+            for (int64_t x = 0; x < c->ncg; ++x) {
+                const int2 sr = src[(size_t)(q * per + x)];
+                if (sr.x & GSRC_OWN) {
+                    own.push_back(make_int2((int)q, lst[x]));
+                    owng.push_back(make_int4((int)q, lst[x], sr.x & (GSRC_OWN - 1), sr.y));
+                }
+            }
         c->n_own = (int64_t)own.size();
         if (!own.empty()) {
             if ((e = hipMalloc(&c->own_cells, sizeof(int2) * own.size())) != hipSuccess) return fail("own cells", e);
             if ((e = hipMemcpy(c->own_cells, own.data(), sizeof(int2) * own.size(), hipMemcpyHostToDevice)) != hipSuccess)
                 return fail("own cells copy", e);
+            if ((e = hipMalloc(&c->own_gather, sizeof(int4) * owng.size())) != hipSuccess) return fail("own gather", e);
+            if ((e = hipMemcpy(c->own_gather, owng.data(), sizeof(int4) * owng.size(), hipMemcpyHostToDevice)) != hipSuccess)
+                return fail("own gather copy", e);
         }
         if ((e = hipMalloc(&c->ghost_src, sizeof(int2) * src.size())) != hipSuccess) return fail("ghost sources", e);
         if ((e = hipMemcpy(c->ghost_src, src.data(), sizeof(int2) * src.size(), hipMemcpyHostToDevice)) != hipSuccess)
@@ -1340,18 +1382,38 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     });
 }
 static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
-    Scope sc(c, "ghost", s);
     const DevGrid& G = c->G;
     const int per = (int)(c->ncg + c->nfg[0] + c->nfg[1] + c->nfg[2]);
-    const bool cells = a.cell_mode != 0;
-    const bool faces = a.B[0] != nullptr;
-    if (!cells && !faces) return;
-    const int e0 = cells ? 0 : (int)c->ncg;
-    const int ne = (faces ? per : (int)c->ncg) - e0;
-    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {   // grid.y is limited to 65535 blocks per launch
-        dim3 grid(nblk(ne), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-        k_gather<<<grid, 256, 0, s>>>(G, c->ghost_list, c->ghost_src, per, e0, ne, (int)c->ncg, (int)c->nfg[0],
-                                      (int)c->nfg[1], (int)q0, a);
+    if (a.B[0]) {
+        Scope sc(c, "ghost_faces", s);
+        const int nfg = (int)(c->nfg[0] + c->nfg[1] + c->nfg[2]);
+        for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
+            dim3 grid(nblk(nfg), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+            k_gather_faces<<<grid, 256, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0],
+                                                (int)c->nfg[1], (int)c->nfg[2], (int)q0, a.B[0], a.B[1], a.B[2],
+                                                G.nface[0], G.nface[1], G.nface[2]);
+        }
+    }
+    if (a.cell_mode == 0) return;
+    Scope sc(c, "ghost", s);
+    GatherChans ch{};
+    auto add = [&](double* base, int nv, int v) { ch.c[ch.nch++] = GatherChan{base, nv, v, G.ncell}; };
+    if (a.st)
+        for (int v = a.v0; v < a.v0 + a.vc; ++v) add(a.st, a.nv, v);
+    if (a.aux)
+        for (int v = 0; v < NAUX; ++v)
+            if (a.aux_mask & (1u << v)) add(a.aux, NAUX, v);
+    if (a.ecen)
+        for (int v = 0; v < 3; ++v) add(a.ecen, 3, v);
+    if (ch.nch == 0) return;
+    if (a.cell_mode == 2) {
+        if (c->n_own > 0)
+            k_gather_list<<<dim3(nblk(c->n_own), 1, ch.nch), 256, 0, s>>>(c->own_gather, (int)c->n_own, ch);
+        return;
+    }
+    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
+        dim3 grid(nblk(c->ncg), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+        k_gather_cells<<<grid, 256, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)q0, a.cell_mode, ch);
     }
 }
 
