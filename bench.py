@@ -204,6 +204,98 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
     }
 
 
+def c3_leg(ga, W, torch, dev, steps=3):
+    """64 uneven blast-wave instances (sides from {32,48,64,80,96}, 16^3 blocks)
+    in one block list: every kernel of the RK3 step covers all instances in one
+    launch; the step is replayed from a hipGraph with autotuned workgroups."""
+    log("[bench] batched uneven instances: building")
+    wl = W.multi_blast()
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    st = torch.from_numpy(wl.state).to(dev)
+    B = [torch.from_numpy(b).to(dev) for b in wl.B]
+    aux = torch.from_numpy(wl.aux).to(dev)
+    zones, nq, sides = wl.zones, int(wl.grid.nq), [int(x) for x in wl.extra["sides"]]
+    del wl
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+
+    def timed(k):
+        with torch.cuda.stream(stream):
+            ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
+            ctx.rk3_step(dt, st, B, aux, stream=sp, graph=True)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
+                ctx.rk3_step(dt, st, B, aux, stream=sp, graph=True)
+            stream.synchronize()
+            return (time.perf_counter() - t0) / k
+
+    t_default = timed(steps)
+    with torch.cuda.stream(stream):
+        tune = ctx.autotune(st, B, aux, reps=2, stream=sp)
+        stream.synchronize()
+    t_tuned = timed(steps)
+    return {"workload": "64 blast instances, sides {32,48,64,80,96} (seeded), 16^3 blocks, ngz 3, "
+                        "WENO-Z + HLLE + UCT, RK3 graph, device CFL dt",
+            "zones": zones, "blocks": nq, "instance_sides": sides, "steps": steps,
+            "zone_updates_per_s": zones / t_tuned, "ms_per_step": 1e3 * t_tuned,
+            "ms_per_step_default_shapes": 1e3 * t_default, "tuning": tune}
+
+
+def c1_leg(ga, W, torch, dev, t_final=0.4, cfl=0.25):
+    """Balsara-1 to t = 0.4 (400 cells, GS CT, device CFL dt read back every
+    step); the oracle replays the same dt sequence for the parity figure."""
+    log("[bench] Balsara-1 full run")
+    wl = W.balsara1(N=400)
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    st = torch.from_numpy(wl.state.copy()).to(dev)
+    B = [torch.from_numpy(b.copy()).to(dev) for b in wl.B]
+    aux = torch.from_numpy(wl.aux.copy()).to(dev)
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    dts, t = [], 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while t < t_final:
+        ctx.compute_dt(st, aux, cfl, dt)
+        d = float(dt.item())
+        if t + d > t_final:
+            d = t_final - t
+            dt.fill_(d)
+        ctx.rk3_step(dt, st, B, aux)
+        dts.append(d)
+        t += d
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    out = {"workload": "Balsara-1, 400x1x1 cells, ngz 3, Gamma 2, WENO-Z + HLLE + GS, RK3, CFL 0.25, t = 0.4",
+           "steps": len(dts), "gpu_seconds": gpu_s, "zone_updates_per_s_gpu": 400 * len(dts) / gpu_s}
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if os.path.exists(path):
+        L = C.CDLL(path)
+        L.orc_stepper_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_stepper_create.restype = C.c_void_p
+        L.orc_stepper_destroy.argtypes = [C.c_void_p]
+        L.orc_rk3_step.argtypes = [C.c_void_p, C.c_double] + [C.c_void_p] * 5
+        s_, B_, a_ = wl.copy_arrays()
+        h = L.orc_stepper_create(C.byref(wl.params), C.byref(wl.grid), wl.instances, len(wl.inst_specs))
+        p = lambda a: C.c_void_p(a.ctypes.data)
+        t0 = time.perf_counter()
+        for d in dts:
+            L.orc_rk3_step(h, d, p(s_), p(B_[0]), p(B_[1]), p(B_[2]), p(a_))
+        cpu_s = time.perf_counter() - t0
+        L.orc_stepper_destroy(h)
+        g = wl.grid.ngz
+        gs = st.cpu().numpy()
+        rel = []
+        for v in range(7):
+            a_int, b_int = gs[0, v, g, g, g:-g], s_[0, v, g, g, g:-g]
+            den = np.max(np.abs(b_int))
+            rel.append(float(np.max(np.abs(a_int - b_int)) / (den if den > 0 else 1.0)))
+        out.update({"cpu_seconds_oracle_1thread": cpu_s, "linf_rel_conserved_vs_oracle": max(rel)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +303,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-c2p", action="store_true", help="skip the learned-vs-plain c2p leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven-instance leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -302,6 +396,8 @@ def main():
     c2p_leg = None
     if not args.no_c2p:
         c2p_leg = c4_leg(ga, W, torch, dev)
+    c1 = None if args.no_c1 else c1_leg(ga, W, torch, dev)
+    c3 = None if args.no_c3 else c3_leg(ga, W, torch, dev)
 
     cpu = None
     if not args.no_cpu:
@@ -330,6 +426,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "c2p_learned_vs_plain": c2p_leg,
+        "balsara1_full_run": c1,
+        "batched_uneven": c3,
     }
     print(json.dumps(out), flush=True)
 

@@ -69,6 +69,7 @@ ABI_SYMBOLS = [
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
+    "grmhd_autotune", "grmhd_get_tuning", "grmhd_set_tuning",
     "grmhd_setup_blast", "grmhd_setup_shocktube", "grmhd_kerr_schild", "grmhd_setup_c2p_states",
     # include/grmhd_mlp.h
     "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
@@ -79,6 +80,15 @@ ABI_SYMBOLS = [
 
 MLP_NPARAM = 5378
 MLP_NSCALE = 24
+
+
+class Tuning(C.Structure):
+    _fields_ = [("flux", C.c_int32 * 3), ("emf", C.c_int32), ("update", C.c_int32), ("gather_faces", C.c_int32),
+                ("gather_cells", C.c_int32), ("c2p", C.c_int32)]
+
+    def as_dict(self):
+        return {"flux": list(self.flux), "emf": self.emf, "update": self.update, "gather_faces": self.gather_faces,
+                "gather_cells": self.gather_cells, "c2p": self.c2p}
 
 
 class MlpFitOpts(C.Structure):
@@ -119,6 +129,9 @@ def lib():
     L.grmhd_rk3_step_graph.argtypes = [P, P, P, P * 3, P, P]
     L.grmhd_set_timing.argtypes = [P, i32]
     L.grmhd_kernel_time_ms.argtypes = [P, C.c_char_p, C.POINTER(d), C.POINTER(i64)]
+    L.grmhd_autotune.argtypes = [P, P, P * 3, P, i32, C.POINTER(Tuning), P]
+    L.grmhd_get_tuning.argtypes = [P, C.POINTER(Tuning)]
+    L.grmhd_set_tuning.argtypes = [P, C.POINTER(Tuning)]
     L.grmhd_setup_blast.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, P, P, d, d, d, d, P, d,
                                     C.c_uint64, P, P, P, P, P]
     L.grmhd_setup_shocktube.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, d, d, P, P, P, P, P, P, P]
@@ -254,6 +267,24 @@ class Context:
     def rk3_step(self, dt_dev, state, B, aux, stream=None, graph=False):
         fn = self.L.grmhd_rk3_step_graph if graph else self.L.grmhd_rk3_step
         check(fn(self.h, _ptr(dt_dev), _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)), "rk3_step")
+
+    def autotune(self, state, B, aux, reps=3, stream=None):
+        t = Tuning()
+        check(self.L.grmhd_autotune(self.h, _ptr(state), ptr3(*B), _ptr(aux), int(reps), C.byref(t), _ptr(stream)),
+              "autotune")
+        return t.as_dict()
+
+    def get_tuning(self):
+        t = Tuning()
+        check(self.L.grmhd_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, flux=(256, 256, 256), emf=256, update=256, gather_faces=256, gather_cells=256, c2p=256):
+        t = Tuning()
+        for d in range(3):
+            t.flux[d] = flux[d]
+        t.emf, t.update, t.gather_faces, t.gather_cells, t.c2p = emf, update, gather_faces, gather_cells, c2p
+        check(self.L.grmhd_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def set_timing(self, on):
         check(self.L.grmhd_set_timing(self.h, int(on)), "set_timing")

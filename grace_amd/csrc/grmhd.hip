@@ -1062,6 +1062,8 @@ struct grmhd_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     const void* gkey[8] = {0};
+    // workgroup sizes picked by grmhd_autotune (all 256 until then)
+    grmhd_tuning tune{{256, 256, 256}, 256, 256, 256, 256, 256};
 };
 
 static hipEvent_t pool_event(grmhd_ctx* c) {
@@ -1079,6 +1081,7 @@ struct Scope {
 };
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+static inline unsigned nblk_bs(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 extern "C" void grmhd_default_params(grmhd_params* p) {
     std::memset(p, 0, sizeof(*p));
@@ -1220,7 +1223,8 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
     if ((e = hipMalloc(&c->dtpart, sizeof(double) * c->n_dtpart)) != hipSuccess) return fail("dtpart", e);
     if ((e = hipMalloc(&c->stats, sizeof(DevStats))) != hipSuccess) return fail("stats", e);
     if ((e = hipMemset(c->stats, 0, sizeof(DevStats))) != hipSuccess) return fail("stats clear", e);
-    c->n_part_cap = nblk(G.ncell * G.nq) + 2 * G.nq + 2;   // per-launch rounding of the 2-D c2p grids
+    // 2-D c2p grids with workgroups down to 64 threads, plus per-launch rounding
+    c->n_part_cap = nblk_bs(G.ncell * G.nq, 64) + 2 * G.nq + 2;
     if ((e = hipMalloc(&c->stats_part, sizeof(DevStats) * c->n_part_cap)) != hipSuccess) return fail("stats part", e);
     {   // ghost-shell tables (local offsets), cells then x/y/z faces
         std::vector<int32_t> lst;
@@ -1315,10 +1319,11 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
     for (int d = 0; d < 3; ++d) {
         Scope sc(c, fam[d], s);
         for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-            dim3 grid(nblk(nf[d]), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-            if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], (int)q0);
-            else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], (int)q0);
-            else k_flux<EOS, REC, FLAT, 2><<<grid, 256, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], (int)q0);
+            const int bs = c->tune.flux[d];
+            dim3 grid(nblk_bs(nf[d], bs), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+            if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], (int)q0);
+            else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], (int)q0);
+            else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], (int)q0);
         }
     }
 }
@@ -1346,27 +1351,27 @@ static void launch_fluxes(grmhd_ctx* c, const double* aux, const double* ms, int
 // grids of the per-site kernels: x over the (n+1)^3 sites of a block, y over
 // blocks (chunks of 65535)
 template <typename F>
-static void site_launch(const DevGrid& G, F&& f) {
+static void site_launch(const DevGrid& G, int bs, F&& f) {
     const int sites = (G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1);
     for (int64_t q0 = 0; q0 < G.nq; q0 += 65535)
-        f(dim3(nblk(sites), (unsigned)std::min<int64_t>(65535, G.nq - q0)), (int)q0);
+        f(dim3(nblk_bs(sites, bs), (unsigned)std::min<int64_t>(65535, G.nq - q0)), (int)q0);
 }
 static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
     Scope sc(c, "emf", s);
     const DevGrid& G = c->G;
     if (c->P.emf == GRMHD_EMF_GS) {
-        site_launch(G, [&](dim3 g, int q0) {
-            k_emf_gs<<<g, 256, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
+        site_launch(G, c->tune.emf, [&](dim3 g, int q0) {
+            k_emf_gs<<<g, c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
                                        c->flux[2], c->emf, q0);
         });
         return;
     }
-    site_launch(G, [&](dim3 g, int q0) {
+    site_launch(G, c->tune.emf, [&](dim3 g, int q0) {
         switch (c->P.recon) {
-            case 0: k_emf_uct<0><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
-            case 1: k_emf_uct<1><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
-            case 2: k_emf_uct<2><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
-            default: k_emf_uct<3><<<g, 256, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            case 0: k_emf_uct<0><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            case 1: k_emf_uct<1><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            case 2: k_emf_uct<2><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            default: k_emf_uct<3><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
         }
     });
 }
@@ -1376,9 +1381,9 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     a.inst = c->inst_d; a.blk = c->blk_d;
     for (int d = 0; d < 3; ++d) a.flux[d] = c->flux[d];
     a.emf = c->emf;
-    site_launch(G, [&](dim3 g, int q0) {
-        if (c->P.flat_metric) k_update<true><<<g, 256, 0, s>>>(G, a, c->P.fd_order, q0);
-        else k_update<false><<<g, 256, 0, s>>>(G, a, c->P.fd_order, q0);
+    site_launch(G, c->tune.update, [&](dim3 g, int q0) {
+        if (c->P.flat_metric) k_update<true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, q0);
+        else k_update<false><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, q0);
     });
 }
 static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
@@ -1388,8 +1393,8 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
         Scope sc(c, "ghost_faces", s);
         const int nfg = (int)(c->nfg[0] + c->nfg[1] + c->nfg[2]);
         for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-            dim3 grid(nblk(nfg), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-            k_gather_faces<<<grid, 256, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0],
+            dim3 grid(nblk_bs(nfg, c->tune.gather_faces), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+            k_gather_faces<<<grid, c->tune.gather_faces, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0],
                                                 (int)c->nfg[1], (int)c->nfg[2], (int)q0, a.B[0], a.B[1], a.B[2],
                                                 G.nface[0], G.nface[1], G.nface[2]);
         }
@@ -1412,8 +1417,8 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
         return;
     }
     for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-        dim3 grid(nblk(c->ncg), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-        k_gather_cells<<<grid, 256, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)q0, a.cell_mode, ch);
+        dim3 grid(nblk_bs(c->ncg, c->tune.gather_cells), (unsigned)std::min<int64_t>(65535, G.nq - q0));
+        k_gather_cells<<<grid, c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)q0, a.cell_mode, ch);
     }
 }
 
@@ -1426,21 +1431,22 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     double* ec = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
     const bool flat = c->P.flat_metric != 0;
     const int ni = G.n[0] * G.n[1] * G.n[2];
+    const int bs = c->tune.c2p;
     int64_t npart = 0;
     auto launch = [&](int cset, int ne, dim3 grid, int qbase) {
         DevStats* part = with_stats ? c->stats_part + npart : nullptr;
         npart += (int64_t)grid.x * grid.y;
-#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<grid, 256, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, qbase)
+#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<grid, bs, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, qbase)
         if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
         else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
 #undef C2PL
     };
     for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
         const unsigned nqq = (unsigned)std::min<int64_t>(65535, G.nq - q0);
-        launch(CS_INTERIOR, ni, dim3(nblk(ni), nqq), (int)q0);
-        if (!selective) launch(CS_SHELL, (int)c->ncg, dim3(nblk(c->ncg), nqq), (int)q0);
+        launch(CS_INTERIOR, ni, dim3(nblk_bs(ni, bs), nqq), (int)q0);
+        if (!selective) launch(CS_SHELL, (int)c->ncg, dim3(nblk_bs(c->ncg, bs), nqq), (int)q0);
     }
-    if (selective && c->n_own > 0) launch(CS_LIST, (int)c->n_own, dim3(nblk(c->n_own), 1), 0);
+    if (selective && c->n_own > 0) launch(CS_LIST, (int)c->n_own, dim3(nblk_bs(c->n_own, bs), 1), 0);
     if (with_stats) {
         k_stats_final<<<1, 256, 0, s>>>(c->stats_part, npart, c->stats);
         c->last_c2p_n = G.ncell * G.nq;
@@ -1723,6 +1729,121 @@ extern "C" grmhd_status grmhd_rk3_step_graph(grmhd_ctx* c, const double* dt_dev,
     Scope sc(c, "step", s);
     HIP_TRY(hipGraphLaunch(c->gexec, s));
     return GRMHD_OK;
+}
+
+static bool tuning_valid(const grmhd_tuning& t) {
+    auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
+    return ok(t.flux[0]) && ok(t.flux[1]) && ok(t.flux[2]) && ok(t.emf) && ok(t.update) && ok(t.gather_faces) &&
+           ok(t.gather_cells) && ok(t.c2p);
+}
+static void drop_graph(grmhd_ctx* c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    std::memset(c->gkey, 0, sizeof(c->gkey));
+}
+
+extern "C" grmhd_status grmhd_get_tuning(grmhd_ctx* c, grmhd_tuning* out) {
+    if (!c || !out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    *out = c->tune;
+    return GRMHD_OK;
+}
+extern "C" grmhd_status grmhd_set_tuning(grmhd_ctx* c, const grmhd_tuning* in) {
+    if (!c || !in || !tuning_valid(*in)) { set_err("invalid tuning"); return GRMHD_ERR_INVALID; }
+    c->tune = *in;
+    drop_graph(c);
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const double* const B[3], const double* aux,
+                                       int32_t reps, grmhd_tuning* out, grmhd_stream_t stream) {
+    if (!c || !state || !B || !aux || reps < 1 || reps > 100) { set_err("invalid autotune arguments"); return GRMHD_ERR_INVALID; }
+    grmhd_status st = ensure_stage_buffers(c);
+    if (st != GRMHD_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const DevGrid& G = c->G;
+    // scratch: a zero dt and a copy of aux for the c2p runs
+    double* dt0 = nullptr;
+    double* aux2 = nullptr;
+    HIP_TRY(hipMalloc(&dt0, sizeof(double)));
+    hipError_t e = hipMalloc(&aux2, sizeof(double) * NAUX * G.ncell * G.nq);
+    if (e != hipSuccess) { (void)hipFree(dt0); set_err("autotune scratch: %s", hipGetErrorString(e)); return GRMHD_ERR_ALLOC; }
+    (void)hipMemsetAsync(dt0, 0, sizeof(double), s);
+    (void)hipMemcpyAsync(aux2, aux, sizeof(double) * NAUX * G.ncell * G.nq, hipMemcpyDeviceToDevice, s);
+    const bool was_timing = c->timing;
+    c->timing = false;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    auto time_it = [&](auto&& fn) {
+        fn();   // warm-up
+        std::vector<float> t;
+        for (int r = 0; r < reps; ++r) {
+            (void)hipEventRecord(a, s);
+            fn();
+            (void)hipEventRecord(b, s);
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    };
+    const int cand[3] = {64, 128, 256};
+    auto pick = [&](int* slot, auto&& fn) {
+        float best = 1e30f;
+        int bv = *slot;
+        for (int v : cand) {
+            *slot = v;
+            float ms = time_it(fn);
+            if (ms < best) { best = ms; bv = v; }
+        }
+        *slot = bv;
+    };
+    const double* const Bc[3] = {B[0], B[1], B[2]};
+    double* const B1[3] = {c->B1[0], c->B1[1], c->B1[2]};
+    const double* const B1c[3] = {c->B1[0], c->B1[1], c->B1[2]};
+    grmhd_tuning keep = c->tune;
+    for (int d = 0; d < 3; ++d) {
+        // one direction at a time: the other two run at their current size
+        pick(&c->tune.flux[d], [&] { launch_fluxes(c, aux, state, NSTATE, Bc, s); });
+    }
+    pick(&c->tune.emf, [&] { launch_emf(c, Bc, s); });
+    pick(&c->tune.update, [&] {
+        UpdArgs u{};
+        u.nst = c->U1; u.nnv = NHRSC; u.ast = state; u.anv = NSTATE; u.bst = nullptr; u.bnv = 0;
+        u.ca = 1.0; u.cb = 0.0; u.mstate = state; u.mnv = NSTATE; u.aux = aux;
+        for (int d = 0; d < 3; ++d) { u.nB[d] = B1[d]; u.aB[d] = Bc[d]; u.bB[d] = nullptr; }
+        u.dt = dt0; u.dtfact = 1.0;
+        launch_update(c, u, s);
+    });
+    pick(&c->tune.gather_faces, [&] {
+        GatherArgs g{};
+        for (int d = 0; d < 3; ++d) g.B[d] = B1[d];
+        launch_gather(c, g, s);
+    });
+    pick(&c->tune.gather_cells, [&] {
+        GatherArgs g{};
+        g.aux = aux2; g.aux_mask = 0x3FFu; g.cell_mode = 3;
+        launch_gather(c, g, s);
+    });
+    {   // ghosts of the stage copy, so the c2p sees valid inputs everywhere
+        GatherArgs g{};
+        g.st = c->U1; g.nv = NHRSC; g.v0 = 0; g.vc = NHRSC; g.cell_mode = 1;
+        for (int d = 0; d < 3; ++d) g.B[d] = B1[d];
+        launch_gather(c, g, s);
+    }
+    pick(&c->tune.c2p, [&] { launch_c2p_cells(c, c->U1, NHRSC, state, NSTATE, B1c, aux2, false, true, s); });
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(dt0);
+    (void)hipFree(aux2);
+    c->timing = was_timing;
+    if (!tuning_valid(c->tune)) c->tune = keep;
+    drop_graph(c);
+    if (out) *out = c->tune;
+    return launch_check("autotune");
 }
 
 extern "C" grmhd_status grmhd_set_timing(grmhd_ctx* c, int32_t on) {

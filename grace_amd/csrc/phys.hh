@@ -329,12 +329,15 @@ GHD void conserved(const Met& m, double rho, double press, double eps, const dou
 
 // ---- roots --------------------------------------------------------------------
 // Brent's zero-in (Brent 1973, ch. 4): same step sequence as rootfinding.hh:73-231.
+constexpr int BRENT_MAX_ITER = 1000;
 template <typename F>
 __host__ __device__ inline double brent_fab(F& f, double a, double b, double fa, double fb, double t) {
     double sa = a, sb = b;
     double c = sa, fc = fa, e = sb - sa, d = e;
     const double macheps = 2.220446049250313e-16;
-    for (;;) {
+    // Finite inputs converge in far fewer steps (the interval at least halves
+    // every few iterations); the cap only ends the loop on NaN input.
+    for (int it = 0; it < BRENT_MAX_ITER; ++it) {
         if (fabs(fc) < fabs(fb)) { sa = sb; sb = c; c = sa; fa = fb; fb = fc; fc = fa; }
         double tol = 2.0 * macheps * fabs(sb) + t;
         double mm = 0.5 * (c - sb);

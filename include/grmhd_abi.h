@@ -175,6 +175,28 @@ grmhd_status grmhd_rk3_step(grmhd_ctx* ctx, const double* dt_dev, double* state,
 grmhd_status grmhd_rk3_step_graph(grmhd_ctx* ctx, const double* dt_dev, double* state,
                                   double* const B[3], double* aux, grmhd_stream_t stream);
 
+/* Workgroup sizes (threads) of the kernel families, chosen by
+ * grmhd_autotune; every entry is one of 64, 128, 256. */
+typedef struct {
+    int32_t flux[3];        /* face-flux kernels, x/y/z */
+    int32_t emf;            /* edge EMFs */
+    int32_t update;         /* fused RK update + CT */
+    int32_t gather_faces;   /* ghost gather of face B */
+    int32_t gather_cells;   /* ghost gather of cell variables */
+    int32_t c2p;            /* c2p over cells */
+} grmhd_tuning;
+
+/* Times each kernel family of one RK3 substep on the context's own block
+ * list with every candidate workgroup size (the inputs are read, never
+ * modified: updates, gathers and c2p run on internal copies) and keeps the
+ * fastest; tuning_out_or_null receives the choice.  Invalidates any
+ * captured RK3 graph.  The reference has no counterpart (its Kokkos launch
+ * shapes are fixed at compile time). */
+grmhd_status grmhd_autotune(grmhd_ctx* ctx, const double* state, const double* const B[3], const double* aux,
+                            int32_t reps, grmhd_tuning* tuning_out_or_null, grmhd_stream_t stream);
+grmhd_status grmhd_get_tuning(grmhd_ctx* ctx, grmhd_tuning* out);
+grmhd_status grmhd_set_tuning(grmhd_ctx* ctx, const grmhd_tuning* in);
+
 /* Total duration (ms) and number of launches of a kernel family since the
  * last query, measured with HIP events recorded on the launch stream around
  * each launch ("flux_x", "flux_y", "flux_z", "emf", "update", "ghost", "c2p",

@@ -593,7 +593,9 @@ inline double brent_with_ends(F& f, double a, double b, double fa, double fb, do
     double c = sa, fc = fa;
     double e = sb - sa, d = e;
     const double macheps = std::numeric_limits<double>::epsilon();
-    for (;;) {
+    // the reference loops without a cap; finite inputs never reach this one,
+    // which only ends the loop on NaN input
+    for (int it = 0; it < 1000; ++it) {
         if (std::fabs(fc) < std::fabs(fb)) {
             sa = sb; sb = c; c = sa;
             fa = fb; fb = fc; fc = fa;
