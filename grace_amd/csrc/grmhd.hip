@@ -82,6 +82,19 @@ GHD int64_t eidx(const DevGrid& G, int i, int j, int k) {
     return i + (int64_t)(G.N[0] + 1) * (j + (int64_t)(G.N[1] + 1) * k);
 }
 
+// Position of this workgroup in a 1-D grid of wpb workgroups per block times
+// the number of blocks.  The hardware deals workgroups to the 8 XCDs round
+// robin; the remap hands each XCD a contiguous range of logical workgroups,
+// so the workgroups of one block share that XCD's L2 instead of all eight
+// L2s fetching the same stencil data.
+struct WgPos { int tile; int64_t q; };
+__device__ __forceinline__ WgPos wg_pos(int wpb) {
+    const int64_t p = blockIdx.x, n = gridDim.x;
+    const int64_t x = p & 7, i = p >> 3, base = n >> 3, rem = n & 7;
+    const int64_t L = x * base + (x < rem ? x : rem) + i;
+    return WgPos{(int)(L % wpb), L / wpb};
+}
+
 // Read the Cowling metric of a cell (state vars GXX_..BETAZ_).
 GHD void load_metric(Met& m, const double* __restrict__ ms, int64_t ncell, int64_t c) {
     for (int a = 0; a < 6; ++a) m.g[a] = ms[(GXX_ + a) * ncell + c];
@@ -97,8 +110,8 @@ template <typename EOS, int REC, bool FLAT, int D>
 __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
                                               const double* __restrict__ mstate, int mnv,
                                               const double* __restrict__ Bd, double* __restrict__ flux,
-                                              double* __restrict__ vbar, int qbase) {
-    // One workgroup row per block (q = blockIdx.y).  UCT: the first
+                                              double* __restrict__ vbar, int wpb) {
+    // wpb workgroups per block (see wg_pos).  UCT: the first
     // (n_D+1) n_T0 n_T1 threads take the faces whose transverse indices are
     // both interior (full fluxes), the next ones the faces with exactly one
     // transverse index in the ghost layers ("ring": they feed only the
@@ -107,8 +120,9 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
     // and get no thread.  GS reads fluxes and face EMFs one layer into the
     // transverse ghosts, so it takes every face (normal [ng, n+ng],
     // transverse [0, N)).
-    const int64_t q = qbase + (int64_t)blockIdx.y;
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const WgPos wp = wg_pos(wpb);
+    const int64_t q = wp.q;
+    const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     int i, j, k;
     bool ring = false;
     {
@@ -314,12 +328,13 @@ template <int REC>
 __global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __restrict__ Bx,
                                                  const double* __restrict__ By, const double* __restrict__ Bz,
                                                  const double* __restrict__ vx, const double* __restrict__ vy,
-                                                 const double* __restrict__ vz, double* __restrict__ emf, int qbase) {
-    // x: the (n+1)^3 sites of one block, y: blocks
+                                                 const double* __restrict__ vz, double* __restrict__ emf, int wpb) {
+    // wpb workgroups per block over its (n+1)^3 sites (see wg_pos)
     const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
-    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const WgPos wp = wg_pos(wpb);
+    const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
-    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int64_t q = wp.q;
     const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
     const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2];
@@ -377,12 +392,13 @@ __global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __rest
 __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restrict__ ef0, const double* __restrict__ ef1,
                                                 const double* __restrict__ ef2, const double* __restrict__ ec,
                                                 const double* __restrict__ f0, const double* __restrict__ f1,
-                                                const double* __restrict__ f2, double* __restrict__ emf, int qbase) {
-    // x: the (n+1)^3 sites of one block, y: blocks
+                                                const double* __restrict__ f2, double* __restrict__ emf, int wpb) {
+    // wpb workgroups per block over its (n+1)^3 sites (see wg_pos)
     const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
-    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const WgPos wp = wg_pos(wpb);
+    const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
-    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int64_t q = wp.q;
     const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
     const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2], nc = G.ncell;
@@ -510,12 +526,13 @@ GHD void geom_sources(const Met& m, const double K[6], const double da[3], const
 }
 
 template <bool FLAT>
-__global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order, int qbase) {
-    // x: the (n+1)^3 sites of one block, y: blocks
+__global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order, int wpb) {
+    // wpb workgroups per block over its (n+1)^3 sites (see wg_pos)
     const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
-    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const WgPos wp = wg_pos(wpb);
+    const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
-    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int64_t q = wp.q;
     const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
     const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
     const DevInst& I = a.inst[a.blk[q].inst];
@@ -670,15 +687,16 @@ struct GatherChan { double* base; int nv, v; int64_t n; };
 constexpr int GATHER_MAXCH = 26;
 struct GatherChans { GatherChan c[GATHER_MAXCH]; int nch; };
 
-// Cell ghosts of every block: x over the shell cells, y over blocks; each
+// Cell ghosts of every block (wpb workgroups per block, see wg_pos); each
 // thread copies every channel of its cell, loads of four channels in flight
 // before their stores.  mode: 1 every ghost cell, 2 only those that run their
 // own c2p, 3 only those that take copies of c2p outputs.
 __global__ __launch_bounds__(256) void k_gather_cells(const int32_t* __restrict__ list, const int2* __restrict__ src,
-                                                      int per, int ncg, int qbase, int mode, GatherChans ch) {
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+                                                      int per, int ncg, int wpb, int mode, GatherChans ch) {
+    const WgPos wp = wg_pos(wpb);
+    const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (e >= ncg) return;
-    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int64_t q = wp.q;
     const int2 sr = src[q * per + e];
     const bool own = (sr.x & GSRC_OWN) != 0;
     if (mode >= 2 && (mode == 2) != own) return;
@@ -713,14 +731,15 @@ __global__ __launch_bounds__(256) void k_gather_list(const int4* __restrict__ en
     C.base[((int64_t)t.x * C.nv + C.v) * C.n + t.y] = C.base[((int64_t)t.z * C.nv + C.v) * C.n + t.w];
 }
 
-// Face ghosts of the three face-B arrays: x over the shell faces, y over blocks.
+// Face ghosts of the three face-B arrays (wpb workgroups per block).
 __global__ __launch_bounds__(256) void k_gather_faces(const int32_t* __restrict__ list, const int2* __restrict__ src,
-                                                      int per, int ncg, int nfg0, int nfg1, int nfg2, int qbase,
+                                                      int per, int ncg, int nfg0, int nfg1, int nfg2, int wpb,
                                                       double* B0, double* B1, double* B2, int64_t nf0, int64_t nf1,
                                                       int64_t nf2) {
-    const int e = ncg + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const WgPos wp = wg_pos(wpb);
+    const int e = ncg + wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (e >= ncg + nfg0 + nfg1 + nfg2) return;
-    const int64_t q = qbase + (int64_t)blockIdx.y;
+    const int64_t q = wp.q;
     const int2 sr = src[q * per + e];
     double* B; int64_t nf;
     if (e < ncg + nfg0) { B = B0; nf = nf0; }
@@ -795,9 +814,9 @@ __global__ __launch_bounds__(256) void k_stats_final(const DevStats* __restrict_
 #ifndef GRMHD_C2P_MINB
 #define GRMHD_C2P_MINB 3
 #endif
-// Cell sets: CS_INTERIOR (grid x over the interior cells of a block, y over
-// blocks), CS_SHELL (x over the ghost-shell list, y over blocks), CS_LIST
-// (x over explicit {block, cell} pairs).
+// Cell sets: CS_INTERIOR (the interior cells of every block, wpb workgroups
+// per block, see wg_pos), CS_SHELL (the ghost-shell cells of every block),
+// CS_LIST (explicit {block, cell} pairs).
 enum { CS_INTERIOR = 0, CS_SHELL = 1, CS_LIST = 2 };
 template <typename EOS, bool FLAT>
 __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EOS eos, Opts o, const DevInst* __restrict__ inst,
@@ -806,15 +825,18 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
                                                    const double* __restrict__ Bx, const double* __restrict__ By,
                                                    const double* __restrict__ Bz, double* aux, double* ecen,
                                                    DevStats* stats, int cset, const int32_t* __restrict__ shell,
-                                                   const int2* __restrict__ cl, int ne, int qbase) {
-    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+                                                   const int2* __restrict__ cl, int ne, int wpb) {
+    WgPos wp{0, 0};
+    if (cset != CS_LIST) wp = wg_pos(wpb);
+    const int e = cset == CS_LIST ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
+                                  : wp.tile * (int)blockDim.x + (int)threadIdx.x;
     bool active = e < ne;
     const int ee = active ? e : 0;
     int64_t q;
     int64_t c;
     int i, j, k;
     if (cset == CS_INTERIOR) {
-        q = qbase + (int)blockIdx.y;
+        q = wp.q;
         const int n0 = G.n[0], n01 = G.n[0] * G.n[1];
         k = ee / n01; const int r = ee - k * n01;
         j = r / n0; i = r - j * n0;
@@ -822,7 +844,7 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
         c = cidx(G, i, j, k);
     } else {
         int off;
-        if (cset == CS_SHELL) { q = qbase + (int)blockIdx.y; off = shell[ee]; }
+        if (cset == CS_SHELL) { q = wp.q; off = shell[ee]; }
         else { const int2 p = cl[ee]; q = p.x; off = p.y; }
         c = off;
         const int N0 = G.N[0], N01 = G.N[0] * G.N[1];
@@ -876,7 +898,7 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
         }
     }
     if (!active) { cnt = C2PCount(); err = 0; }
-    if (stats) stats_block(stats + (int64_t)blockIdx.y * gridDim.x, active, active && out.fatal, err, cnt);
+    if (stats) stats_block(stats, active, active && out.fatal, err, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1318,13 +1340,12 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
     static const char* fam[3] = {"flux_x", "flux_y", "flux_z"};
     for (int d = 0; d < 3; ++d) {
         Scope sc(c, fam[d], s);
-        for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-            const int bs = c->tune.flux[d];
-            dim3 grid(nblk_bs(nf[d], bs), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-            if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], (int)q0);
-            else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], (int)q0);
-            else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], (int)q0);
-        }
+        const int bs = c->tune.flux[d];
+        const int wpb = (int)nblk_bs(nf[d], bs);
+        const dim3 grid((unsigned)(wpb * G.nq));
+        if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], wpb);
+        else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], wpb);
+        else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], wpb);
     }
 }
 template <typename EOS, bool FLAT>
@@ -1348,30 +1369,30 @@ static void launch_fluxes(grmhd_ctx* c, const double* aux, const double* ms, int
         else flux_rec<EosHybrid, false>(c, c->hybrid, aux, ms, mnv, B, s);
     }
 }
-// grids of the per-site kernels: x over the (n+1)^3 sites of a block, y over
-// blocks (chunks of 65535)
+// 1-D grids of the per-site kernels: wpb workgroups per block over its
+// (n+1)^3 sites, times the blocks
 template <typename F>
 static void site_launch(const DevGrid& G, int bs, F&& f) {
     const int sites = (G.n[0] + 1) * (G.n[1] + 1) * (G.n[2] + 1);
-    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535)
-        f(dim3(nblk_bs(sites, bs), (unsigned)std::min<int64_t>(65535, G.nq - q0)), (int)q0);
+    const int wpb = (int)nblk_bs(sites, bs);
+    f(dim3((unsigned)(wpb * G.nq)), wpb);
 }
 static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
     Scope sc(c, "emf", s);
     const DevGrid& G = c->G;
     if (c->P.emf == GRMHD_EMF_GS) {
-        site_launch(G, c->tune.emf, [&](dim3 g, int q0) {
+        site_launch(G, c->tune.emf, [&](dim3 g, int wpb) {
             k_emf_gs<<<g, c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
-                                       c->flux[2], c->emf, q0);
+                                       c->flux[2], c->emf, wpb);
         });
         return;
     }
-    site_launch(G, c->tune.emf, [&](dim3 g, int q0) {
+    site_launch(G, c->tune.emf, [&](dim3 g, int wpb) {
         switch (c->P.recon) {
-            case 0: k_emf_uct<0><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
-            case 1: k_emf_uct<1><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
-            case 2: k_emf_uct<2><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
-            default: k_emf_uct<3><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, q0); break;
+            case 0: k_emf_uct<0><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, wpb); break;
+            case 1: k_emf_uct<1><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, wpb); break;
+            case 2: k_emf_uct<2><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, wpb); break;
+            default: k_emf_uct<3><<<g, c->tune.emf, 0, s>>>(G, B[0], B[1], B[2], c->vbar[0], c->vbar[1], c->vbar[2], c->emf, wpb); break;
         }
     });
 }
@@ -1381,9 +1402,9 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     a.inst = c->inst_d; a.blk = c->blk_d;
     for (int d = 0; d < 3; ++d) a.flux[d] = c->flux[d];
     a.emf = c->emf;
-    site_launch(G, c->tune.update, [&](dim3 g, int q0) {
-        if (c->P.flat_metric) k_update<true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, q0);
-        else k_update<false><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, q0);
+    site_launch(G, c->tune.update, [&](dim3 g, int wpb) {
+        if (c->P.flat_metric) k_update<true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
+        else k_update<false><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
     });
 }
 static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
@@ -1392,12 +1413,10 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     if (a.B[0]) {
         Scope sc(c, "ghost_faces", s);
         const int nfg = (int)(c->nfg[0] + c->nfg[1] + c->nfg[2]);
-        for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-            dim3 grid(nblk_bs(nfg, c->tune.gather_faces), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-            k_gather_faces<<<grid, c->tune.gather_faces, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0],
-                                                (int)c->nfg[1], (int)c->nfg[2], (int)q0, a.B[0], a.B[1], a.B[2],
-                                                G.nface[0], G.nface[1], G.nface[2]);
-        }
+        const int wpb = (int)nblk_bs(nfg, c->tune.gather_faces);
+        k_gather_faces<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_faces, 0, s>>>(
+            c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0], (int)c->nfg[1], (int)c->nfg[2], wpb,
+            a.B[0], a.B[1], a.B[2], G.nface[0], G.nface[1], G.nface[2]);
     }
     if (a.cell_mode == 0) return;
     Scope sc(c, "ghost", s);
@@ -1416,10 +1435,9 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
             k_gather_list<<<dim3(nblk(c->n_own), 1, ch.nch), 256, 0, s>>>(c->own_gather, (int)c->n_own, ch);
         return;
     }
-    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-        dim3 grid(nblk_bs(c->ncg, c->tune.gather_cells), (unsigned)std::min<int64_t>(65535, G.nq - q0));
-        k_gather_cells<<<grid, c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)q0, a.cell_mode, ch);
-    }
+    const int wpb = (int)nblk_bs(c->ncg, c->tune.gather_cells);
+    k_gather_cells<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per,
+                                                                                 (int)c->ncg, wpb, a.cell_mode, ch);
 }
 
 // c2p over the cells of every block: the interior cells, plus either every
@@ -1433,18 +1451,18 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     const int ni = G.n[0] * G.n[1] * G.n[2];
     const int bs = c->tune.c2p;
     int64_t npart = 0;
-    auto launch = [&](int cset, int ne, dim3 grid, int qbase) {
+    auto launch = [&](int cset, int ne, dim3 grid, int wpb) {
         DevStats* part = with_stats ? c->stats_part + npart : nullptr;
         npart += (int64_t)grid.x * grid.y;
-#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<grid, bs, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, qbase)
+#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<grid, bs, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, wpb)
         if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
         else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
 #undef C2PL
     };
-    for (int64_t q0 = 0; q0 < G.nq; q0 += 65535) {
-        const unsigned nqq = (unsigned)std::min<int64_t>(65535, G.nq - q0);
-        launch(CS_INTERIOR, ni, dim3(nblk_bs(ni, bs), nqq), (int)q0);
-        if (!selective) launch(CS_SHELL, (int)c->ncg, dim3(nblk_bs(c->ncg, bs), nqq), (int)q0);
+    {
+        const int wi = (int)nblk_bs(ni, bs), ws = (int)nblk_bs(c->ncg, bs);
+        launch(CS_INTERIOR, ni, dim3((unsigned)(wi * G.nq)), wi);
+        if (!selective) launch(CS_SHELL, (int)c->ncg, dim3((unsigned)(ws * G.nq)), ws);
     }
     if (selective && c->n_own > 0) launch(CS_LIST, (int)c->n_own, dim3(nblk_bs(c->n_own, bs), 1), 0);
     if (with_stats) {
