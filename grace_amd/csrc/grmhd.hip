@@ -66,6 +66,8 @@ struct Opts {
     Atmo atmo; C2PPar cp; ExcPar ex;
     int riemann, emf, thermo, fd_order;
     double c_limit;
+    int fofc, dmp;      // first-order flux correction, its DMP test
+    double dmp_M;
 };
 
 GHD int64_t cidx(const DevGrid& G, int i, int j, int k) { return i + (int64_t)G.N[0] * (j + (int64_t)G.N[1] * k); }
@@ -110,7 +112,8 @@ template <typename EOS, int REC, bool FLAT, int D>
 __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
                                               const double* __restrict__ mstate, int mnv,
                                               const double* __restrict__ Bd, double* __restrict__ flux,
-                                              double* __restrict__ vbar, int wpb) {
+                                              double* __restrict__ vbar, int wpb,
+                                              const uint8_t* __restrict__ only) {
     // wpb workgroups per block (see wg_pos).  UCT: the first
     // (n_D+1) n_T0 n_T1 threads take the faces whose transverse indices are
     // both interior (full fluxes), the next ones the faces with exactly one
@@ -131,11 +134,14 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
         const int n0 = G.n[T0], n1 = G.n[T1];
         int a = 0, b = 0, c = 0;   // (normal, t0, t1)
         if (o.emf != 0) {
+            // GS: every face; FOFC widens the normal range by one face on each side
             const int N0 = G.N[T0], N1 = G.N[T1];
-            if (e >= Rn * N0 * N1) return;
-            if (D == 0) { a = e % Rn; const int r = e / Rn; b = r % N0; c = r / N0; }
-            else if (D == 1) { b = e % N0; const int r = e / N0; a = r % Rn; c = r / Rn; }
+            const int Rg = Rn + 2 * o.fofc;
+            if (e >= Rg * N0 * N1) return;
+            if (D == 0) { a = e % Rg; const int r = e / Rg; b = r % N0; c = r / N0; }
+            else if (D == 1) { b = e % N0; const int r = e / N0; a = r % Rg; c = r / Rg; }
             else { b = e % N0; const int r = e / N0; c = r % N1; a = r / N1; }
+            a -= o.fofc;
         } else {
             const int nint = Rn * n0 * n1, g2 = 2 * ng;
             const int nA = Rn * g2 * n1, nB = Rn * n0 * g2;
@@ -164,6 +170,9 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
         if (D == 0) { i = a; j = b; k = c; }
         else if (D == 1) { i = b; j = a; k = c; }
         else { i = b; j = c; k = a; }
+        // FOFC correction pass: only the tagged faces of the normal range
+        // [ng, n+ng] (evolve.cpp:646-702)
+        if (only && (a < ng || a > ng + G.n[D] || !only[q * G.nface[D] + fidx<D>(G, i, j, k)])) return;
     }
     const int64_t nc = G.ncell;
     const double* A = aux + q * NAUX * nc;
@@ -392,8 +401,10 @@ __global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __rest
 __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restrict__ ef0, const double* __restrict__ ef1,
                                                 const double* __restrict__ ef2, const double* __restrict__ ec,
                                                 const double* __restrict__ f0, const double* __restrict__ f1,
-                                                const double* __restrict__ f2, double* __restrict__ emf, int wpb) {
-    // wpb workgroups per block over its (n+1)^3 sites (see wg_pos)
+                                                const double* __restrict__ f2, double* __restrict__ emf, int wpb,
+                                                const uint8_t* __restrict__ only) {
+    // wpb workgroups per block over its (n+1)^3 sites (see wg_pos); only !=
+    // null: FOFC correction of the tagged edges (evolve.cpp:780-808)
     const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
     const WgPos wp = wg_pos(wpb);
     const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
@@ -416,7 +427,7 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
     (void)nfx; (void)nfy; (void)nfz;
     double* E = emf + q * 3 * G.nedge;
     const int64_t eo = eidx(G, i, j, k);
-    if (i < G.n[0] + G.ng) {
+    if (i < G.n[0] + G.ng && (!only || only[(q * 3 + 0) * G.nedge + eo])) {
         double Exzf = EF(2, 0, i, j, k), Exzfm = EF(2, 0, i, j - 1, k), Exyf = EF(1, 0, i, j, k), Exyfm = EF(1, 0, i, j, k - 1);
         double NE = EC(0, i, j, k), NW = EC(0, i, j - 1, k), SE = EC(0, i, j, k - 1), SW = EC(0, i, j - 1, k - 1);
         double Eavg = 0.25 * (Exzf + Exzfm + Exyf + Exyfm);
@@ -428,7 +439,7 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
         double dW = (1. - Szm) * (Exyf - NW) + (1. + Szm) * (Exyfm - SW);
         E[eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
     }
-    if (j < G.n[1] + G.ng) {
+    if (j < G.n[1] + G.ng && (!only || only[(q * 3 + 1) * G.nedge + eo])) {
         double Eyzf = EF(2, 1, i, j, k), Eyzfm = EF(2, 1, i - 1, j, k), Eyxf = EF(0, 0, i, j, k), Eyxfm = EF(0, 0, i, j, k - 1);
         double NE = EC(1, i, j, k), NW = EC(1, i - 1, j, k), SE = EC(1, i, j, k - 1), SW = EC(1, i - 1, j, k - 1);
         double Eavg = 0.25 * (Eyzf + Eyzfm + Eyxf + Eyxfm);
@@ -440,7 +451,7 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
         double dW = (1. - Szm) * (Eyxf - NW) + (1. + Szm) * (Eyxfm - SW);
         E[G.nedge + eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
     }
-    if (k < G.n[2] + G.ng) {
+    if (k < G.n[2] + G.ng && (!only || only[(q * 3 + 2) * G.nedge + eo])) {
         double Ezyf = EF(1, 1, i, j, k), Ezyfm = EF(1, 1, i - 1, j, k), Ezxf = EF(0, 1, i, j, k), Ezxfm = EF(0, 1, i, j - 1, k);
         double NE = EC(2, i, j, k), NW = EC(2, i - 1, j, k), SE = EC(2, i, j - 1, k), SW = EC(2, i - 1, j - 1, k);
         double Eavg = 0.25 * (Ezyf + Ezyfm + Ezxf + Ezxfm);
@@ -614,6 +625,137 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
 // Ghost gather.  Index space per block: [cells | x faces | y faces | z faces].
 // Each ghost element copies the element at the same global position (clamped
 // for outflow, wrapped for periodic) from the owning block.
+// ---------------------------------------------------------------------------
+// First-order flux correction (evolve.cpp:456-811)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t* __restrict__ p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+struct FofcArgs {
+    const double* ast; int anv;        // the stage's linear combination lin = ca A + cb B
+    const double* bst; int bnv;
+    double ca, cb;
+    const double* mstate; int mnv;     // metric source
+    const double* flux[3];
+    const double* emf;
+    const double* oB[3];               // stage input face B
+    const double* dt; double dtfact;
+    const DevInst* inst; const DevBlk* blk;
+    uint8_t* ftag[3]; uint8_t* etag;
+};
+
+// flag_fofc_cells (evolve.cpp:456-633): every cell of [ng-1, n+ng+1)^3 of
+// every block.  The tentative state (lin + dt dtfact div F, the stage input
+// face B advanced by the edge EMFs) is formed with separately rounded
+// products and sums, the way the scalar reference code evaluates it, so that
+// the discrete tests below see the same values.  A cell whose tentative D or
+// tau leaves [min/M, M max] of the 27-cell neighbourhood of lin, or whose
+// lenient dry-run c2p floors, tags its 6 faces and 12 edges (plain byte
+// stores: every writer stores 1).
+template <typename EOS, bool FLAT>
+__global__ __launch_bounds__(256) void k_fofc_flag(DevGrid G, EOS eos, Opts o, FofcArgs a, int wpb) {
+    const WgPos wp = wg_pos(wpb);
+    const int m0 = G.n[0] + 2, m1 = G.n[1] + 2, m2 = G.n[2] + 2;
+    const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
+    if (r >= m0 * m1 * m2) return;
+    const int64_t q = wp.q;
+    const int kz = r / (m0 * m1), rxy = r - kz * (m0 * m1);
+    const int i = G.ng - 1 + rxy % m0, j = G.ng - 1 + rxy / m0, k = G.ng - 1 + kz;
+    const int64_t nc = G.ncell;
+    const int64_t c = cidx(G, i, j, k);
+    const DevInst& I = a.inst[a.blk[q].inst];
+    const double ix = I.idx[0], iy = I.idx[1], iz = I.idx[2];
+    const double dt = *a.dt;
+    const double dte = __dmul_rn(dt, a.dtfact);
+    const double* A = a.ast + q * (int64_t)a.anv * nc;
+    const double* Bs = a.bst ? a.bst + q * (int64_t)a.bnv * nc : nullptr;
+    auto lin = [&](int v, int64_t cc) {
+        return a.cb == 0.0 ? A[v * nc + cc] : __dadd_rn(__dmul_rn(a.ca, A[v * nc + cc]), __dmul_rn(a.cb, Bs[v * nc + cc]));
+    };
+    const int64_t f0 = fidx<0>(G, i, j, k), f1 = fidx<1>(G, i, j, k), f2 = fidx<2>(G, i, j, k);
+    const int64_t nf0 = G.nface[0], nf1 = G.nface[1], nf2 = G.nface[2];
+    const double* F0 = a.flux[0] + q * NHRSC * nf0;
+    const double* F1 = a.flux[1] + q * NHRSC * nf1;
+    const double* F2 = a.flux[2] + q * NHRSC * nf2;
+    const int64_t s1 = G.N[0], s2 = (int64_t)G.N[0] * G.N[1];
+    double cons[NCONS];
+    for (int v = 0; v < NHRSC; ++v) {
+        const double dFx = __dmul_rn(__dsub_rn(F0[v * nf0 + f0], F0[v * nf0 + f0 + 1]), ix);
+        const double dFy = __dmul_rn(__dsub_rn(F1[v * nf1 + f1], F1[v * nf1 + f1 + s1]), iy);
+        const double dFz = __dmul_rn(__dsub_rn(F2[v * nf2 + f2], F2[v * nf2 + f2 + s2]), iz);
+        cons[v] = __dadd_rn(lin(v, c), __dmul_rn(dte, __dadd_rn(__dadd_rn(dFx, dFy), dFz)));
+    }
+    // ct_update_B{x,y,z}_face (grmhd_helpers.hh:441-475)
+    const double* E = a.emf + q * 3 * G.nedge;
+    auto Ex = [&](int ii, int jj, int kk) { return E[eidx(G, ii, jj, kk)]; };
+    auto Ey = [&](int ii, int jj, int kk) { return E[G.nedge + eidx(G, ii, jj, kk)]; };
+    auto Ez = [&](int ii, int jj, int kk) { return E[2 * G.nedge + eidx(G, ii, jj, kk)]; };
+    const double* bx = a.oB[0] + q * nf0;
+    const double* by = a.oB[1] + q * nf1;
+    const double* bz = a.oB[2] + q * nf2;
+    auto ctx = [&](int ii) {
+        return __dadd_rn(bx[fidx<0>(G, ii, j, k)],
+                         __dmul_rn(dte, __dadd_rn(__dmul_rn(__dsub_rn(Ey(ii, j, k + 1), Ey(ii, j, k)), iz),
+                                                  __dmul_rn(__dsub_rn(Ez(ii, j, k), Ez(ii, j + 1, k)), iy))));
+    };
+    auto cty = [&](int jj) {
+        return __dadd_rn(by[fidx<1>(G, i, jj, k)],
+                         __dmul_rn(dte, __dadd_rn(__dmul_rn(__dsub_rn(Ez(i + 1, jj, k), Ez(i, jj, k)), ix),
+                                                  __dmul_rn(__dsub_rn(Ex(i, jj, k), Ex(i, jj, k + 1)), iz))));
+    };
+    auto ctz = [&](int kk) {
+        return __dadd_rn(bz[fidx<2>(G, i, j, kk)],
+                         __dmul_rn(dte, __dadd_rn(__dmul_rn(__dsub_rn(Ex(i, j + 1, kk), Ex(i, j, kk)), iy),
+                                                  __dmul_rn(__dsub_rn(Ey(i, j, kk), Ey(i + 1, j, kk)), ix))));
+    };
+    cons[BSXL] = 0.5 * __dadd_rn(ctx(i), ctx(i + 1));
+    cons[BSYL] = 0.5 * __dadd_rn(cty(j), cty(j + 1));
+    cons[BSZL] = 0.5 * __dadd_rn(ctz(k), ctz(k + 1));
+    bool dmp = false;
+    if (o.dmp) {
+        const double inv_M = 1.0 / o.dmp_M;
+        const int vars[2] = {DENS_, TAU_};
+        for (int n = 0; n < 2 && !dmp; ++n) {
+            double vmax = -1.7976931348623157e308, vmin = 1.7976931348623157e308;
+            for (int kt = -1; kt <= 1; ++kt)
+                for (int jt = -1; jt <= 1; ++jt)
+                    for (int it = -1; it <= 1; ++it) {
+                        const double v = lin(vars[n], cidx(G, i + it, j + jt, k + kt));
+                        vmax = fmax(vmax, v);
+                        vmin = fmin(vmin, v);
+                    }
+            const double test = cons[vars[n]];
+            if (test > __dmul_rn(o.dmp_M, vmax) || test < __dmul_rn(vmin, inv_M)) dmp = true;
+        }
+    }
+    Met m;
+    if constexpr (FLAT) m.flat();
+    else load_metric(m, a.mstate + q * (int64_t)a.mnv * nc, nc, c);
+    C2PPar cp = o.cp;
+    cp.alp_bh_thresh = 1e30;   // lenient everywhere during the dry run (evolve.cpp:496-500)
+    double pr[NPRIM];
+    uint64_t err = 0;
+    C2PCount cnt;
+    C2PGuess gs{0, 0, 0, 0, 0, 0};
+    C2PRoots rt{-1, -1, -1, -1};
+    const C2POut out = c2p(cons, pr, m, eos, o.atmo, o.ex, cp, 1e300, err, true, cnt, gs, rt);
+    if (!(out.floored || dmp)) return;
+    for (int t = 0; t < 2; ++t) {
+        a.ftag[0][q * nf0 + fidx<0>(G, i + t, j, k)] = 1;
+        a.ftag[1][q * nf1 + fidx<1>(G, i, j + t, k)] = 1;
+        a.ftag[2][q * nf2 + fidx<2>(G, i, j, k + t)] = 1;
+    }
+    uint8_t* T = a.etag + q * 3 * G.nedge;
+    for (int u = 0; u < 2; ++u)
+        for (int w = 0; w < 2; ++w) {
+            T[eidx(G, i, j + u, k + w)] = 1;
+            T[G.nedge + eidx(G, i + u, j, k + w)] = 1;
+            T[2 * G.nedge + eidx(G, i + u, j + w, k)] = 1;
+        }
+}
+
 // ---------------------------------------------------------------------------
 GHD int map_cc(int g, int N, int lo, int hi) {
     if (g < 0) return lo == 1 ? ((g % N) + N) % N : 0;
@@ -1072,6 +1214,8 @@ struct grmhd_ctx {
     int64_t n_part_cap = 0;
     int32_t* ghost_list = nullptr;
     int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
+    uint8_t* ftag[3] = {nullptr, nullptr, nullptr};   // FOFC face tags (per face of every block)
+    uint8_t* etag = nullptr;                  // FOFC edge tags, 3 components
     int2* own_cells = nullptr;                // {block, cell} of ghosts that run their own c2p
     int4* own_gather = nullptr;               // the same with their sources
     int64_t n_own = 0;
@@ -1122,6 +1266,7 @@ extern "C" void grmhd_default_params(grmhd_params* p) {
     p->recon = GRMHD_RECON_WENOZ; p->riemann = GRMHD_RIEMANN_HLLE; p->emf = GRMHD_EMF_UCT;
     p->thermo_recon = GRMHD_THERMO_TEMP; p->fd_order = 6; p->rusanov_c_limit = 0.0;
     p->flat_metric = 0;
+    p->fofc = 0; p->fofc_dmp_enable = 1; p->fofc_dmp_M = 1.2;
 }
 
 static void build_eos(grmhd_ctx* c) {
@@ -1154,6 +1299,7 @@ static void build_eos(grmhd_ctx* c) {
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
+    F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1178,6 +1324,10 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
     if (p->eos_kind == GRMHD_EOS_HYBRID_PWPOLY && (p->pwp_n < 1 || p->pwp_n > 8)) { set_err("pwp_n"); return GRMHD_ERR_INVALID; }
     if (p->eos_kind == GRMHD_EOS_IDEAL_GAS && !(p->gamma > 1.0)) { set_err("gamma must be > 1"); return GRMHD_ERR_INVALID; }
     if (p->fd_order / 2 > g->ngz) { set_err("fd_order needs more ghosts"); return GRMHD_ERR_INVALID; }
+    // evolve.cpp:476-481: FOFC needs the GS EMF and one ghost layer beyond the 6-point stencil
+    if (p->fofc && p->emf != GRMHD_EMF_GS) { set_err("FOFC is not supported with UCT"); return GRMHD_ERR_UNSUPPORTED; }
+    if (p->fofc && g->ngz < 4) { set_err("FOFC requires ngz >= 4"); return GRMHD_ERR_INVALID; }
+    if (p->fofc && !(p->fofc_dmp_M > 0)) { set_err("fofc_dmp_M must be > 0"); return GRMHD_ERR_INVALID; }
     // instance / block tables
     std::vector<DevBlk> blk(g->nq, DevBlk{-1, {0, 0, 0}});
     for (int a = 0; a < n_inst; ++a) {
@@ -1211,6 +1361,7 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
     O.ex = ExcPar{p->excision_radius, p->excision_lapse, p->excise_by_radius};
     O.riemann = p->riemann; O.emf = p->emf; O.thermo = p->thermo_recon; O.fd_order = p->fd_order;
     O.c_limit = p->rusanov_c_limit;
+    O.fofc = p->fofc; O.dmp = p->fofc_dmp_enable; O.dmp_M = p->fofc_dmp_M;
     build_eos(c);
     c->blk_h = blk;
     for (int a = 0; a < n_inst; ++a) {
@@ -1237,6 +1388,11 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
         if ((e = hipMalloc(&c->vbar[d], sizeof(double) * 4 * G.nface[d] * G.nq)) != hipSuccess) return fail("vbar", e);
     }
     if ((e = hipMalloc(&c->emf, sizeof(double) * 3 * G.nedge * G.nq)) != hipSuccess) return fail("emf", e);
+    if (p->fofc) {
+        for (int d = 0; d < 3; ++d)
+            if ((e = hipMalloc(&c->ftag[d], G.nface[d] * G.nq)) != hipSuccess) return fail("fofc face tags", e);
+        if ((e = hipMalloc(&c->etag, 3 * G.nedge * G.nq)) != hipSuccess) return fail("fofc edge tags", e);
+    }
     if ((e = hipMalloc(&c->ecen, sizeof(double) * 3 * G.ncell * G.nq)) != hipSuccess) return fail("ecen", e);
     // E_center starts at zero, as the reference's freshly allocated view
     if ((e = hipMemset(c->ecen, 0, sizeof(double) * 3 * G.ncell * G.nq)) != hipSuccess) return fail("ecen clear", e);
@@ -1326,47 +1482,58 @@ static grmhd_status launch_check(const char* what) {
 }
 
 // ---- dispatch helpers -------------------------------------------------------
+// only != null: FOFC correction pass over the tagged faces with the options O
+// (donor cell + LLF); otherwise every face the later passes read.
 template <typename EOS, int REC, bool FLAT>
 static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
-                         const double* const B[3], hipStream_t s) {
+                         const double* const B[3], hipStream_t s, const Opts& O, uint8_t* const* only) {
     const DevGrid& G = c->G;
     const bool uct = c->P.emf != GRMHD_EMF_GS;
     int nf[3];
     for (int d = 0; d < 3; ++d) {
         const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
         const int Rn = G.n[d] + 1, n0 = G.n[t0], n1 = G.n[t1];
-        nf[d] = uct ? Rn * (n0 * n1 + 2 * G.ng * (n0 + n1)) : Rn * G.N[t0] * G.N[t1];
+        nf[d] = uct ? Rn * (n0 * n1 + 2 * G.ng * (n0 + n1)) : (Rn + 2 * O.fofc) * G.N[t0] * G.N[t1];
     }
     static const char* fam[3] = {"flux_x", "flux_y", "flux_z"};
     for (int d = 0; d < 3; ++d) {
-        Scope sc(c, fam[d], s);
+        Scope sc(c, only ? "fofc" : fam[d], s);
         const int bs = c->tune.flux[d];
         const int wpb = (int)nblk_bs(nf[d], bs);
         const dim3 grid((unsigned)(wpb * G.nq));
-        if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], wpb);
-        else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], wpb);
-        else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, c->O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], wpb);
+        const uint8_t* od = only ? only[d] : nullptr;
+        if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], wpb, od);
+        else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], wpb, od);
+        else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], wpb, od);
     }
 }
 template <typename EOS, bool FLAT>
 static void flux_rec(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
-                     const double* const B[3], hipStream_t s) {
-    switch (c->P.recon) {
-        case 0: launch_flux3<EOS, 0, FLAT>(c, eos, aux, ms, mnv, B, s); break;
-        case 1: launch_flux3<EOS, 1, FLAT>(c, eos, aux, ms, mnv, B, s); break;
-        case 2: launch_flux3<EOS, 2, FLAT>(c, eos, aux, ms, mnv, B, s); break;
-        default: launch_flux3<EOS, 3, FLAT>(c, eos, aux, ms, mnv, B, s); break;
+                     const double* const B[3], hipStream_t s, int recon, const Opts& O, uint8_t* const* only) {
+    switch (recon) {
+        case 0: launch_flux3<EOS, 0, FLAT>(c, eos, aux, ms, mnv, B, s, O, only); break;
+        case 1: launch_flux3<EOS, 1, FLAT>(c, eos, aux, ms, mnv, B, s, O, only); break;
+        case 2: launch_flux3<EOS, 2, FLAT>(c, eos, aux, ms, mnv, B, s, O, only); break;
+        default: launch_flux3<EOS, 3, FLAT>(c, eos, aux, ms, mnv, B, s, O, only); break;
     }
 }
 static void launch_fluxes(grmhd_ctx* c, const double* aux, const double* ms, int mnv, const double* const B[3],
-                          hipStream_t s) {
-    bool flat = c->P.flat_metric != 0;
+                          hipStream_t s, bool fofc_pass = false) {
+    const bool flat = c->P.flat_metric != 0;
+    Opts O = c->O;
+    int recon = c->P.recon;
+    uint8_t* const* only = nullptr;
+    if (fofc_pass) {   // apply_fofc_correction: donor cell + LLF (evolve.cpp:749-771)
+        O.riemann = GRMHD_RIEMANN_LLF;
+        recon = GRMHD_RECON_DONOR;
+        only = c->ftag;
+    }
     if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) {
-        if (flat) flux_rec<EosIdeal, true>(c, c->ideal, aux, ms, mnv, B, s);
-        else flux_rec<EosIdeal, false>(c, c->ideal, aux, ms, mnv, B, s);
+        if (flat) flux_rec<EosIdeal, true>(c, c->ideal, aux, ms, mnv, B, s, recon, O, only);
+        else flux_rec<EosIdeal, false>(c, c->ideal, aux, ms, mnv, B, s, recon, O, only);
     } else {
-        if (flat) flux_rec<EosHybrid, true>(c, c->hybrid, aux, ms, mnv, B, s);
-        else flux_rec<EosHybrid, false>(c, c->hybrid, aux, ms, mnv, B, s);
+        if (flat) flux_rec<EosHybrid, true>(c, c->hybrid, aux, ms, mnv, B, s, recon, O, only);
+        else flux_rec<EosHybrid, false>(c, c->hybrid, aux, ms, mnv, B, s, recon, O, only);
     }
 }
 // 1-D grids of the per-site kernels: wpb workgroups per block over its
@@ -1383,7 +1550,7 @@ static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
     if (c->P.emf == GRMHD_EMF_GS) {
         site_launch(G, c->tune.emf, [&](dim3 g, int wpb) {
             k_emf_gs<<<g, c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
-                                       c->flux[2], c->emf, wpb);
+                                       c->flux[2], c->emf, wpb, nullptr);
         });
         return;
     }
@@ -1471,6 +1638,42 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     }
 }
 
+// flag_fofc_cells + apply_fofc_correction for one substep: clear the tags,
+// flag, then recompute the tagged faces (donor + LLF) and GS edge EMFs.
+// lin = ca A + cb B is the stage's linear combination (cb = 0: A alone).
+static void launch_fofc(grmhd_ctx* c, const double* ast, int anv, const double* bst, int bnv, double ca, double cb,
+                        const double* ms, const double* const oB[3], const double* aux, const double* dt_dev,
+                        double dtfact, hipStream_t s) {
+    Scope sc(c, "fofc", s);
+    const DevGrid& G = c->G;
+    for (int d = 0; d < 3; ++d)
+        k_fill_u8<<<nblk(G.nface[d] * G.nq), 256, 0, s>>>(c->ftag[d], G.nface[d] * G.nq);
+    k_fill_u8<<<nblk(3 * G.nedge * G.nq), 256, 0, s>>>(c->etag, 3 * G.nedge * G.nq);
+    FofcArgs a{};
+    a.ast = ast; a.anv = anv; a.bst = bst; a.bnv = bnv; a.ca = ca; a.cb = cb;
+    a.mstate = ms; a.mnv = NSTATE;
+    for (int d = 0; d < 3; ++d) { a.flux[d] = c->flux[d]; a.oB[d] = oB[d]; a.ftag[d] = c->ftag[d]; }
+    a.emf = c->emf; a.etag = c->etag;
+    a.dt = dt_dev; a.dtfact = dtfact;
+    a.inst = c->inst_d; a.blk = c->blk_d;
+    const int cells = (G.n[0] + 2) * (G.n[1] + 2) * (G.n[2] + 2);
+    const int wpb = (int)nblk(cells);
+    const dim3 grid((unsigned)(wpb * G.nq));
+    const bool flat = c->P.flat_metric != 0;
+    if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) {
+        if (flat) k_fofc_flag<EosIdeal, true><<<grid, 256, 0, s>>>(G, c->ideal, c->O, a, wpb);
+        else k_fofc_flag<EosIdeal, false><<<grid, 256, 0, s>>>(G, c->ideal, c->O, a, wpb);
+    } else {
+        if (flat) k_fofc_flag<EosHybrid, true><<<grid, 256, 0, s>>>(G, c->hybrid, c->O, a, wpb);
+        else k_fofc_flag<EosHybrid, false><<<grid, 256, 0, s>>>(G, c->hybrid, c->O, a, wpb);
+    }
+    launch_fluxes(c, aux, ms, NSTATE, oB, s, true);
+    site_launch(G, c->tune.emf, [&](dim3 g, int w) {
+        k_emf_gs<<<g, c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
+                                           c->flux[2], c->emf, w, c->etag);
+    });
+}
+
 // ---- ABI entry points -------------------------------------------------------
 extern "C" grmhd_status grmhd_advance_substep(grmhd_ctx* c, const double* dt_dev, double dtfact, double* new_state,
                                               const double* old_state, double* const new_B[3],
@@ -1479,6 +1682,8 @@ extern "C" grmhd_status grmhd_advance_substep(grmhd_ctx* c, const double* dt_dev
     hipStream_t s = (hipStream_t)stream;
     launch_fluxes(c, aux, old_state, NSTATE, old_B, s);
     launch_emf(c, old_B, s);
+    if (c->P.fofc)
+        launch_fofc(c, new_state, NSTATE, nullptr, 0, 1.0, 0.0, old_state, old_B, aux, dt_dev, dtfact, s);
     UpdArgs a{};
     a.nst = new_state; a.nnv = NSTATE;
     a.ast = new_state; a.anv = NSTATE; a.bst = nullptr; a.bnv = 0; a.ca = 1.0; a.cb = 0.0;
@@ -1681,6 +1886,8 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         Stage& S = st[sidx];
         launch_fluxes(c, aux, state, NSTATE, S.oB, s);
         launch_emf(c, S.oB, s);
+        if (c->P.fofc)
+            launch_fofc(c, S.ast, S.anv, S.bst, S.bnv, S.ca, S.cb, state, S.oB, aux, dt_dev, S.dtf, s);
         UpdArgs a{};
         a.nst = S.nst; a.nnv = S.nnv; a.ast = S.ast; a.anv = S.anv; a.bst = S.bst; a.bnv = S.bnv;
         a.ca = S.ca; a.cb = S.cb; a.mstate = state; a.mnv = NSTATE; a.aux = aux;
@@ -1688,8 +1895,9 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         a.dt = dt_dev; a.dtfact = S.dtf;
         launch_update(c, a, s);
         {   // ghost faces everywhere; conserved only where a ghost runs its own c2p
+            // (everywhere with FOFC: its next-stage tests read the ghost layer)
             GatherArgs g{};
-            g.st = S.nst; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC; g.cell_mode = 2;
+            g.st = S.nst; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC; g.cell_mode = c->P.fofc ? 1 : 2;
             for (int d = 0; d < 3; ++d) g.B[d] = S.nB[d];
             launch_gather(c, g, s);
         }

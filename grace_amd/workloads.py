@@ -45,7 +45,7 @@ def _p(a):
 
 
 def blast(nb=(8, 8, 8), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01, emf=EMF_UCT, recon=0, riemann=0,
-          flat=True):
+          flat=True, rho=1e-2, p=1e-3, p_in=1.0, r_in=0.1, Bx=0.1, fofc=0):
     """Magnetised blast wave: 128^3 on [-0.5,0.5]^3 in 16^3 blocks by default.
     Ambient rho=1e-2, p=1e-3; p=1.0 for r<0.1; B=(0.1,0,0); +-1% noise."""
     nq = nb[0] * nb[1] * nb[2]
@@ -56,11 +56,12 @@ def blast(nb=(8, 8, 8), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01, emf
     state, B, aux = alloc(grid)
     lo = np.array([-0.5, -0.5, -0.5])
     hi = np.array([0.5, 0.5, 0.5])
-    Bv = np.array([0.1, 0.0, 0.0])
-    rc = lib().grmhd_setup_blast(grid, inst, 1, gamma, _p(lo), _p(hi), 1e-2, 1e-3, 1.0, 0.1, _p(Bv), noise, seed,
+    Bv = np.array([Bx, 0.0, 0.0])
+    rc = lib().grmhd_setup_blast(grid, inst, 1, gamma, _p(lo), _p(hi), rho, p, p_in, r_in, _p(Bv), noise, seed,
                                  _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux))
     assert rc == 0
-    params = default_params(gamma=gamma, emf=emf, recon=recon, riemann=riemann, flat_metric=1 if flat else 0)
+    params = default_params(gamma=gamma, emf=emf, recon=recon, riemann=riemann, flat_metric=1 if flat else 0,
+                            fofc=fofc)
     return Workload("blast", params, grid, spec, state, B, aux, zones=nq * n ** 3,
                     extra=dict(N=N, dx=1.0 / N[0]))
 

@@ -80,6 +80,12 @@ typedef struct {
     /* 1: the metric is Minkowski everywhere; kernels skip the 16 metric
      * arrays and the source terms (identically zero there). */
     int32_t flat_metric;
+    /* first-order flux correction (evolve.cpp:456-811; grmhd.fofc.dmp_enable,
+     * dmp_M, parameters/grmhd.yaml:162-174).  Needs the GS EMF scheme and
+     * ngz >= 4, as in the reference. */
+    int32_t fofc;
+    int32_t fofc_dmp_enable;
+    double fofc_dmp_M;
 } grmhd_params;
 
 /* Block grid: every block has the same interior extents. */

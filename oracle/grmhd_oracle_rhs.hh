@@ -33,6 +33,9 @@ enum BCKind { BC_OUTFLOW = 0, BC_PERIODIC = 1 };
 struct Options {
     int recon = REC_WENOZ, riemann = RS_HLLE, emf = EMF_UCT, thermo = TH_TEMP, fd_order = 6;
     double rusanov_c_limit = 0.0;
+    // first-order flux correction (evolve.cpp:456-811, grmhd_helpers.hh:63-75,226-231)
+    int fofc = 0, dmp_enable = 1;
+    double dmp_M = 1.2;
 };
 
 // Block grid.  All blocks share (nx,ny,nz,ngz).
@@ -324,7 +327,8 @@ struct Stepper {
             for (int d = 0; d < 3; ++d) {
                 int lo[3] = {0, 0, 0}, hi[3] = {g.Ni, g.Nj, g.Nk};
                 int n[3] = {g.nx, g.ny, g.nz};
-                lo[d] = g.ngz; hi[d] = n[d] + g.ngz + 1;
+                // FOFC needs one more face on each side along the normal (evolve.cpp:895-920)
+                lo[d] = g.ngz - (opt.fofc ? 1 : 0); hi[d] = n[d] + g.ngz + 1 + (opt.fofc ? 1 : 0);
 #pragma omp parallel for collapse(2) schedule(static)
                 for (int k = lo[2]; k < hi[2]; ++k)
                     for (int j = lo[1]; j < hi[1]; ++j)
@@ -408,6 +412,56 @@ struct Stepper {
     double Ec(int c, int i, int j, int k, int64_t q) const { return ecen[G().cc(i, j, k, c, q, 3)]; }
     double Fd(int fd, int i, int j, int k, int64_t q) const { return flux[fd][G().fc(fd, i, j, k, DENS_, q, NHRSC)]; }
 
+    // Gardiner-Stone edge EMF of component c at edge (i,j,k) (grmhd_helpers.hh:492-627)
+    double gs_edge(int c, int i, int j, int k, int64_t q) const {
+        if (c == 0) {
+            double Exzf = Ef(2, 0, i, j, k, q), Exzfm = Ef(2, 0, i, j - 1, k, q);
+            double Exyf = Ef(1, 0, i, j, k, q), Exyfm = Ef(1, 0, i, j, k - 1, q);
+            double NE = Ec(0, i, j, k, q), NW = Ec(0, i, j - 1, k, q);
+            double SE = Ec(0, i, j, k - 1, q), SW = Ec(0, i, j - 1, k - 1, q);
+            double Eavg = 0.25 * (Exzf + Exzfm + Exyf + Exyfm);
+            double Sy = std::copysign(1., Fd(1, i, j, k, q)), Sym = std::copysign(1., Fd(1, i, j, k - 1, q));
+            double dN = (1. - Sy) * (NE - Exzf) + (1. + Sy) * (NW - Exzfm);
+            double dS = (1. - Sym) * (Exzf - SE) + (1. + Sym) * (Exzfm - SW);
+            double dEdz = (1. / 8.) * (dS - dN);
+            double Sz = std::copysign(1., Fd(2, i, j, k, q)), Szm = std::copysign(1., Fd(2, i, j - 1, k, q));
+            double dE = (1. - Sz) * (NE - Exyf) + (1. + Sz) * (SE - Exyfm);
+            double dW = (1. - Szm) * (Exyf - NW) + (1. + Szm) * (Exyfm - SW);
+            double dEdy = (1. / 8.) * (dW - dE);
+            return Eavg + dEdz + dEdy;
+        }
+        if (c == 1) {
+            double Eyzf = Ef(2, 1, i, j, k, q), Eyzfm = Ef(2, 1, i - 1, j, k, q);
+            double Eyxf = Ef(0, 0, i, j, k, q), Eyxfm = Ef(0, 0, i, j, k - 1, q);
+            double NE = Ec(1, i, j, k, q), NW = Ec(1, i - 1, j, k, q);
+            double SE = Ec(1, i, j, k - 1, q), SW = Ec(1, i - 1, j, k - 1, q);
+            double Eavg = 0.25 * (Eyzf + Eyzfm + Eyxf + Eyxfm);
+            double Sx = std::copysign(1., Fd(0, i, j, k, q)), Sxm = std::copysign(1., Fd(0, i, j, k - 1, q));
+            double dN = (1. - Sx) * (NE - Eyzf) + (1. + Sx) * (NW - Eyzfm);
+            double dS = (1. - Sxm) * (Eyzf - SE) + (1. + Sxm) * (Eyzfm - SW);
+            double dEdz = (1. / 8.) * (dS - dN);
+            double Sz = std::copysign(1., Fd(2, i, j, k, q)), Szm = std::copysign(1., Fd(2, i - 1, j, k, q));
+            double dE = (1. - Sz) * (NE - Eyxf) + (1. + Sz) * (SE - Eyxfm);
+            double dW = (1. - Szm) * (Eyxf - NW) + (1. + Szm) * (Eyxfm - SW);
+            double dEdx = (1. / 8.) * (dW - dE);
+            return Eavg + dEdz + dEdx;
+        }
+        double Ezyf = Ef(1, 1, i, j, k, q), Ezyfm = Ef(1, 1, i - 1, j, k, q);
+        double Ezxf = Ef(0, 1, i, j, k, q), Ezxfm = Ef(0, 1, i, j - 1, k, q);
+        double NE = Ec(2, i, j, k, q), NW = Ec(2, i - 1, j, k, q);
+        double SE = Ec(2, i, j - 1, k, q), SW = Ec(2, i - 1, j - 1, k, q);
+        double Eavg = 0.25 * (Ezyf + Ezyfm + Ezxf + Ezxfm);
+        double Sx = std::copysign(1., Fd(0, i, j, k, q)), Sxm = std::copysign(1., Fd(0, i, j - 1, k, q));
+        double dN = (1. - Sx) * (NE - Ezyf) + (1. + Sx) * (NW - Ezyfm);
+        double dS = (1. - Sxm) * (Ezyf - SE) + (1. + Sxm) * (Ezyfm - SW);
+        double dEdy = (1. / 8.) * (dS - dN);
+        double Sy = std::copysign(1., Fd(1, i, j, k, q)), Sym = std::copysign(1., Fd(1, i - 1, j, k, q));
+        double dE = (1. - Sy) * (NE - Ezxf) + (1. + Sy) * (SE - Ezxfm);
+        double dW = (1. - Sym) * (Ezxf - NW) + (1. + Sym) * (Ezxfm - SW);
+        double dEdx = (1. / 8.) * (dW - dE);
+        return Eavg + dEdy + dEdx;
+    }
+
     void compute_emfs_gs() {
         const Grid& g = G();
         int ng = g.ngz;
@@ -416,54 +470,9 @@ struct Stepper {
             for (int k = ng; k < g.nz + ng + 1; ++k)
                 for (int j = ng; j < g.ny + ng + 1; ++j)
                     for (int i = ng; i < g.nx + ng + 1; ++i) {
-                        if (i < g.nx + ng) {
-                            double Exzf = Ef(2, 0, i, j, k, q), Exzfm = Ef(2, 0, i, j - 1, k, q);
-                            double Exyf = Ef(1, 0, i, j, k, q), Exyfm = Ef(1, 0, i, j, k - 1, q);
-                            double NE = Ec(0, i, j, k, q), NW = Ec(0, i, j - 1, k, q);
-                            double SE = Ec(0, i, j, k - 1, q), SW = Ec(0, i, j - 1, k - 1, q);
-                            double Eavg = 0.25 * (Exzf + Exzfm + Exyf + Exyfm);
-                            double Sy = std::copysign(1., Fd(1, i, j, k, q)), Sym = std::copysign(1., Fd(1, i, j, k - 1, q));
-                            double dN = (1. - Sy) * (NE - Exzf) + (1. + Sy) * (NW - Exzfm);
-                            double dS = (1. - Sym) * (Exzf - SE) + (1. + Sym) * (Exzfm - SW);
-                            double dEdz = (1. / 8.) * (dS - dN);
-                            double Sz = std::copysign(1., Fd(2, i, j, k, q)), Szm = std::copysign(1., Fd(2, i, j - 1, k, q));
-                            double dE = (1. - Sz) * (NE - Exyf) + (1. + Sz) * (SE - Exyfm);
-                            double dW = (1. - Szm) * (Exyf - NW) + (1. + Szm) * (Exyfm - SW);
-                            double dEdy = (1. / 8.) * (dW - dE);
-                            emf[g.ed(i, j, k, 0, q)] = Eavg + dEdz + dEdy;
-                        }
-                        if (j < g.ny + ng) {
-                            double Eyzf = Ef(2, 1, i, j, k, q), Eyzfm = Ef(2, 1, i - 1, j, k, q);
-                            double Eyxf = Ef(0, 0, i, j, k, q), Eyxfm = Ef(0, 0, i, j, k - 1, q);
-                            double NE = Ec(1, i, j, k, q), NW = Ec(1, i - 1, j, k, q);
-                            double SE = Ec(1, i, j, k - 1, q), SW = Ec(1, i - 1, j, k - 1, q);
-                            double Eavg = 0.25 * (Eyzf + Eyzfm + Eyxf + Eyxfm);
-                            double Sx = std::copysign(1., Fd(0, i, j, k, q)), Sxm = std::copysign(1., Fd(0, i, j, k - 1, q));
-                            double dN = (1. - Sx) * (NE - Eyzf) + (1. + Sx) * (NW - Eyzfm);
-                            double dS = (1. - Sxm) * (Eyzf - SE) + (1. + Sxm) * (Eyzfm - SW);
-                            double dEdz = (1. / 8.) * (dS - dN);
-                            double Sz = std::copysign(1., Fd(2, i, j, k, q)), Szm = std::copysign(1., Fd(2, i - 1, j, k, q));
-                            double dE = (1. - Sz) * (NE - Eyxf) + (1. + Sz) * (SE - Eyxfm);
-                            double dW = (1. - Szm) * (Eyxf - NW) + (1. + Szm) * (Eyxfm - SW);
-                            double dEdx = (1. / 8.) * (dW - dE);
-                            emf[g.ed(i, j, k, 1, q)] = Eavg + dEdz + dEdx;
-                        }
-                        if (k < g.nz + ng) {
-                            double Ezyf = Ef(1, 1, i, j, k, q), Ezyfm = Ef(1, 1, i - 1, j, k, q);
-                            double Ezxf = Ef(0, 1, i, j, k, q), Ezxfm = Ef(0, 1, i, j - 1, k, q);
-                            double NE = Ec(2, i, j, k, q), NW = Ec(2, i - 1, j, k, q);
-                            double SE = Ec(2, i, j - 1, k, q), SW = Ec(2, i - 1, j - 1, k, q);
-                            double Eavg = 0.25 * (Ezyf + Ezyfm + Ezxf + Ezxfm);
-                            double Sx = std::copysign(1., Fd(0, i, j, k, q)), Sxm = std::copysign(1., Fd(0, i, j - 1, k, q));
-                            double dN = (1. - Sx) * (NE - Ezyf) + (1. + Sx) * (NW - Ezyfm);
-                            double dS = (1. - Sxm) * (Ezyf - SE) + (1. + Sxm) * (Ezyfm - SW);
-                            double dEdy = (1. / 8.) * (dS - dN);
-                            double Sy = std::copysign(1., Fd(1, i, j, k, q)), Sym = std::copysign(1., Fd(1, i - 1, j, k, q));
-                            double dE = (1. - Sy) * (NE - Ezxf) + (1. + Sy) * (SE - Ezxfm);
-                            double dW = (1. - Sym) * (Ezxf - NW) + (1. + Sym) * (Ezxfm - SW);
-                            double dEdx = (1. / 8.) * (dW - dE);
-                            emf[g.ed(i, j, k, 2, q)] = Eavg + dEdy + dEdx;
-                        }
+                        if (i < g.nx + ng) emf[g.ed(i, j, k, 0, q)] = gs_edge(0, i, j, k, q);
+                        if (j < g.ny + ng) emf[g.ed(i, j, k, 1, q)] = gs_edge(1, i, j, k, q);
+                        if (k < g.nz + ng) emf[g.ed(i, j, k, 2, q)] = gs_edge(2, i, j, k, q);
                     }
         }
     }
@@ -543,11 +552,125 @@ struct Stepper {
         }
     }
 
-    // advance_substep (evolve.cpp:1671-1717) without FOFC / reflux (uniform grids)
+    // ---------------- FOFC (evolve.cpp:456-811) ----------------
+    // Per-block byte tags of faces and edges, cleared every substep.
+    std::vector<uint8_t> ftag[3], etag;
+    int64_t fofc_cells_flagged = 0;
+
+    // flag_fofc_cells: tentative update of every cell of [ngz-1, n+ngz+1)^3
+    // (interior widened by one ghost layer) with the high-order fluxes and
+    // EMFs, a discrete-maximum-principle test on D and tau against the
+    // 27-cell neighbourhood of `newst`, and a lenient dry-run c2p; a cell
+    // that fails tags its 6 faces and 12 edges.
+    void flag_fofc(double dt, double dtfact, const double* newst, const Fields& old) {
+        const Grid& g = G();
+        const int ng = g.ngz;
+        for (int d = 0; d < 3; ++d) ftag[d].assign((size_t)g.fc_size(d, 1), 0);
+        etag.assign((size_t)g.ed_size(), 0);
+        C2PParams lenient = c2pp;
+        lenient.alp_bh_thresh = 1e30;
+        const double dte = dt * dtfact;
+        int64_t nflag = 0;
+        for (int64_t q = 0; q < g.nq; ++q) {
+            const double ix = L.idx(q, 0), iy = L.idx(q, 1), iz = L.idx(q, 2);
+            auto E = [&](int i, int j, int k, int c) { return emf[g.ed(i, j, k, c, q)]; };
+            auto Bo = [&](int d, int i, int j, int k) { return old.B[d][g.fc(d, i, j, k, 0, q, 1)]; };
+#pragma omp parallel for collapse(2) schedule(dynamic) reduction(+ : nflag)
+            for (int k = ng - 1; k < g.nz + ng + 1; ++k)
+                for (int j = ng - 1; j < g.ny + ng + 1; ++j)
+                    for (int i = ng - 1; i < g.nx + ng + 1; ++i) {
+                        double cons[NCONS];
+                        for (int v = 0; v <= ENTROPYSTAR_; ++v) {
+                            double dFx = (flux[0][g.fc(0, i, j, k, v, q, NHRSC)] - flux[0][g.fc(0, i + 1, j, k, v, q, NHRSC)]) * ix;
+                            double dFy = (flux[1][g.fc(1, i, j, k, v, q, NHRSC)] - flux[1][g.fc(1, i, j + 1, k, v, q, NHRSC)]) * iy;
+                            double dFz = (flux[2][g.fc(2, i, j, k, v, q, NHRSC)] - flux[2][g.fc(2, i, j, k + 1, v, q, NHRSC)]) * iz;
+                            cons[v] = newst[g.cc(i, j, k, v, q, NSTATE)] + dt * dtfact * ((dFx + dFy) + dFz);
+                        }
+                        // ct_update_B{x,y,z}_face (grmhd_helpers.hh:441-475) on the stage input B
+                        auto bx = [&](int ii) { return Bo(0, ii, j, k) + dte * ((E(ii, j, k + 1, 1) - E(ii, j, k, 1)) * iz + (E(ii, j, k, 2) - E(ii, j + 1, k, 2)) * iy); };
+                        auto by = [&](int jj) { return Bo(1, i, jj, k) + dte * ((E(i + 1, jj, k, 2) - E(i, jj, k, 2)) * ix + (E(i, jj, k, 0) - E(i, jj, k + 1, 0)) * iz); };
+                        auto bz = [&](int kk) { return Bo(2, i, j, kk) + dte * ((E(i, j + 1, kk, 0) - E(i, j, kk, 0)) * iy + (E(i, j, kk, 1) - E(i + 1, j, kk, 1)) * ix); };
+                        cons[BSXL] = 0.5 * (bx(i) + bx(i + 1));
+                        cons[BSYL] = 0.5 * (by(j) + by(j + 1));
+                        cons[BSZL] = 0.5 * (bz(k) + bz(k + 1));
+                        bool dmp = false;
+                        if (opt.dmp_enable) {
+                            const double inv_M = 1.0 / opt.dmp_M;
+                            const int vars[2] = {DENS_, TAU_};
+                            for (int n = 0; n < 2 && !dmp; ++n) {
+                                double vmax = -std::numeric_limits<double>::max(), vmin = std::numeric_limits<double>::max();
+                                for (int kt = -1; kt <= 1; ++kt)
+                                    for (int jt = -1; jt <= 1; ++jt)
+                                        for (int it = -1; it <= 1; ++it) {
+                                            double v = newst[g.cc(i + it, j + jt, k + kt, vars[n], q, NSTATE)];
+                                            vmax = std::fmax(vmax, v);
+                                            vmin = std::fmin(vmin, v);
+                                        }
+                                double test = cons[vars[n]];
+                                if (test > opt.dmp_M * vmax || test < vmin * inv_M) dmp = true;
+                            }
+                        }
+                        Metric m = metric_at(old.state, i, j, k, q);
+                        double prims[NPRIM] = {0};
+                        Bits err;
+                        C2PResult r = conservs_to_prims(cons, prims, m, eos, atmo, exc, lenient, nullptr, err, true);
+                        if (!(r.floored || dmp)) continue;
+                        nflag++;
+                        for (int t = 0; t < 2; ++t) {
+                            ftag[0][g.fc(0, i + t, j, k, 0, q, 1)] = 1;
+                            ftag[1][g.fc(1, i, j + t, k, 0, q, 1)] = 1;
+                            ftag[2][g.fc(2, i, j, k + t, 0, q, 1)] = 1;
+                        }
+                        for (int a = 0; a < 2; ++a)
+                            for (int b = 0; b < 2; ++b) {
+                                etag[g.ed(i, j + a, k + b, 0, q)] = 1;
+                                etag[g.ed(i + a, j, k + b, 1, q)] = 1;
+                                etag[g.ed(i + a, j + b, k, 2, q)] = 1;
+                            }
+                    }
+        }
+        fofc_cells_flagged += nflag;
+    }
+
+    // apply_fofc_correction: donor-cell + LLF fluxes (and GS face EMFs) on the
+    // tagged faces, then the GS edge EMFs on the tagged edges (the compaction
+    // ranges of evolve.cpp:646-702).
+    void apply_fofc(const Fields& old, const double* aux) {
+        const Grid& g = G();
+        const int ng = g.ngz;
+        const int n[3] = {g.nx, g.ny, g.nz};
+        Options keep = opt;
+        opt.recon = REC_DONOR;
+        opt.riemann = RS_LLF;
+        for (int64_t q = 0; q < g.nq; ++q)
+            for (int d = 0; d < 3; ++d) {
+                int lo[3] = {ng - 1, ng - 1, ng - 1}, hi[3] = {n[0] + ng + 1, n[1] + ng + 1, n[2] + ng + 1};
+                lo[d] = ng;
+                for (int k = lo[2]; k < hi[2]; ++k)
+                    for (int j = lo[1]; j < hi[1]; ++j)
+                        for (int i = lo[0]; i < hi[0]; ++i)
+                            if (ftag[d][g.fc(d, i, j, k, 0, q, 1)]) getflux(d, old, aux, i, j, k, q);
+            }
+        opt = keep;
+        for (int64_t q = 0; q < g.nq; ++q)
+            for (int k = ng; k < g.nz + ng + 1; ++k)
+                for (int j = ng; j < g.ny + ng + 1; ++j)
+                    for (int i = ng; i < g.nx + ng + 1; ++i) {
+                        if (i < g.nx + ng && etag[g.ed(i, j, k, 0, q)]) emf[g.ed(i, j, k, 0, q)] = gs_edge(0, i, j, k, q);
+                        if (j < g.ny + ng && etag[g.ed(i, j, k, 1, q)]) emf[g.ed(i, j, k, 1, q)] = gs_edge(1, i, j, k, q);
+                        if (k < g.nz + ng && etag[g.ed(i, j, k, 2, q)]) emf[g.ed(i, j, k, 2, q)] = gs_edge(2, i, j, k, q);
+                    }
+    }
+
+    // advance_substep (evolve.cpp:1671-1717); no reflux (uniform grids)
     void advance_substep(double dt, double dtfact, double* newst, double* nB[3], const Fields& old, const double* aux) {
         compute_fluxes(old, aux);
         if (opt.emf == EMF_GS) compute_emfs_gs();
         else compute_emfs_uct(old);
+        if (opt.fofc) {
+            flag_fofc(dt, dtfact, newst, old);
+            apply_fofc(old, aux);
+        }
         add_fluxes_and_sources(dt, dtfact, newst, old, aux);
         update_ct(dt, dtfact, nB);
     }

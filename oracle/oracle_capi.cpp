@@ -48,6 +48,7 @@ ExcisionParams make_exc(const grmhd_params* p) {
 }
 Options make_opt(const grmhd_params* p) {
     Options o; o.recon = p->recon; o.riemann = p->riemann; o.emf = p->emf; o.thermo = p->thermo_recon;
+    o.fofc = p->fofc; o.dmp_enable = p->fofc_dmp_enable; o.dmp_M = p->fofc_dmp_M;
     o.fd_order = p->fd_order; o.rusanov_c_limit = p->rusanov_c_limit; return o;
 }
 
@@ -98,6 +99,7 @@ struct StepperBase {
     virtual void aux(double* st, double* B[3], double* aux) = 0;
     virtual double dt(const double* st, const double* aux, double cfl) = 0;
     virtual int64_t fatal() = 0;
+    virtual int64_t fofc_flagged() = 0;
 };
 
 template <typename EOS>
@@ -132,6 +134,7 @@ struct StepperT : StepperBase {
     void aux(double* st, double* B[3], double* aux) override { S->compute_auxiliaries(st, B, aux); }
     double dt(const double* st, const double* aux, double cfl) override { return S->find_dt(st, aux, cfl); }
     int64_t fatal() override { return S->n_fatal; }
+    int64_t fofc_flagged() override { return S->fofc_cells_flagged; }
 };
 
 }  // namespace
@@ -291,5 +294,6 @@ double orc_find_dt(void* h, const double* st, const double* aux, double cfl) {
     return static_cast<StepperBase*>(h)->dt(st, aux, cfl);
 }
 int64_t orc_fatal_count(void* h) { return static_cast<StepperBase*>(h)->fatal(); }
+int64_t orc_fofc_flagged(void* h) { return static_cast<StepperBase*>(h)->fofc_flagged(); }
 
 }  // extern "C"

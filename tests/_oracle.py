@@ -47,6 +47,8 @@ def olib(fast=False):
     L.orc_find_dt.restype = d
     L.orc_fatal_count.argtypes = [P]
     L.orc_fatal_count.restype = i64
+    L.orc_fofc_flagged.argtypes = [P]
+    L.orc_fofc_flagged.restype = i64
     _libs[name] = L
     return L
 
@@ -107,3 +109,6 @@ class Stepper:
 
     def fatal(self):
         return self.L.orc_fatal_count(self.h)
+
+    def fofc_flagged(self):
+        return self.L.orc_fofc_flagged(self.h)
