@@ -387,3 +387,54 @@ def mlp_targets(n, roots, feat, target, stream=None):
 def c2p_batch_learned(ctx, mlp, n, cons, met, prims, err, floored, r=None, roots=None, stream=None):
     check(lib().grmhd_c2p_batch_learned(ctx.h, mlp.h, n, _ptr(cons), _ptr(met), _ptr(prims), _ptr(err),
                                         _ptr(floored), _ptr(r), _ptr(roots), _ptr(stream)), "c2p_batch_learned")
+
+
+# ---- HDF5 state dumps (include/grmhd_io.h, libgrmhd_io.so) -------------------
+IO_LIB_PATH = os.path.join(HERE, "libgrmhd_io.so")
+_io = None
+
+
+def io_lib():
+    global _io
+    if _io is not None:
+        return _io
+    if not os.path.exists(IO_LIB_PATH):
+        raise RuntimeError(f"{IO_LIB_PATH} is not built: run `python -m grace_amd.build`")
+    L = C.CDLL(IO_LIB_PATH)
+    P = C.c_void_p
+    L.grmhd_io_write.argtypes = [C.c_char_p, C.c_double, C.c_uint32, C.POINTER(Grid), C.c_int32, P, P, P * 3]
+    L.grmhd_io_info.argtypes = [C.c_char_p, C.POINTER(Grid), C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                C.POINTER(C.c_uint32), P]
+    L.grmhd_io_read.argtypes = [C.c_char_p, P, P * 3]
+    L.grmhd_io_last_error.restype = C.c_char_p
+    _io = L
+    return L
+
+
+def _io_check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {io_lib().grmhd_io_last_error().decode()}")
+
+
+def write_state(path, grid, state, B, time=0.0, iteration=0, extents=(0, 1, 0, 1, 0, 1)):
+    """Host numpy arrays (layout of the C ABI) -> HDF5 dump."""
+    import numpy as np
+    ext = np.ascontiguousarray(extents, dtype=np.float64)
+    nv = state.shape[1]
+    _io_check(io_lib().grmhd_io_write(os.fsencode(path), float(time), int(iteration), C.byref(grid), nv, _ptr(ext),
+                                      _ptr(np.ascontiguousarray(state)), ptr3(*[np.ascontiguousarray(b) for b in B])),
+              "write_state")
+
+
+def read_state(path):
+    """HDF5 dump -> (grid, state, B, time, iteration, extents) as host numpy arrays."""
+    import numpy as np
+    g, nv, t, it = Grid(), C.c_int32(), C.c_double(), C.c_uint32()
+    ext = np.zeros(6)
+    _io_check(io_lib().grmhd_io_info(os.fsencode(path), C.byref(g), C.byref(nv), C.byref(t), C.byref(it), _ptr(ext)),
+              "read_state")
+    N = [g.nx + 2 * g.ngz, g.ny + 2 * g.ngz, g.nz + 2 * g.ngz]
+    state = np.zeros((g.nq, nv.value, N[2], N[1], N[0]))
+    B = [np.zeros((g.nq, 1, N[2] + (d == 2), N[1] + (d == 1), N[0] + (d == 0))) for d in range(3)]
+    _io_check(io_lib().grmhd_io_read(os.fsencode(path), _ptr(state), ptr3(*B)), "read_state")
+    return g, state, B, t.value, it.value, ext

@@ -48,6 +48,24 @@ def build_lib():
     return LIB
 
 
+IO_LIB = os.path.join(HERE, "libgrmhd_io.so")
+HDF5_PREFIX = os.environ.get("GRMHD_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io():
+    """libgrmhd_io.so: HDF5 state dumps (host code, linked against HDF5)."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"HDF5 not found under {HDF5_PREFIX}: libgrmhd_io.so not built", flush=True)
+        return None
+    tmp = IO_LIB + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + inc, "-o", tmp,
+          os.path.join(CSRC, "state_io.cpp"), "-L" + libdir, "-lhdf5", "-Wl,-rpath," + libdir])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j4"])
 
@@ -55,6 +73,7 @@ def build_oracle():
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     build_lib()
+    build_io()
     if "--oracle" in argv:
         build_oracle()
 
