@@ -72,6 +72,7 @@ ABI_SYMBOLS = [
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
     "grmhd_autotune", "grmhd_get_tuning", "grmhd_set_tuning",
     "grmhd_setup_blast", "grmhd_setup_shocktube", "grmhd_kerr_schild", "grmhd_setup_c2p_states",
+    "grmhd_kerr_schild_adm", "grmhd_setup_blast_ks",
     # include/grmhd_mlp.h
     "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
     "grmhd_mlp_get_params", "grmhd_mlp_set_scaling", "grmhd_mlp_get_scaling", "grmhd_mlp_set_bracket",
@@ -137,6 +138,9 @@ def lib():
                                     C.c_uint64, P, P, P, P, P]
     L.grmhd_setup_shocktube.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, d, d, P, P, P, P, P, P, P]
     L.grmhd_kerr_schild.argtypes = [d, d, d, d, P, P, P]
+    L.grmhd_kerr_schild_adm.argtypes = [d, d, d, d, P, P, P, P]
+    L.grmhd_setup_blast_ks.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, P, P, d, d, d, d, d, P, d,
+                                       C.c_uint64, P, P, P, P, P]
     L.grmhd_setup_c2p_states.argtypes = [i64, C.c_uint64, d, d, P, P, P, P, P, P, P, P, P]
     f32 = C.c_float
     L.grmhd_mlp_init_params.argtypes = [C.c_uint64, P]

@@ -66,6 +66,28 @@ def blast(nb=(8, 8, 8), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01, emf
                     extra=dict(N=N, dx=1.0 / N[0]))
 
 
+def blast_ks(nb=(2, 2, 2), n=16, ngz=3, spin=0.9, center=(8.0, 0.0, 0.0), size=4.0, seed=SEED, gamma=4.0 / 3.0,
+             noise=0.01, rho=1e-2, p=1e-3, p_in=1.0, r_in=0.4, Bx=0.1, emf=EMF_UCT, fd_order=6):
+    """The blast wave in the Kerr-Schild metric of a spin-a hole at the origin
+    (Cowling, curved metric kernels): a cube of side `size` M centred at
+    `center`, the blast at the cube's centre."""
+    nq = nb[0] * nb[1] * nb[2]
+    grid = Grid(n, n, n, ngz, nq)
+    N = [nb[d] * n for d in range(3)]
+    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(N[d] / size for d in range(3)))]
+    inst = make_instances(spec)
+    state, B, aux = alloc(grid)
+    lo = np.array(center, dtype=np.float64) - 0.5 * size
+    hi = lo + size
+    Bv = np.array([Bx, 0.0, 0.0])
+    rc = lib().grmhd_setup_blast_ks(grid, inst, 1, gamma, _p(lo), _p(hi), spin, rho, p, p_in, r_in, _p(Bv), noise,
+                                    seed, _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux))
+    assert rc == 0
+    params = default_params(gamma=gamma, emf=emf, flat_metric=0, fd_order=fd_order)
+    return Workload("blast_ks", params, grid, spec, state, B, aux, zones=nq * n ** 3,
+                    extra=dict(N=N, dx=size / N[0]))
+
+
 def multi_blast(n_inst=64, sizes=(32, 48, 64, 80, 96), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01):
     """Uneven instances: each a cubic blast-wave grid of side drawn from `sizes`,
     cut into n^3 blocks; all instances share one block list."""

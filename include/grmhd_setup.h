@@ -29,6 +29,16 @@ int grmhd_setup_shocktube(const grmhd_grid* g, const grmhd_instance* inst, int32
 
 /* Kerr-Schild (M=1, spin a) lapse, shift and 3-metric at a Cartesian point. */
 void grmhd_kerr_schild(double a, double x, double y, double z, double g[6], double* alp, double beta[3]);
+/* The same plus the extrinsic curvature of the stationary slice (K_ij). */
+void grmhd_kerr_schild_adm(double a, double x, double y, double z, double g[6], double* alp, double beta[3],
+                           double K[6]);
+/* grmhd_setup_blast centred in [lo, hi] with the Kerr-Schild metric (hole of
+ * spin a at the origin) in every cell; face B = Bvec (densitised, so
+ * discretely divergence free). */
+int grmhd_setup_blast_ks(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
+                         const double lo[3], const double hi[3], double spin, double rho_amb, double p_amb,
+                         double p_in, double r_in, const double Bvec[3], double noise, uint64_t seed, double* state,
+                         double* Bx, double* By, double* Bz, double* aux);
 
 /* Seeded c2p test states (SoA outputs, see grmhd_abi.h for the layouts). */
 int grmhd_setup_c2p_states(int64_t n, uint64_t seed, double gamma, double spin, const double rho_rng[2],
