@@ -9,10 +9,13 @@ import grace_amd as ga
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+IO_HEADER = "grmhd_io.h"   # exported by libgrmhd_io.so, the HDF5 state-dump library
+
+
+def declared_functions(io=False):
     names = set()
     for h in os.listdir(os.path.join(ROOT, "include")):
-        if not h.endswith(".h"):
+        if not h.endswith(".h") or (h == IO_HEADER) != io:
             continue
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
@@ -31,6 +34,13 @@ def test_library_exports_all_declared_symbols():
 
 def test_python_symbol_list_matches_headers():
     assert set(ga.ABI_SYMBOLS) == declared_functions()
+
+
+def test_io_library_exports_declared_symbols():
+    L = ga.io_lib()
+    names = declared_functions(io=True)
+    assert len(names) == 4
+    assert not [n for n in sorted(names) if not hasattr(L, n)]
 
 
 def test_struct_layouts():

@@ -222,11 +222,15 @@ def test_gpu_compute_auxiliaries(dev):
     assert linf_rel(to_host(sg)[:, :7], st[:, :7]) < 1e-13
 
 
-@pytest.mark.parametrize("recon,riemann", [(0, 0), (1, 1)])
-def test_gpu_substep(dev, recon, riemann):
+@pytest.mark.parametrize("recon,riemann,emf,thermo", [(0, 0, 0, 0), (1, 1, 0, 0), (2, 0, 1, 0), (3, 1, 1, 0),
+                                                      (0, 0, 0, 1), (0, 1, 1, 1)])
+def test_gpu_substep(dev, recon, riemann, emf, thermo):
+    """WENO-Z / MC2 / minmod / donor x HLLE / LLF x UCT / GS x TEMP / PRESS
+    reconstruction of the thermodynamic variable."""
     import torch
-    wl = W.blast(nb=(2, 2, 2), n=16, recon=recon, riemann=riemann)
-    say(f"[substep] recon={recon} riemann={riemann}")
+    wl = W.blast(nb=(2, 2, 2), n=16, recon=recon, riemann=riemann, emf=emf)
+    wl.params.thermo_recon = thermo
+    say(f"[substep] recon={recon} riemann={riemann} emf={emf} thermo={thermo}")
     dt = 0.25 * wl.extra["dx"]
     st, B, aux = wl.copy_arrays()
     new_o, nB_o = st.copy(), [b.copy() for b in B]
