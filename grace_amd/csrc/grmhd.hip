@@ -1904,12 +1904,18 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
         launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s);
         {   // c2p outputs into the other ghost cells: the reconstructed
-            // primitives between stages, everything after the last stage
+            // primitives (and the GS cell E, which the next stage's edge EMFs
+            // read one cell into the ghosts) between stages, everything after
+            // the last stage; with FOFC also the (possibly rewritten)
+            // conserved at every stage, which the next stage's flag test
+            // reads in the ghost layer.  (Pushing these from the c2p kernel
+            // instead was measured slower: scattered stores from a
+            // 3-wave/SIMD FP64 kernel.)
             GatherArgs g{};
             const bool last = sidx == 2;
             uint32_t mask = 0x3FFu | (c->P.thermo_recon ? (1u << PRESS_) : 0u);
             g.aux = aux; g.aux_mask = last ? 0x7FFFu : mask;
-            g.st = last ? S.nst : nullptr; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC;
+            g.st = (last || c->P.fofc) ? S.nst : nullptr; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC;
             g.ecen = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
             g.cell_mode = 3;
             launch_gather(c, g, s);
