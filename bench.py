@@ -125,8 +125,9 @@ def state_residual(a, b):
 
 def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
     """Kastaun c2p alone on the 4M Kerr-Schild states: plain FP64 solve vs the
-    learned bracket (MLP trained on the GPU on 1M states of another seed);
-    bracket width and moves autotuned by timing on a 1M-state slice."""
+    learned guess (MLP trained on the GPU on 1M states of another seed) with
+    its Newton finish, and vs the learned brackets alone (width and moves
+    picked by timing on a 1M-state slice)."""
     log(f"[bench] c2p leg: {n} states, training on {n_train}")
     st = W.c2p_states(n)
     tr = W.c2p_states(n_train, seed=W.SEED + 100)
@@ -167,7 +168,10 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
                 ts.append(e0.elapsed_time(e1))
         return float(np.median(ts)), ctx.c2p_stats()
 
+    # bracket-only variant: width and moves picked by timing on a 1M-state slice
     de0, ds0, _ = mlp.get_bracket()
+    newton = mlp.get_newton()
+    mlp.set_newton(0)
     best = None
     for fac in (0.5, 1.0, 2.0, 4.0):
         for nexp in (2, 3, 4):
@@ -178,10 +182,24 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
     mlp.set_bracket(best[1], ds0, best[2])
     t_pl, s_pl = run(False, full, n)
     p_pl, e_pl = prims.clone(), err.clone()
-    t_ln, s_ln = run(True, full, n)
-    same = e_pl == err
-    res = state_residual(prims, p_pl)[same]
-    solves = s_ln["n_cells"] + s_ln["n_ent_backup"]
+
+    def learned(newton_evals):
+        mlp.set_newton(newton_evals)
+        t, st_ = run(True, full, n)
+        same = e_pl == err
+        res = state_residual(prims, p_pl)[same]
+        solves = st_["n_cells"] + st_["n_ent_backup"]
+        return t, {"ms": round(t, 4), "states_per_s": n / (t * 1e-3),
+                   "root_evals_per_state": round(st_["n_root_evals"] / n, 3),
+                   "bracket_evals_per_state": round(st_["n_bracket_evals"] / n, 3),
+                   "max_root_evals": st_["max_root_evals"],
+                   "fallback_rate": round(1.0 - st_["n_guess_used"] / max(solves, 1), 5),
+                   "speedup": round(t_pl / t, 4),
+                   "error_vs_plain": {"max_state_rel": float(res.max()), "median_state_rel": float(res.median()),
+                                      "same_error_bits": float(same.double().mean())}}
+
+    _, brk = learned(0)
+    t_ln, lrn = learned(newton)
     return {
         "workload": "c2p_4M_kerr_schild_a0.9_r1.5-50_ideal_gamma4/3",
         "states": n,
@@ -189,15 +207,10 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
                   "root_evals_per_state": round(s_pl["n_root_evals"] / n, 3),
                   "bracket_evals_per_state": round(s_pl["n_bracket_evals"] / n, 3),
                   "max_root_evals": s_pl["max_root_evals"]},
-        "learned": {"ms": round(t_ln, 4), "states_per_s": n / (t_ln * 1e-3),
-                    "root_evals_per_state": round(s_ln["n_root_evals"] / n, 3),
-                    "bracket_evals_per_state": round(s_ln["n_bracket_evals"] / n, 3),
-                    "max_root_evals": s_ln["max_root_evals"],
-                    "fallback_rate": round(1.0 - s_ln["n_guess_used"] / max(solves, 1), 5)},
+        "learned": dict(lrn, finish=f"Newton from the predicted roots, <= {newton} evaluations, then plain"),
+        "learned_bracket_only": dict(brk, bracket={"delta_e": best[1], "delta_s": ds0, "nexp": best[2]}),
         "speedup": round(t_pl / t_ln, 4),
-        "error_vs_plain": {"max_state_rel": float(res.max()), "median_state_rel": float(res.median()),
-                           "same_error_bits": float(same.double().mean())},
-        "bracket": {"delta_e": best[1], "delta_s": ds0, "nexp": best[2]},
+        "error_vs_plain": lrn["error_vs_plain"],
         "train": {"states": n_train, "seconds": round(fit_s, 3), "steps": rep["steps"],
                   "val_p50_log_err": rep["val_p50_e"], "val_p99_log_err": rep["val_p99_e"],
                   "loss_first": rep["loss_first"], "loss_last": rep["loss_last"]},

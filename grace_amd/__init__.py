@@ -76,7 +76,7 @@ ABI_SYMBOLS = [
     # include/grmhd_mlp.h
     "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
     "grmhd_mlp_get_params", "grmhd_mlp_set_scaling", "grmhd_mlp_get_scaling", "grmhd_mlp_set_bracket",
-    "grmhd_mlp_get_bracket", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
+    "grmhd_mlp_get_bracket", "grmhd_mlp_set_newton", "grmhd_mlp_get_newton", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
     "grmhd_mlp_train_step", "grmhd_mlp_default_fit_opts", "grmhd_mlp_fit", "grmhd_c2p_batch_learned",
 ]
 
@@ -153,6 +153,8 @@ def lib():
     L.grmhd_mlp_get_scaling.argtypes = [P, P]
     L.grmhd_mlp_set_bracket.argtypes = [P, d, d, i32]
     L.grmhd_mlp_get_bracket.argtypes = [P, C.POINTER(d), C.POINTER(d), C.POINTER(i32)]
+    L.grmhd_mlp_set_newton.argtypes = [P, i32]
+    L.grmhd_mlp_get_newton.argtypes = [P, C.POINTER(i32)]
     L.grmhd_mlp_features.argtypes = [P, i64, P, P, P, P]
     L.grmhd_mlp_targets.argtypes = [i64, P, P, P, P]
     L.grmhd_mlp_forward.argtypes = [P, i64, P, P, P]
@@ -348,6 +350,14 @@ class Mlp:
 
     def set_bracket(self, delta_e, delta_s, nexp):
         check(self.L.grmhd_mlp_set_bracket(self.h, float(delta_e), float(delta_s), int(nexp)), "mlp_set_bracket")
+
+    def set_newton(self, max_evals):
+        check(self.L.grmhd_mlp_set_newton(self.h, int(max_evals)), "mlp_set_newton")
+
+    def get_newton(self):
+        v = C.c_int32()
+        check(self.L.grmhd_mlp_get_newton(self.h, C.byref(v)), "mlp_get_newton")
+        return v.value
 
     def get_bracket(self):
         a, b, c = C.c_double(), C.c_double(), C.c_int32()

@@ -1047,8 +1047,10 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
 // c2p over a flat batch
 // ---------------------------------------------------------------------------
 // LEARNED: the brackets come from the MLP (features -> BF16 MFMA forward in
-// LDS/registers, one pass per wave) instead of the guess array.
-template <typename EOS, bool LEARNED>
+// LDS/registers, one pass per wave) instead of the guess array.  NEWTON
+// (ideal gas): the predicted roots start Newton iterations, and where those
+// do not converge the plain path runs (no bracket code in this variant).
+template <typename EOS, bool LEARNED, bool NEWTON = false>
 __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_batch(int64_t n, EOS eos, Opts o, double* cons_soa,
                                                    const double* __restrict__ met, double* prims,
                                                    uint64_t* errb, uint8_t* floored, int dry_run,
@@ -1078,13 +1080,19 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_batch(int64_t n, EO
         // energy root: FP32 exp is ample for a bracket of half-width >= 1e-6;
         // the entropy/energy ratio is close to 1, so its bracket is formed in FP64
         const double xe = (double)__expf((float)lxhi + y0);
-        gs.e_lo = xe * mlp.fe_lo;
-        gs.e_hi = fmin(xe * mlp.fe_hi, 1.0);
         const double z = (double)y1;
         const double rs = (fabs(z) < 1e-3) ? 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0))) : exp(z);
-        gs.s_lo = rs * mlp.fs_lo;
-        gs.s_hi = rs * mlp.fs_hi;
-        gs.nexp = mlp.nexp;
+        if constexpr (NEWTON) {
+            gs.newton = mlp.newton;
+            gs.e_c = xe;
+            gs.s_c = rs;
+        } else {
+            gs.e_lo = xe * mlp.fe_lo;
+            gs.e_hi = fmin(xe * mlp.fe_hi, 1.0);
+            gs.s_lo = rs * mlp.fs_lo;
+            gs.s_hi = rs * mlp.fs_hi;
+            gs.nexp = mlp.nexp;
+        }
         gs.s_rel = 1;
     } else if (guess) {
         gs.e_lo = guess[s]; gs.e_hi = guess[n + s]; gs.s_lo = guess[2 * n + s]; gs.s_hi = guess[3 * n + s];
@@ -1277,6 +1285,8 @@ static void build_eos(grmhd_ctx* c) {
     c->ideal.eps_max = p.eps_max;
     c->ideal.temp_atm = p.temp_fl;
     c->ideal.ent_min = (p.temp_fl > 0.0) ? (p.temp_fl / c->ideal.gm1) : 1e-100;
+    c->ideal.ls_min = std::log(c->ideal.ent_min) / c->ideal.gm1;
+    c->ideal.ls_max = std::log(c->ideal.eps_max) / c->ideal.gm1;
     // hybrid over a piecewise polytrope (eos_storage.cpp:52-99)
     EosHybrid& h = c->hybrid;
     h.n = p.pwp_n;
@@ -1764,7 +1774,11 @@ extern "C" grmhd_status grmhd_c2p_batch_learned(grmhd_ctx* c, grmhd_mlp* mlp, in
     if (st != GRMHD_OK) return st;
     Scope sc(c, "c2p", s);
     if (n > 0) {
-        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
+        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS && mlp->newton > 0)
+            k_c2p_batch<EosIdeal, true, true><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa,
+                                                                     prims_soa, err_bits, floored, 0, r, nullptr,
+                                                                     roots, c->stats_part, mlp->dev());
+        else if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
             k_c2p_batch<EosIdeal, true><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa,
                                                                err_bits, floored, 0, r, nullptr, roots,
                                                                c->stats_part, mlp->dev());

@@ -58,6 +58,7 @@ struct MlpDev {
     double fe_lo, fe_hi;      // exp(-delta_e), exp(delta_e): energy bracket factors
     double fs_lo, fs_hi;      // the same for the entropy/energy root ratio
     int nexp;                 // bracket moves allowed before the plain path
+    int newton;               // Newton evaluations from the predicted roots (0: brackets only)
 };
 constexpr int MLP_NORM = 24;
 
@@ -69,13 +70,16 @@ constexpr int MLP_NORM = 24;
 template <typename EOS>
 GHD void mlp_features(const EOS& eos, const Met& m, const double cons_dens[NCONS], float f[MLP_NF],
                       double& lxhi) {
+    // products with reciprocals where the c2p divides: the features are
+    // rounded to FP32 (and then bf16), far above the last-bit differences
+    const double isg = 1.0 / m.sqrtg;
     double c[NCONS];
-    for (int v = 0; v < NCONS; ++v) c[v] = cons_dens[v] / m.sqrtg;
+    for (int v = 0; v < NCONS; ++v) c[v] = cons_dens[v] * isg;
     double D = fmax(c[DENSL], 1e-300);
-    double q = c[TAUL] / D;
-    double r0 = c[STXL] / D, r1 = c[STYL] / D, r2c = c[STZL] / D;
-    double sD = sqrt(D);
-    double b0 = c[BSXL] / sD, b1 = c[BSYL] / sD, b2 = c[BSZL] / sD;
+    const double iD = 1.0 / D, isD = sqrt(iD);
+    double q = c[TAUL] * iD;
+    double r0 = c[STXL] * iD, r1 = c[STYL] * iD, r2c = c[STZL] * iD;
+    double b0 = c[BSXL] * isD, b1 = c[BSYL] * isD, b2 = c[BSZL] * isD;
     double r2 = m.sqcov(r0, r1, r2c);
     double Bt2 = m.sqv(b0, b1, b2);
     double rB = r0 * b0 + r1 * b1 + r2c * b2;
@@ -96,20 +100,33 @@ GHD void mlp_features(const EOS& eos, const Met& m, const double cons_dens[NCONS
     f[4] = flog(xhi);
     f[5] = flog(xlo);
     f[6] = log1pf((float)qp);
-    f[7] = (float)(r2 / ((1.0 + qp) * (1.0 + qp)));
-    f[8] = (float)(Bt2 / (1.0 + qp));
+    const double i1q = 1.0 / (1.0 + qp);
+    f[7] = (float)(r2 * i1q * i1q);
+    f[8] = (float)(Bt2 * i1q);
     f[9] = flog(D);
     lxhi = (double)f[4];
 }
 
 __device__ __forceinline__ float mlp_tanh(float z) {
-    // 1 - 2/(1+e^{2z}); saturates to +-1 for large |z|
-    return 1.0f - __fdividef(2.0f, 1.0f + __expf(2.0f * z));
+    // 1 - 2/(1+e^{2z}) with the hardware exp2 and reciprocal (about 1 ulp
+    // each, far below the bf16 rounding that follows); saturates to +-1
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * z));
+}
+
+// Round-to-nearest-even float -> bf16 pairs with v_cvt_pk_bf16_f32 (the same
+// rounding as f2bf for finite inputs).
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+typedef float f32v2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+    const bf16v2 h = __builtin_convertvector((f32v2){a, b}, bf16v2);
+    return __builtin_bit_cast(uint32_t, h);
 }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    const uint32_t lo = cvt_pk_bf16(a, b), hi = cvt_pk_bf16(c, d);
     bf16x4 r;
-    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
+    r[0] = (short)(lo & 0xFFFF); r[1] = (short)(lo >> 16);
+    r[2] = (short)(hi & 0xFFFF); r[3] = (short)(hi >> 16);
     return r;
 }
 __device__ __forceinline__ bf16x4 as_bf16x4(uint2 u) {
@@ -186,12 +203,12 @@ __device__ __forceinline__ f32x4 mlp_tile(const bf16x4 A[MLP_NFRAG], const MlpBi
 // Normalised bf16 features of this lane's state into the wave's LDS rows.
 __device__ __forceinline__ void mlp_stage_features(const MlpDev& w, const float feat[MLP_NF], uint16_t* xs) {
     const int l = threadIdx.x & 63;
+    float v[MLP_KP];
     #pragma unroll
-    for (int k = 0; k < MLP_KP; ++k) {
-        float v = 0.0f;
-        if (k < MLP_NF) v = (feat[k] - w.norm[k]) * w.norm[MLP_NF + k];
-        xs[l * MLP_KP + k] = f2bf(v);
-    }
+    for (int k = 0; k < MLP_KP; ++k) v[k] = (k < MLP_NF) ? (feat[k] - w.norm[k]) * w.norm[MLP_NF + k] : 0.0f;
+    uint32_t* row = reinterpret_cast<uint32_t*>(xs + l * MLP_KP);
+    #pragma unroll
+    for (int k = 0; k < MLP_KP; k += 2) row[k / 2] = cvt_pk_bf16(v[k], v[k + 1]);
 }
 
 // Whole-block forward pass: each wave runs the 64 states of its lanes.
@@ -238,9 +255,10 @@ struct grmhd_mlp {
     float norm_h[gm::MLP_NORM];
     double delta_e = 0.02, delta_s = 1e-6;
     int nexp = 2;
+    int newton = 8;
     int64_t step = 0;
     gm::MlpDev dev() const {
         return gm::MlpDev{frag, bias, norm, std::exp(-delta_e), std::exp(delta_e), std::exp(-delta_s),
-                          std::exp(delta_s), nexp};
+                          std::exp(delta_s), nexp, newton};
     }
 };

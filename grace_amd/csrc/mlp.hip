@@ -375,6 +375,18 @@ extern "C" grmhd_status grmhd_mlp_set_bracket(grmhd_mlp* m, double delta_e, doub
     return GRMHD_OK;
 }
 
+extern "C" grmhd_status grmhd_mlp_set_newton(grmhd_mlp* m, int32_t max_evals) {
+    if (!m || max_evals < 0 || max_evals > 64) { mset_err("invalid Newton evaluation limit"); return GRMHD_ERR_INVALID; }
+    m->newton = max_evals;
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_mlp_get_newton(grmhd_mlp* m, int32_t* max_evals) {
+    if (!m || !max_evals) { mset_err("null argument"); return GRMHD_ERR_INVALID; }
+    *max_evals = m->newton;
+    return GRMHD_OK;
+}
+
 extern "C" grmhd_status grmhd_mlp_get_bracket(grmhd_mlp* m, double* delta_e, double* delta_s, int32_t* nexp) {
     if (!m) { mset_err("null argument"); return GRMHD_ERR_INVALID; }
     if (delta_e) *delta_e = m->delta_e;

@@ -78,6 +78,7 @@ struct Met {
 struct EosIdeal {
     static constexpr bool has_ye = false, is_ideal = true;
     double gm1, gamma, ggm1, eps_max, ent_min, temp_atm;
+    double ls_min, ls_max;   // log(ent_min)/gm1, log(eps_max)/gm1 (entropy bounds are these minus log rho)
     GHD double hmin() const { return 1. + 1e-15; }
     GHD double rhomax() const { return 1e100; }
     GHD double rhomin() const { return 1e-100; }
@@ -104,10 +105,11 @@ struct EosIdeal {
     }
     GHD double press_full_ent(double& h, double& cs2, double& temp, double& eps, double& ent, double rho,
                               unsigned& e) const {
-        double smin = ent_of(ent_min, rho), smax = ent_of(eps_max, rho);
+        const double lr = log(rho);
+        double smin = ls_min - lr, smax = ls_max - lr;
         if (ent < smin) { ent = smin; e |= BIT(EOS_ENTROPY_TOO_LOW); }
         if (ent > smax) { ent = smax; e |= BIT(EOS_ENTROPY_TOO_HIGH); }
-        eps = exp(gm1 * (ent + log(rho)));
+        eps = exp(gm1 * (ent + lr));
         double p = gm1 * eps * rho;
         h = 1.0 + gamma * eps; temp = eps * gm1; cs2 = ggm1 * eps / h;
         return p;
@@ -443,6 +445,38 @@ struct FMu {
         double hbw = fmax((1 + a) * (1 + q - mu * r), h / w);
         return mu - 1. / (hbw + r * mu);
     }
+    // Value and derivative in mu of the function above, ideal gas only (a =
+    // (gamma-1) eps/(1+eps) depends on eps alone); the eps, velocity and
+    // density clamps contribute zero slope where they are active.
+    GHD void deriv(double mu, double& f, double& df) {
+        (*cnt)++;
+        const double x = xm(mu), dx = -bsqr * x * x;
+        const double r = rf(mu, x), dr = 2.0 * rsqr * x * dx + rbsqr * (x * (x + 1.0) + mu * dx * (2.0 * x + 1.0));
+        const double mx = mu * x, q = qf(mu, x), dq = -mx * (x + mu * dx) * brosqr;
+        const double vs = r * mu * mu, dvs = dr * mu * mu + 2.0 * r * mu;
+        double w, dw;
+        if (vs > vsqrmax) { w = wmax; dw = 0.0; } else { w = 1 / sqrt(1 - vs); dw = 0.5 * w * w * w * dvs; }
+        const double rho = fmin(eos.rhomax(), fmax(eos.rhomin(), d / w));
+        const double i1w = 1.0 / (1.0 + w);
+        const double u = mu * w * i1w, du = (w * (1.0 + w) + mu * dw) * i1w * i1w;
+        const double g = mu * r * (1.0 - u), dg = (r + mu * dr) * (1.0 - u) - mu * r * du;
+        double eps = w * (q - g), deps = dw * (q - g) + w * (dq - dg);
+        double emin, emax; eos.eps_range(emin, emax, rho);
+        if (eps > emax || eps < emin) deps = 0.0;
+        eps = fmin(emax, fmax(emin, eps));
+        unsigned ee = 0;
+        const double p = eos.press_eps(eps, rho, ee);
+        const double i1e = 1.0 / (1 + eps);
+        const double a = p / (rho * (1 + eps)), da = eos.gm1 * deps * i1e * i1e;
+        const double h = (1 + eps) * (1 + a), dh = deps * (1 + a) + (1 + eps) * da;
+        const double A = (1 + a) * (1 + q - mu * r), B = h / w;
+        double hbw, dhbw;
+        if (A >= B) { hbw = A; dhbw = da * (1 + q - mu * r) + (1 + a) * (dq - r - mu * dr); }
+        else { hbw = B; dhbw = (dh - h * dw / w) / w; }
+        const double den = hbw + r * mu, iden = 1.0 / den;
+        f = mu - iden;
+        df = 1.0 + (dhbw + r + mu * dr) * iden * iden;
+    }
     GHD double prims(double mu, unsigned& sig, double& x, double& w, double& rho, double& press, double& eps,
                      double& temp, double& ent) {
         x = xm(mu);
@@ -488,6 +522,28 @@ struct FMuEnt {
         double a = p / (rho * (1 + eps));
         double h = (1 + eps) * (1 + a);
         return mu - 1. / (h / w + r * mu);
+    }
+    // Value and derivative in mu, ideal gas only: eps = exp((gamma-1)(s + log rho))
+    // gives d eps/d rho = (gamma-1) eps/rho while s is inside its clamps.
+    GHD void deriv(double mu, double& f, double& df) {
+        (*cnt)++;
+        const double x = xm(mu), dx = -bsqr * x * x;
+        const double r = rf(mu, x), dr = 2.0 * rsqr * x * dx + rbsqr * (x * (x + 1.0) + mu * dx * (2.0 * x + 1.0));
+        const double vs = r * mu * mu, dvs = dr * mu * mu + 2.0 * r * mu;
+        double w = 1 / sqrt(1 - vs), dw = 0.5 * w * w * w * dvs;
+        if (vs > vsqrmax) { w = wmax; dw = 0.0; }
+        const double rr = d / w;
+        const double rho = fmin(eos.rhomax(), fmax(eos.rhomin(), rr));
+        const double drho = (rr == rho) ? -rho * dw / w : 0.0;
+        double hh, cs2, temp, eps, sl = s; unsigned ee = 0;
+        const double p = eos.press_full_ent(hh, cs2, temp, eps, sl, rho, ee);
+        const double deps = (ee & (BIT(EOS_ENTROPY_TOO_LOW) | BIT(EOS_ENTROPY_TOO_HIGH))) ? 0.0 : eos.gm1 * eps * drho / rho;
+        const double i1e = 1.0 / (1 + eps);
+        const double a = p / (rho * (1 + eps)), da = eos.gm1 * deps * i1e * i1e;
+        const double h = (1 + eps) * (1 + a), dh = deps * (1 + a) + (1 + eps) * da;
+        const double den = h / w + r * mu, iden = 1.0 / den;
+        f = mu - iden;
+        df = 1.0 + ((dh - h * dw / w) / w + r + mu * dr) * iden * iden;
     }
     GHD double prims(double mu, unsigned& sig, double& x, double& w, double& rho, double& press, double& eps,
                      double& temp, double& ent) {
@@ -546,7 +602,11 @@ struct C2PCount { int root = 0, bracket = 0; int guess_used = 0; };
 // runs unchanged.
 // nexp > 0 lets a bracket that misses the root be moved towards it (see
 // guess_bracket); s_rel makes [s_lo, s_hi] factors on the energy root.
-struct C2PGuess { double e_lo, e_hi, s_lo, s_hi; int nexp, s_rel; };
+// newton > 0 (ideal gas only) first runs a safeguarded Newton iteration on
+// the master function from e_c (entropy: s_c, a factor on the energy root
+// when s_rel), at most `newton` evaluations; where it does not converge, or
+// its root fails the residual check, the bracket path above follows.
+struct C2PGuess { double e_lo, e_hi, s_lo, s_hi; int nexp, s_rel; int newton; double e_c, s_c; };
 // Diagnostics of the root solves, all scaled by h0: the roots (x_e, x_s) and
 // the default bracket ends mu+ h0 (m_e, m_s); -1 when not computed.
 struct C2PRoots { double x_e, x_s, m_e, m_s; };
@@ -576,14 +636,60 @@ GHD bool guess_bracket(F& f, double lo, double hi, double umax, int nexp, double
     }
 }
 
+// Newton iteration for the one root of a master function on (0, umax], which
+// is negative left of the root and positive right of it: every evaluation
+// narrows the bracket [lo, hi] (initially (0, umax]), and a step that leaves
+// it (or a non-finite step) is replaced by bisection.  It stops after a
+// Newton step dx after which the new iterate's error |f''/(2 f')| dx^2
+// (Taylor) is within NEWTON_ACCEPT of it: either |dx| <= NEWTON_TINY x, or
+// |dx| <= NEWTON_RTOL x with f'' estimated from the slopes of the last two
+// iterates (both reached by Newton steps).  Otherwise it iterates on, up to
+// maxit evaluations.
+constexpr double NEWTON_RTOL = 1e-7;
+constexpr double NEWTON_TINY = 1e-10;
+constexpr double NEWTON_ACCEPT = 1e-13;
+template <typename F>
+GHD bool newton_guess(F& f, double x, double umax, int maxit, double& root) {
+    double lo = 0.0, hi = umax;
+    if (!(x > lo && x < hi)) x = 0.5 * (lo + hi);
+    double xp = 0.0, dfp = 0.0;
+    bool prev_newton = false;
+    for (int it = 0; it < maxit; ++it) {
+        double fx, df;
+        f.deriv(x, fx, df);
+        if (fx == 0.0) { root = x; return true; }
+        if (fx < 0.0) lo = x; else hi = x;
+        const double dx = fx / df;
+        double xn = x - dx;
+        const bool newton = xn > lo && xn < hi;
+        if (!newton) xn = 0.5 * (lo + hi);
+        if (newton) {
+            if (fabs(dx) <= NEWTON_TINY * xn) { root = xn; return true; }
+            if (prev_newton && fabs(dx) <= NEWTON_RTOL * xn &&
+                fabs(0.5 * (df - dfp) / ((x - xp) * df)) * dx * dx <= NEWTON_ACCEPT * xn) {
+                root = xn;
+                return true;
+            }
+        }
+        prev_newton = newton;
+        xp = x; dfp = df; x = xn;
+    }
+    return false;
+}
+
 template <typename EOS>
-GHD double kastaun(const EOS& eos, const Met& m, double cons[NCONS], double pr[NPRIM], unsigned& sig,
-                   C2PCount& cnt, const C2PGuess& gs, C2PRoots& rt) {
-    C2PIn c; c2p_setup(eos, m, cons, c);
+GHD double kastaun(const EOS& eos, const C2PIn& c, double pr[NPRIM], unsigned& sig, C2PCount& cnt,
+                   const C2PGuess& gs, C2PRoots& rt) {
     double h0 = c.h0;
     FMu<EOS> f(eos, c.D, c.q, c.r2, c.rBt2, c.Bt2, h0, &cnt.root);
     double mu = -1.0;
-    if (guess_ok(gs.e_lo, gs.e_hi)) {
+    if constexpr (EOS::is_ideal) {
+        double r;
+        if (gs.newton > 0 && gs.e_c > 0.0 && newton_guess(f, gs.e_c / h0, 1.0 / h0, gs.newton, r)) {
+            mu = r; cnt.guess_used |= 1;
+        }
+    }
+    if (mu < 0.0 && guess_ok(gs.e_lo, gs.e_hi)) {
         double a, b, fa, fb;
         if (guess_bracket(f, gs.e_lo / h0, gs.e_hi / h0, 1.0 / h0, gs.nexp, a, b, fa, fb)) {
             mu = brent_fab(f, a, b, fa, fb, 1e-15);
@@ -613,13 +719,18 @@ GHD double kastaun(const EOS& eos, const Met& m, double cons[NCONS], double pr[N
 }
 
 template <typename EOS>
-GHD double kastaun_ent(const EOS& eos, const Met& m, double cons[NCONS], double pr[NPRIM], unsigned& sig,
-                       C2PCount& cnt, const C2PGuess& gs, C2PRoots& rt) {
-    C2PIn c; c2p_setup(eos, m, cons, c);
+GHD double kastaun_ent(const EOS& eos, const C2PIn& c, double pr[NPRIM], unsigned& sig, C2PCount& cnt,
+                       const C2PGuess& gs, C2PRoots& rt) {
     double h0 = c.h0;
     FMuEnt<EOS> f(eos, c.D, c.r2, c.rBt2, c.Bt2, c.s, h0, &cnt.root);
     double mu = -1.0;
-    if (guess_ok(gs.s_lo, gs.s_hi)) {
+    if constexpr (EOS::is_ideal) {
+        double r;
+        if (gs.newton > 0 && gs.s_c > 0.0 && newton_guess(f, gs.s_c / h0, 1.0 / h0, gs.newton, r)) {
+            mu = r; cnt.guess_used |= 2;
+        }
+    }
+    if (mu < 0.0 && guess_ok(gs.s_lo, gs.s_hi)) {
         double a, b, fa, fb;
         if (guess_bracket(f, gs.s_lo / h0, gs.s_hi / h0, 1.0 / h0, gs.nexp, a, b, fa, fb)) {
             mu = brent_fab(f, a, b, fa, fb, 1e-15);
@@ -694,16 +805,20 @@ GHD bool handle_eos_sig(unsigned e, bool lenient, uint64_t& err) {
     return ok;
 }
 
-GHD void p2c(const double pr[NPRIM], double cons[NCONS], const Met& m) {
+// prims_to_conservs with W and b^mu of pr already at hand
+GHD void p2c_wb(const double pr[NPRIM], double cons[NCONS], const Met& m, double W, const double b[4], double b2) {
     const double* z = &pr[ZXL];
-    double W = lorentz(m.g, z);
-    double b[4], b2; smallb(m, &pr[BXL], z, W, b, b2);
     double D, tau, S[3], Ss;
     conserved(m, pr[RHOL], pr[PRESSL], pr[EPSL], z, pr[ENTL], W, b2, b, D, tau, S, Ss);
     cons[DENSL] = m.sqrtg * D;
     cons[STXL] = m.sqrtg * S[0]; cons[STYL] = m.sqrtg * S[1]; cons[STZL] = m.sqrtg * S[2];
     cons[TAUL] = m.sqrtg * tau; cons[ENTSL] = m.sqrtg * Ss;
     cons[YESL] = cons[DENSL] * pr[YEL];
+}
+GHD void p2c(const double pr[NPRIM], double cons[NCONS], const Met& m) {
+    double W = lorentz(m.g, &pr[ZXL]);
+    double b[4], b2; smallb(m, &pr[BXL], &pr[ZXL], W, b, b2);
+    p2c_wb(pr, cons, m, W, b, b2);
 }
 
 template <typename EOS>
@@ -754,7 +869,10 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
     }
     if (floored && dry_run) { out.floored = true; return out; }
     if (cons[DENSL] >= at.rho_fl * (1 + at.atmo_tol)) {
-        double res = kastaun(eos, m, cons, pr, ret, cnt, gs, rt);
+        // both inversions start from the same invariants of the (limited)
+        // conserved state, so they are formed once
+        C2PIn ci; c2p_setup(eos, m, cons, ci);
+        double res = kastaun(eos, ci, pr, ret, cnt, gs, rt);
         failed = (fabs(res) > cp.tol) || (ret & BIT(SIG_RHO_TOO_HIGH)) || (ret & BIT(SIG_EPS_TOO_HIGH)) || !isfinite(res);
         double beta;
         {
@@ -768,8 +886,11 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
             for (int i = 0; i < NPRIM; ++i) save[i] = pr[i];
             unsigned bret = 0;
             C2PGuess gse = gs;
-            if (gs.s_rel) { gse.s_lo = gs.s_lo * rt.x_e; gse.s_hi = fmin(gs.s_hi * rt.x_e, 1.0); }
-            double rb = kastaun_ent(eos, m, cons, pr, bret, cnt, gse, rt);
+            if (gs.s_rel) {
+                gse.s_lo = gs.s_lo * rt.x_e; gse.s_hi = fmin(gs.s_hi * rt.x_e, 1.0);
+                gse.s_c = gs.s_c * rt.x_e;
+            }
+            double rb = kastaun_ent(eos, ci, pr, bret, cnt, gse, rt);
             bool ok = (fabs(rb) <= cp.tol) && isfinite(rb) && (pr[EPSL] < eos.eps_max);
             if (ok) { ret = bret; failed = false; err |= BIT(E_ENT_BACKUP_USED); }
             else if (!failed) { for (int i = 0; i < NPRIM; ++i) pr[i] = save[i]; }
@@ -800,7 +921,7 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
             pr[ZXL] *= zn; pr[ZYL] *= zn; pr[ZZL] *= zn;
             sg |= BIT(SIG_VEL_TOO_HIGH);
         }
-        double W = lorentz(m.g, &pr[ZXL]);
+        const double W = lorentz(m.g, &pr[ZXL]);
         double b[4], b2; smallb(m, &pr[BXL], &pr[ZXL], W, b, b2);
         double sigma = b2 / pr[RHOL];
         if (sigma >= cp.max_sigma) {
@@ -811,6 +932,10 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
         }
         if (!handle_sig<EOS>(sg, lenient, err)) out.fatal = true;
         if (sg & (BIT(SIG_VEL_TOO_HIGH) | BIT(SIG_SIGMA_TOO_HIGH))) floored = true;
+        // the sigma limit changes rho and p only: W and b^mu still hold
+        p2c_wb(pr, cons, m, W, b, b2);
+        out.floored = floored;
+        return out;
     }
     p2c(pr, cons, m);
     out.floored = floored;

@@ -60,6 +60,15 @@ grmhd_status grmhd_mlp_get_scaling(grmhd_mlp* mlp, float* scale_host);
 grmhd_status grmhd_mlp_set_bracket(grmhd_mlp* mlp, double delta_e, double delta_s, int32_t nexp);
 grmhd_status grmhd_mlp_get_bracket(grmhd_mlp* mlp, double* delta_e, double* delta_s, int32_t* nexp);
 
+/* Newton finish (ideal gas): the FP64 solve first runs a safeguarded Newton
+ * iteration on the Kastaun master function (analytic slope) from the
+ * predicted root, at most max_evals evaluations, and keeps its root when the
+ * residual divided by the slope is within 1e-12 of it; otherwise the
+ * bracket above, then the plain path, run.  0 disables it; default 8.
+ * Other EOS use the brackets only. */
+grmhd_status grmhd_mlp_set_newton(grmhd_mlp* mlp, int32_t max_evals);
+grmhd_status grmhd_mlp_get_newton(grmhd_mlp* mlp, int32_t* max_evals);
+
 /* Device features of a batch of densitised conserved states (SoA as in
  * grmhd_c2p_batch); the EOS is the context's. */
 grmhd_status grmhd_mlp_features(grmhd_ctx* ctx, int64_t n, const double* cons_soa, const double* metric_soa,

@@ -148,8 +148,9 @@ def test_gpu_mlp_adam_step(dev):
 
 def test_gpu_learned_c2p_trained(dev):
     """Train on states the plain c2p solves (seed A), then run the fused
-    learned c2p on other states (seed B): same error bits, primitives within
-    1e-10 of the FP64 oracle, fewer master-function evaluations than plain."""
+    learned c2p on other states (seed B), with the Newton finish (default)
+    and with brackets only: same error bits, primitives within 1e-10 of the
+    FP64 oracle, fewer master-function evaluations than plain."""
     import torch
     ntr, n = 1 << 17, 20000
     say(f"[learned c2p] fit on {ntr} states")
@@ -163,7 +164,9 @@ def test_gpu_learned_c2p_trained(dev):
     assert rep["val_p50_e"] < 0.05
     te = W.c2p_states(n, seed=W.SEED + 8)
     res = {}
-    for mode in ("plain", "learned"):
+    assert mlp.get_newton() == 8
+    for mode in ("plain", "learned", "bracket"):
+        mlp.set_newton(0 if mode == "bracket" else 8)
         cons = to_dev(te["cons"], dev)
         met = to_dev(te["met"], dev)
         prims = torch.zeros((13, n), dtype=torch.float64, device=dev)
@@ -178,16 +181,19 @@ def test_gpu_learned_c2p_trained(dev):
         say(f"  {mode}: {res[mode][2]}")
     op, oc, oe, of, ev, fatal = O.c2p_batch(te["params"], te["cons"], te["met"], r=te["rad"])
     from test_oracle_c2p import prim_residual
-    lp, le, ls = res["learned"]
     pp, pe, ps = res["plain"]
-    same = le == oe
-    assert same.mean() > 0.999
-    r = prim_residual(lp, op)[same]
-    say(f"  learned vs oracle: max state-normalised diff {r.max():.3e}")
-    assert r.max() < 1e-10
-    assert ls["n_fatal"] == fatal == 0
-    assert ls["n_guess_used"] > 0.5 * n
     plain_evals = ps["n_root_evals"] + ps["n_bracket_evals"]
-    learned_evals = ls["n_root_evals"] + ls["n_bracket_evals"]
-    say(f"  evals/state plain {plain_evals / n:.2f} learned {learned_evals / n:.2f}")
-    assert learned_evals < plain_evals
+    for mode in ("learned", "bracket"):
+        lp, le, ls = res[mode]
+        same = le == oe
+        assert same.mean() > 0.999
+        r = prim_residual(lp, op)[same]
+        say(f"  {mode} vs oracle: max state-normalised diff {r.max():.3e}")
+        assert r.max() < 1e-10
+        assert ls["n_fatal"] == fatal == 0
+        assert ls["n_guess_used"] > 0.5 * n
+        learned_evals = ls["n_root_evals"] + ls["n_bracket_evals"]
+        say(f"  evals/state plain {plain_evals / n:.2f} {mode} {learned_evals / n:.2f}")
+        assert learned_evals < plain_evals
+    # the Newton finish needs fewer evaluations than the brackets
+    assert res["learned"][2]["n_root_evals"] < 0.7 * res["bracket"][2]["n_root_evals"]
