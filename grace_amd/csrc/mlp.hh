@@ -1,7 +1,7 @@
 // mlp.hh -- the bracket predictor for the Kastaun root solves: features,
 // BF16 MFMA forward pass (FP32 accumulation) and the shared constants.
 //
-// Network: 10 features -> 64 -> 64 -> 2, tanh hidden activations, linear
+// Network: 10 features -> 64 -> 64 -> 2, ReLU hidden activations, linear
 // output.  Outputs (normalised): y0 = log(x_e / x_hi), the energy root
 // x_e = mu h0 relative to the closed-form upper end x_hi of the bracket of
 // the mu+ Newton solve (kastaun_c2p.hh:44-118), and y1 = log(x_s / x_e), the
@@ -107,11 +107,11 @@ GHD void mlp_features(const EOS& eos, const Met& m, const double cons_dens[NCONS
     lxhi = (double)f[4];
 }
 
-__device__ __forceinline__ float mlp_tanh(float z) {
-    // 1 - 2/(1+e^{2z}) with the hardware exp2 and reciprocal (about 1 ulp
-    // each, far below the bf16 rounding that follows); saturates to +-1
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * z));
-}
+// Hidden activation: ReLU (one v_max_f32; a tanh costs two quarter-rate
+// transcendentals per unit, 128 units per state) and its slope in terms of
+// the activation value.
+__device__ __forceinline__ float mlp_act(float z) { return fmaxf(z, 0.0f); }
+__device__ __forceinline__ float mlp_act_slope(float h) { return h > 0.0f ? 1.0f : 0.0f; }
 
 // Round-to-nearest-even float -> bf16 pairs with v_cvt_pk_bf16_f32 (the same
 // rounding as f2bf for finite inputs).
@@ -174,7 +174,7 @@ __device__ __forceinline__ f32x4 mlp_tile(const bf16x4 A[MLP_NFRAG], const MlpBi
         for (int e = 0; e < 4; ++e) c[e] = b.b1[jb * 4 + e];
         c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[jb], xin, c, 0, 0, 0);
         #pragma unroll
-        for (int e = 0; e < 4; ++e) c[e] = mlp_tanh(c[e]);
+        for (int e = 0; e < 4; ++e) c[e] = mlp_act(c[e]);
         if (KEEP) h1keep[jb] = c;
         h1b[jb] = pack4(c[0], c[1], c[2], c[3]);
     }
@@ -187,7 +187,7 @@ __device__ __forceinline__ f32x4 mlp_tile(const bf16x4 A[MLP_NFRAG], const MlpBi
         for (int kb = 0; kb < 4; ++kb)
             c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[4 + jb * 4 + kb], h1b[kb], c, 0, 0, 0);
         #pragma unroll
-        for (int e = 0; e < 4; ++e) c[e] = mlp_tanh(c[e]);
+        for (int e = 0; e < 4; ++e) c[e] = mlp_act(c[e]);
         if (KEEP) h2keep[jb] = c;
         h2b[jb] = pack4(c[0], c[1], c[2], c[3]);
     }

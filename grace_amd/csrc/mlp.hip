@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(int64_t n, MlpDev w, const 
 // m_sj = 0 for NaN targets and states beyond n.  Forward: the MFMA tiles of
 // the inference path (bf16 operands, FP32 accumulation).  Backward in FP32:
 //   d3 = 2 m (y - t) inv_n
-//   dz2 = (d3 W3q^T) (1 - h2^2)      dz1 = (dz2 W2q^T) (1 - h1^2)
+//   dz2 = (d3 W3q^T) [h2 > 0]        dz1 = (dz2 W2q^T) [h1 > 0]
 //   dW3 = bf(h2)^T d3   dW2 = bf(h1)^T dz2   dW1 = bf(x)^T dz1   db = sums
 // with Wq the bf16-rounded weights and bf() the bf16 rounding the forward
 // applied to each layer input.  Partial sums go to part[wg][MLP_NP + 1]
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t n, int64_t ld, MlpDev w
     for (int k = 0; k < MLP_H; ++k) {
         float h = H2[l * HS + k];
         float dh = d3[0] * wq[MLP_W3 + k * MLP_NO] + d3[1] * wq[MLP_W3 + k * MLP_NO + 1];
-        DZ2[l * HS + k] = dh * (1.0f - h * h);
+        DZ2[l * HS + k] = dh * mlp_act_slope(h);
     }
     __syncthreads();
     float* P = part + (int64_t)blockIdx.x * (MLP_NP + 1);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t n, int64_t ld, MlpDev w
         }
         for (int k = 0; k < MLP_H; ++k) {
             float h = H1[l * HS + k];
-            H2[l * HS + k] = acc[k] * (1.0f - h * h);
+            H2[l * HS + k] = acc[k] * mlp_act_slope(h);
         }
     }
     __syncthreads();

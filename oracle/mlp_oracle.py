@@ -126,9 +126,9 @@ def forward(p, scale, feat, quant=True):
     x = q(normalise(feat, scale)).astype(np.float64)
     w1q, w2q, w3q = (q(w).astype(np.float64) for w in (w1, w2, w3))
     z1 = w1q.T @ x + b1[:, None]
-    h1 = np.tanh(z1)
+    h1 = np.maximum(z1, 0.0)
     z2 = w2q.T @ q(h1).astype(np.float64) + b2[:, None]
-    h2 = np.tanh(z2)
+    h2 = np.maximum(z2, 0.0)
     y = w3q.T @ q(h2).astype(np.float64) + b3[:, None]
     return y, dict(x=x, h1=h1, h2=h2, w2q=w2q, w3q=w3q)
 
@@ -150,8 +150,8 @@ def grad(p, scale, feat, target, quant=True):
     q = bf16 if quant else (lambda v: np.asarray(v, dtype=np.float64))
     h1, h2, x = a["h1"], a["h2"], a["x"]
     h1q, h2q = q(h1).astype(np.float64), q(h2).astype(np.float64)
-    dz2 = (a["w3q"] @ d3) * (1 - h2 * h2)
-    dz1 = (a["w2q"] @ dz2) * (1 - h1 * h1)
+    dz2 = (a["w3q"] @ d3) * (h2 > 0)
+    dz1 = (a["w2q"] @ dz2) * (h1 > 0)
     g = np.zeros(NP)
     g[W3:B3] = (h2q @ d3.T).ravel()
     g[B3:NP] = d3.sum(1)

@@ -4,8 +4,8 @@ numpy restatement (oracle/mlp_oracle.py) and the CPU c2p oracle.
 Tolerances: features within 4 FP32 ulp (+1e-6 absolute; device logf vs a correctly
 rounded log);
 forward outputs within 3e-2 (max) / 1e-3 (median) in normalised units --
-the GPU accumulates in a different order and its tanh differs by an ulp,
-which can flip single bf16 roundings of hidden activations (a wrong MFMA
+the GPU accumulates in a different order, which can flip single bf16
+roundings of hidden activations (a wrong MFMA
 operand layout or a transposed weight gives O(1)); gradients within 2e-2
 relative L2 per parameter block; learned c2p primitives within 1e-10 of the
 FP64 oracle (state-normalised, test/test_c2p.cpp:87-103).

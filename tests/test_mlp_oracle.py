@@ -31,7 +31,7 @@ def random_problem(n=40, seed=3):
 def test_grad_matches_finite_differences():
     """The backward pass of the oracle is the derivative of its forward pass
     (rounding off, FP64), checked by central differences on every kind of
-    parameter: a transposed weight, a dropped tanh' or a wrong sign fails."""
+    parameter: a transposed weight, a dropped ReLU slope or a wrong sign fails."""
     p, scale, feat, target = random_problem()
     g, loss = M.grad(p, scale, feat, target, quant=False)
     assert np.all(g[M.W1 + M.NF * M.H:M.B1] == 0)
