@@ -176,10 +176,11 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
         once(False, full, n, cons_k)
         once(True, full, n, cons_k)
         tp, tl = [], []
-        for _ in range(reps):
+        for _ in range(2 * reps + 1):
             tp.append(once(False, full, n, cons_k))
             tl.append(once(True, full, n, cons_k))
-        return float(np.median(tp)), float(np.median(tl))
+        ratio = float(np.median(np.array(tp) / np.array(tl)))
+        return float(np.median(tp)), float(np.median(tl)), ratio
 
     # bracket-only variant: width and moves picked by timing on a 1M-state slice
     de0, ds0, _ = mlp.get_bracket()
@@ -197,7 +198,7 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
     p_pl, e_pl = prims.clone(), err.clone()
 
     def learned(newton_evals):
-        t_p, t = run_pair(newton_evals)
+        t_p, t, ratio = run_pair(newton_evals)
         st_ = ctx.c2p_stats()
         same = e_pl == err
         res = state_residual(prims, p_pl)[same]
@@ -208,7 +209,7 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
                    "max_root_evals": st_["max_root_evals"],
                    "fallback_rate": round(1.0 - st_["n_guess_used"] / max(solves, 1), 5),
                    "plain_ms_alternated": round(t_p, 4),
-                   "speedup": round(t_p / t, 4),
+                   "speedup": round(ratio, 4),   # median over alternated plain/learned pairs
                    "error_vs_plain": {"max_state_rel": float(res.max()), "median_state_rel": float(res.median()),
                                       "same_error_bits": float(same.double().mean())}}
 

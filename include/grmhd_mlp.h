@@ -1,13 +1,16 @@
-/* grmhd_mlp.h -- learned Kastaun bracket: a 10 -> 64 -> 64 -> 2 MLP (BF16
- * MFMA, FP32 accumulation) that predicts the roots of the energy and
- * entropy-backup master functions, so that the FP64 solves start from a
- * narrow bracket instead of the Newton solve for mu+ and [0, mu+].
+/* grmhd_mlp.h -- learned Kastaun guess: a 10 -> 64 -> 64 -> 2 MLP (ReLU,
+ * BF16 MFMA, FP32 accumulation) that predicts the roots of the energy and
+ * entropy-backup master functions, so that the FP64 solves start from them
+ * (Newton finish, ideal gas) or from a narrow bracket around them, instead
+ * of the Newton solve for mu+ and Brent on [0, mu+].
  *
  * The reference has no counterpart: the network only changes where the
  * FP64 root solves of kastaun_c2p.hh:355-372 and ent_based_c2p.hh:214-229
- * start.  Every primitive still comes from the FP64 Kastaun solve, and any
- * bracket that misses the root falls back to the plain path, so results
- * agree with grmhd_c2p_batch to the c2p root tolerance.
+ * start.  Every primitive still comes from the FP64 Kastaun master
+ * function, and a start that does not converge (or a bracket that misses
+ * the root) falls back to the plain path, so results agree with
+ * grmhd_c2p_batch to within the root tolerances (1e-13 relative for the
+ * Newton finish, the plain solve's own for brackets).
  *
  * Parameters: FP32, GRMHD_MLP_NPARAM floats, row-major W[k][j]
  * (k input, j output):
@@ -123,9 +126,10 @@ grmhd_status grmhd_mlp_fit(grmhd_ctx* ctx, grmhd_mlp* mlp, int64_t n, const doub
                            const double* metric_soa, const double* r_or_null, const grmhd_mlp_fit_opts* opts,
                            grmhd_mlp_fit_report* report, grmhd_stream_t stream);
 
-/* grmhd_c2p_batch with the learned brackets: features, MLP and the FP64
- * Kastaun solves in one fused kernel.  Same outputs and statistics as
- * grmhd_c2p_batch (no dry run); roots_or_null as there. */
+/* grmhd_c2p_batch with the learned guess: features, MLP and the FP64
+ * Kastaun solves (Newton finish or brackets, grmhd_mlp_set_newton) in one
+ * fused kernel.  Same outputs and statistics as grmhd_c2p_batch (no dry
+ * run); roots_or_null as there. */
 grmhd_status grmhd_c2p_batch_learned(grmhd_ctx* ctx, grmhd_mlp* mlp, int64_t n, double* cons_soa,
                                      const double* metric_soa, double* prims_soa, uint64_t* err_bits,
                                      uint8_t* floored, const double* r_or_null, double* roots_or_null,
