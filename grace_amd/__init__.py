@@ -72,7 +72,7 @@ ABI_SYMBOLS = [
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
     "grmhd_autotune", "grmhd_get_tuning", "grmhd_set_tuning",
     "grmhd_setup_blast", "grmhd_setup_shocktube", "grmhd_kerr_schild", "grmhd_setup_c2p_states",
-    "grmhd_kerr_schild_adm", "grmhd_setup_blast_ks",
+    "grmhd_kerr_schild_adm", "grmhd_setup_blast_ks", "grmhd_setup_fm_torus", "grmhd_fm_torus_point",
     # include/grmhd_mlp.h
     "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
     "grmhd_mlp_get_params", "grmhd_mlp_set_scaling", "grmhd_mlp_get_scaling", "grmhd_mlp_set_bracket",
@@ -142,6 +142,9 @@ def lib():
     L.grmhd_setup_blast_ks.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, P, P, d, d, d, d, d, P, d,
                                        C.c_uint64, P, P, P, P, P]
     L.grmhd_setup_c2p_states.argtypes = [i64, C.c_uint64, d, d, P, P, P, P, P, P, P, P, P]
+    L.grmhd_setup_fm_torus.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, P, d, d, d, d, d, d, d,
+                                       C.c_uint64, P, P, P, P, P, P]
+    L.grmhd_fm_torus_point.argtypes = [d, d, d, d, d, d, d, P]
     f32 = C.c_float
     L.grmhd_mlp_init_params.argtypes = [C.c_uint64, P]
     L.grmhd_mlp_create.argtypes = [C.c_uint64, C.POINTER(P)]

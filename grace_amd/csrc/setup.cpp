@@ -7,9 +7,11 @@
 // (set_conservs_from_prims, :676-759).  Noise is drawn from a counter-based
 // SplitMix64 keyed by (seed, instance, padded global cell index) so any block
 // order gives bit-identical inputs.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 #include "phys.hh"
 #include "../../include/grmhd_setup.h"
@@ -28,6 +30,15 @@ inline uint64_t splitmix(uint64_t z) {
 inline double u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
     uint64_t h = splitmix(seed ^ splitmix(a + 0x632BE59BD9B4E019ull * (b + 1)) ^ splitmix(c * 0x8CB92BA72F3D8DD7ull + 7));
     return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// f(q0, q1, t) over [0, nq) split across host threads; t = thread index.
+template <typename F>
+void parallel_blocks(int64_t nq, F f) {
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nq, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back([&, t] { f(nq * t / nt, nq * (t + 1) / nt, t); });
+    for (auto& x : th) x.join();
 }
 
 struct Idx {
@@ -141,6 +152,102 @@ int setup_blast_impl(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_
     }
     return 0;
 }
+// ---------------------------------------------------------------------------
+// Fishbone-Moncrief torus (Fishbone & Moncrief 1976, ApJ 207, 962) around a
+// Kerr hole (M = 1, spin a) in Boyer-Lindquist (r, theta), with the constant
+// l = u^t u_phi.  With Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2r + a^2,
+// A = (r^2+a^2)^2 - a^2 Delta sin^2 and e = Sigma^2 Delta / (A^2 sin^2)
+// (= alpha^2 / g_phiphi):
+//   ln h = 1/2 ln[(1 + sqrt(1 + 4 l^2 e)) A / (Sigma Delta)]
+//          - 1/2 sqrt(1 + 4 l^2 e) - 2 a r l / A  - (the same at r_in, pi/2)
+// and the azimuthal flow: U = u_(phi) in the zero-angular-momentum frame
+// solves U^2 (1 + U^2) = l^2 e, u^t = sqrt(1 + U^2)/alpha,
+// u^phi = u^t omega + U / sqrt(g_phiphi), omega = 2 a r / A.  (These satisfy
+// u.u = -1, u^t u_phi = l and the relativistic Euler equation of a
+// stationary azimuthal flow, d ln h = -d ln(-u_t) + Omega/(1 - Omega l_s) d l_s
+// with l_s = -u_phi/u_t; tests/test_torus_setup.py checks all three.)
+// ---------------------------------------------------------------------------
+struct FMTorus {
+    double a, l;
+    void parts(double r, double th, double& Sig, double& Del, double& A, double& s2) const {
+        const double st = std::sin(th), ct = std::cos(th);
+        s2 = st * st;
+        Sig = r * r + a * a * ct * ct;
+        Del = r * r - 2.0 * r + a * a;
+        A = (r * r + a * a) * (r * r + a * a) - a * a * Del * s2;
+    }
+    double lnh_raw(double r, double th) const {
+        double Sig, Del, A, s2;
+        parts(r, th, Sig, Del, A, s2);
+        const double t = std::sqrt(1.0 + 4.0 * l * l * Sig * Sig * Del / (A * A * s2));
+        return 0.5 * std::log((1.0 + t) * A / (Sig * Del)) - 0.5 * t - 2.0 * a * r * l / A;
+    }
+    // BL u^t and u^phi of the torus flow
+    void vel(double r, double th, double& ut, double& uph) const {
+        double Sig, Del, A, s2;
+        parts(r, th, Sig, Del, A, s2);
+        const double e = Sig * Sig * Del / (A * A * s2);
+        const double U = std::sqrt(0.5 * (-1.0 + std::sqrt(1.0 + 4.0 * l * l * e)));
+        const double alpha = std::sqrt(Sig * Del / A), om = 2.0 * a * r / A, gpp = A * s2 / Sig;
+        ut = std::sqrt(1.0 + U * U) / alpha;
+        uph = ut * om + U / std::sqrt(gpp);
+    }
+};
+
+// l such that the equatorial enthalpy peaks at r_max: d(ln h)/dr = 0 there
+// (bisection on a central difference; the slope decreases with l).
+double fm_ell(double a, double r_max) {
+    auto slope = [&](double l) {
+        FMTorus T{a, l};
+        const double h = 1e-5 * r_max;
+        return (T.lnh_raw(r_max + h, 0.5 * M_PI) - T.lnh_raw(r_max - h, 0.5 * M_PI)) / (2.0 * h);
+    };
+    double lo = 1.0, hi = 50.0, flo = slope(lo), fhi = slope(hi);
+    if (!(flo * fhi < 0.0)) return -1.0;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi), fm = slope(mid);
+        if ((fm < 0.0) == (flo < 0.0)) { lo = mid; flo = fm; } else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// Boyer-Lindquist r and theta of a Cartesian Kerr-Schild point
+// (x + iy = (r + ia) e^{i phi} sin(theta), z = r cos(theta)).
+void ks_r_theta(double a, double x, double y, double z, double& r, double& th) {
+    const double R2 = x * x + y * y + z * z, a2 = a * a;
+    const double r2 = 0.5 * (R2 - a2 + std::sqrt((R2 - a2) * (R2 - a2) + 4.0 * a2 * z * z));
+    r = std::sqrt(r2);
+    th = std::acos(std::fmax(-1.0, std::fmin(1.0, z / r)));
+}
+
+struct FMPoint { double rho, p, z[3], ut; bool in; };
+// Primitives of the torus at a Cartesian Kerr-Schild point (ideal gas, rho
+// normalised to 1 at the pressure maximum; atmosphere outside).
+FMPoint fm_point(const FMTorus& T, double lnh_in, double kappa, double gamma, double r_in, double rho_atm,
+                 double p_atm, double x, double y, double z, const double beta_up[3]) {
+    FMPoint P{rho_atm, p_atm, {0.0, 0.0, 0.0}, 0.0, false};
+    double r, th;
+    ks_r_theta(T.a, x, y, z, r, th);
+    const double rh = 1.0 + std::sqrt(1.0 - T.a * T.a);
+    if (r < r_in || r <= rh || std::sin(th) < 1e-12) return P;
+    const double lnh = T.lnh_raw(r, th) - lnh_in;
+    if (!(lnh > 0.0)) return P;
+    const double h = std::exp(lnh);
+    const double rho = std::pow((h - 1.0) * (gamma - 1.0) / (kappa * gamma), 1.0 / (gamma - 1.0));
+    if (!(rho > rho_atm)) return P;
+    double ut, uph;
+    T.vel(r, th, ut, uph);
+    // Kerr-Schild u^t, u^phi equal the Boyer-Lindquist ones for u^r = 0;
+    // u^x = dx/dphi u^phi = -y u^phi, u^y = x u^phi; Valencia z^i = u^i + u^t beta^i
+    const double u[3] = {-y * uph, x * uph, 0.0};
+    P.rho = rho;
+    P.p = kappa * std::pow(rho, gamma);
+    for (int i = 0; i < 3; ++i) P.z[i] = u[i] + ut * beta_up[i];
+    P.ut = ut;
+    P.in = true;
+    return P;
+}
+
 }  // namespace
 
 extern "C" {
@@ -305,6 +412,155 @@ int grmhd_setup_blast_ks(const grmhd_grid* g, const grmhd_instance* inst, int32_
         finish_cells(g, X, state, B, aux, I.idx, q);
     }
     return 0;
+}
+
+// FM torus in the Kerr-Schild metric (Cowling): metric + K_ij in every cell,
+// torus primitives (atmosphere rho_atm, p_atm outside, p times
+// (1 + noise U[-1,1])), and a single poloidal loop from the vector potential
+// A_phi = max(rho/rho_max - 0.2, 0) (A = A_phi dphi with phi = atan2(y, x)),
+// whose edge values give the densitised face fluxes by the discrete curl
+// (divergence free to round-off), scaled so that max p / max(b^2/2) = beta
+// over the interior cells.  info (or null): l, kappa, B scale, max b^2/2.
+int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
+                         const double lo[3], double spin, double r_in, double r_max, double beta, double rho_atm,
+                         double p_atm, double noise, uint64_t seed, double* state, double* Bx, double* By,
+                         double* Bz, double* aux, double info[4]) {
+    const double l = fm_ell(spin, r_max);
+    if (!(l > 0.0)) return 2;
+    FMTorus T{spin, l};
+    const double lnh_in = T.lnh_raw(r_in, 0.5 * M_PI);
+    const double hmax = std::exp(T.lnh_raw(r_max, 0.5 * M_PI) - lnh_in);
+    if (!(hmax > 1.0)) return 3;
+    const double kappa = (hmax - 1.0) * (gamma - 1.0) / gamma;   // rho(r_max) = 1
+    const double gm1 = gamma - 1.0;
+    Idx X(g);
+    double* B[3] = {Bx, By, Bz};
+    std::memset(aux, 0, sizeof(double) * NAUX * X.ncell * g->nq);
+    const int n[3] = {g->nx, g->ny, g->nz};
+    std::vector<double> pmax_t(16, 0.0), pmag_t(16, 0.0);
+    std::vector<int> bad(16, 0);
+    // pass 1: metric, primitives, unscaled face fluxes
+    parallel_blocks(g->nq, [&](int64_t qa, int64_t qb, int t) {
+    double& pmax = pmax_t[t];
+    for (int64_t q = qa; q < qb; ++q) {
+        int a, b[3];
+        if (!find_block(inst, n_inst, q, a, b)) { bad[t] = 1; return; }
+        const grmhd_instance& I = inst[a];
+        double dx[3];
+        int Ntot[3];
+        for (int d = 0; d < 3; ++d) { dx[d] = 1.0 / I.idx[d]; Ntot[d] = I.nb[d] * n[d]; }
+        auto pos = [&](int d, int loc, double off) { return lo[d] + (b[d] * n[d] + loc - g->ngz + off) * dx[d]; };
+        for (int k = 0; k < X.N[2]; ++k)
+            for (int j = 0; j < X.N[1]; ++j)
+                for (int i = 0; i < X.N[0]; ++i) {
+                    const int64_t c = X.c(i, j, k);
+                    const double x = pos(0, i, 0.5), y = pos(1, j, 0.5), z = pos(2, k, 0.5);
+                    double gg[6], al, be[3], K[6];
+                    grmhd_kerr_schild_adm(spin, x, y, z, gg, &al, be, K);
+                    auto S = [&](int v) -> double& { return state[(q * NSTATE + v) * X.ncell + c]; };
+                    for (int s2 = 0; s2 < 6; ++s2) { S(GXX_ + s2) = gg[s2]; S(KXX_ + s2) = K[s2]; }
+                    S(ALP_) = al;
+                    for (int d = 0; d < 3; ++d) S(BETAX_ + d) = be[d];
+                    FMPoint P = fm_point(T, lnh_in, kappa, gamma, r_in, rho_atm, p_atm, x, y, z, be);
+                    const int gl[3] = {b[0] * n[0] + i - g->ngz, b[1] * n[1] + j - g->ngz, b[2] * n[2] + k - g->ngz};
+                    const int64_t key = (int64_t)(gl[0] + g->ngz) + (int64_t)(Ntot[0] + 2 * g->ngz) *
+                                        ((gl[1] + g->ngz) + (int64_t)(Ntot[1] + 2 * g->ngz) * (gl[2] + g->ngz));
+                    double p = P.p * (1.0 + noise * (2.0 * u01(seed, a, (uint64_t)key, 1) - 1.0));
+                    const bool interior = i >= g->ngz && i < g->ngz + n[0] && j >= g->ngz && j < g->ngz + n[1] &&
+                                          k >= g->ngz && k < g->ngz + n[2];
+                    if (interior) pmax = std::fmax(pmax, p);
+                    auto A = [&](int v) -> double& { return aux[(q * NAUX + v) * X.ncell + c]; };
+                    const double eps = p / (gm1 * P.rho);
+                    A(RHO_) = P.rho; A(PRESS_) = p; A(EPS_) = eps; A(TEMP_) = eps * gm1;
+                    A(ENTROPY_) = std::log(eps) / gm1 - std::log(P.rho);
+                    A(ZVECX_) = P.z[0]; A(ZVECY_) = P.z[1]; A(ZVECZ_) = P.z[2]; A(YE_) = 0.0;
+                }
+        // A_phi at a point (zero outside the torus), A_x = -y A_phi/R^2, A_y = x A_phi/R^2
+        auto Aphi = [&](double x, double y, double z) {
+            double gg[6], al, be[3];
+            grmhd_kerr_schild(spin, x, y, z, gg, &al, be);
+            const FMPoint P = fm_point(T, lnh_in, kappa, gamma, r_in, rho_atm, p_atm, x, y, z, be);
+            return P.in ? std::fmax(P.rho - 0.2, 0.0) : 0.0;
+        };
+        auto Ax = [&](double x, double y, double z) { const double R2 = x * x + y * y; return R2 > 0 ? -y * Aphi(x, y, z) / R2 : 0.0; };
+        auto Ay = [&](double x, double y, double z) { const double R2 = x * x + y * y; return R2 > 0 ? x * Aphi(x, y, z) / R2 : 0.0; };
+        // densitised face fluxes = discrete curl of the edge potential (A_z = 0):
+        // Bx = -dAy/dz, By = dAx/dz, Bz = dAy/dx - dAx/dy
+        for (int d = 0; d < 3; ++d) {
+            const int E[3] = {X.N[0] + (d == 0), X.N[1] + (d == 1), X.N[2] + (d == 2)};
+            for (int k = 0; k < E[2]; ++k)
+                for (int j = 0; j < E[1]; ++j)
+                    for (int i = 0; i < E[0]; ++i) {
+                        double v = 0.0;
+                        if (d == 0) {   // face (x_i, y_j+1/2, z_k+1/2): A_y on the z-edges
+                            const double x = pos(0, i, 0.0), y = pos(1, j, 0.5);
+                            v = -(Ay(x, y, pos(2, k, 1.0)) - Ay(x, y, pos(2, k, 0.0))) / dx[2];
+                        } else if (d == 1) {
+                            const double x = pos(0, i, 0.5), y = pos(1, j, 0.0);
+                            v = (Ax(x, y, pos(2, k, 1.0)) - Ax(x, y, pos(2, k, 0.0))) / dx[2];
+                        } else {
+                            const double z = pos(2, k, 0.0);
+                            v = (Ay(pos(0, i, 1.0), pos(1, j, 0.5), z) - Ay(pos(0, i, 0.0), pos(1, j, 0.5), z)) / dx[0] -
+                                (Ax(pos(0, i, 0.5), pos(1, j, 1.0), z) - Ax(pos(0, i, 0.5), pos(1, j, 0.0), z)) / dx[1];
+                        }
+                        B[d][q * X.nface[d] + X.f(d, i, j, k)] = v;
+                    }
+        }
+    }
+    });
+    for (int t = 0; t < 16; ++t) if (bad[t]) return 1;
+    // b^2/2 maximum over the interior cells (cell B from the face average)
+    parallel_blocks(g->nq, [&](int64_t qa, int64_t qb, int t) {
+    double& pmag = pmag_t[t];
+    for (int64_t q = qa; q < qb; ++q)
+        for (int k = g->ngz; k < g->ngz + n[2]; ++k)
+            for (int j = g->ngz; j < g->ngz + n[1]; ++j)
+                for (int i = g->ngz; i < g->ngz + n[0]; ++i) {
+                    const int64_t c = X.c(i, j, k);
+                    Met m;
+                    for (int s2 = 0; s2 < 6; ++s2) m.g[s2] = state[(q * NSTATE + GXX_ + s2) * X.ncell + c];
+                    m.alp = state[(q * NSTATE + ALP_) * X.ncell + c];
+                    for (int s2 = 0; s2 < 3; ++s2) m.beta[s2] = state[(q * NSTATE + BETAX_ + s2) * X.ncell + c];
+                    m.init();
+                    const double Bc[3] = {
+                        0.5 * (B[0][q * X.nface[0] + X.f(0, i, j, k)] + B[0][q * X.nface[0] + X.f(0, i + 1, j, k)]) / m.sqrtg,
+                        0.5 * (B[1][q * X.nface[1] + X.f(1, i, j, k)] + B[1][q * X.nface[1] + X.f(1, i, j + 1, k)]) / m.sqrtg,
+                        0.5 * (B[2][q * X.nface[2] + X.f(2, i, j, k)] + B[2][q * X.nface[2] + X.f(2, i, j, k + 1)]) / m.sqrtg};
+                    const double zz[3] = {aux[(q * NAUX + ZVECX_) * X.ncell + c], aux[(q * NAUX + ZVECY_) * X.ncell + c],
+                                          aux[(q * NAUX + ZVECZ_) * X.ncell + c]};
+                    const double W = lorentz(m.g, zz);
+                    double bb[4], b2;
+                    smallb(m, Bc, zz, W, bb, b2);
+                    pmag = std::fmax(pmag, 0.5 * b2);
+                }
+    });
+    double pmax = 0.0, pmag = 0.0;
+    for (int t = 0; t < 16; ++t) { pmax = std::fmax(pmax, pmax_t[t]); pmag = std::fmax(pmag, pmag_t[t]); }
+    const double scale = (pmag > 0.0 && beta > 0.0) ? std::sqrt(pmax / (beta * pmag)) : 0.0;
+    parallel_blocks(g->nq, [&](int64_t qa, int64_t qb, int) {
+        for (int64_t q = qa; q < qb; ++q) {
+            int a, b[3];
+            find_block(inst, n_inst, q, a, b);
+            for (int d = 0; d < 3; ++d)
+                for (int64_t f = 0; f < X.nface[d]; ++f) B[d][q * X.nface[d] + f] *= scale;
+            finish_cells(g, X, state, B, aux, inst[a].idx, q);
+        }
+    });
+    if (info) { info[0] = l; info[1] = kappa; info[2] = scale; info[3] = pmag * scale * scale; }
+    return 0;
+}
+
+void grmhd_fm_torus_point(double spin, double gamma, double r_in, double r_max, double x, double y, double z,
+                          double out[6]) {
+    const double l = fm_ell(spin, r_max);
+    FMTorus T{spin, l};
+    const double lnh_in = T.lnh_raw(r_in, 0.5 * M_PI);
+    const double hmax = std::exp(T.lnh_raw(r_max, 0.5 * M_PI) - lnh_in);
+    const double kappa = (hmax - 1.0) * (gamma - 1.0) / gamma;
+    double gg[6], al, be[3];
+    grmhd_kerr_schild(spin, x, y, z, gg, &al, be);
+    const FMPoint P = fm_point(T, lnh_in, kappa, gamma, r_in, 0.0, 0.0, x, y, z, be);
+    out[0] = P.rho; out[1] = P.p; out[2] = P.z[0]; out[3] = P.z[1]; out[4] = P.z[2]; out[5] = P.ut;
 }
 
 // Config-4 c2p states: rho log-U[1e-12,1e-2], eps log-U[1e-6,1], W U[1,10]

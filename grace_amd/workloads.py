@@ -88,6 +88,43 @@ def blast_ks(nb=(2, 2, 2), n=16, ngz=3, spin=0.9, center=(8.0, 0.0, 0.0), size=4
                     extra=dict(N=N, dx=size / N[0]))
 
 
+def fm_torus(nb=(4, 4, 4), n=16, ngz=3, extent=48.0, spin=0.9375, r_in=6.0, r_max=12.0, beta=100.0,
+             rho_atm=1e-10, p_atm=1e-14, gamma=4.0 / 3.0, noise=0.01, seed=SEED, emf=EMF_UCT, excision_lapse=0.55):
+    """Fishbone-Moncrief torus (r_in 6 M, pressure maximum 12 M, Gamma 4/3,
+    single poloidal loop with max p / max p_mag = 100) around a spin-0.9375
+    hole at the centre of the cube [-extent, extent]^3, Kerr-Schild metric
+    (Cowling), atmosphere rho 1e-10; one uniform level of n^3 blocks.
+    Cells near the ring singularity (lapse <= excision_lapse, all inside the
+    horizon) are excised."""
+    nq = nb[0] * nb[1] * nb[2]
+    grid = Grid(n, n, n, ngz, nq)
+    N = [nb[d] * n for d in range(3)]
+    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(N[d] / (2.0 * extent) for d in range(3)))]
+    inst = make_instances(spec)
+    state, B, aux = alloc(grid)
+    lo = np.array([-extent] * 3, dtype=np.float64)
+    info = np.zeros(4)
+    rc = lib().grmhd_setup_fm_torus(grid, inst, 1, gamma, _p(lo), spin, r_in, r_max, beta, rho_atm, p_atm, noise,
+                                    seed, _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux), _p(info))
+    assert rc == 0, rc
+    params = default_params(gamma=gamma, emf=emf, flat_metric=0)
+    params.rho_fl = rho_atm
+    # atmosphere temperature p/rho: a zero floor would give the reset cells
+    # eps = 0 and an entropy of -inf (ideal_gas_eos.hh:78-81)
+    params.temp_fl = p_atm / rho_atm
+    params.excision_lapse = excision_lapse
+    return Workload("fm_torus", params, grid, spec, state, B, aux, zones=nq * n ** 3,
+                    extra=dict(N=N, dx=2.0 * extent / N[0], ell=info[0], kappa=info[1], bscale=info[2],
+                               pmag_max=info[3]))
+
+
+def fm_torus_point(x, y, z, spin=0.9375, gamma=4.0 / 3.0, r_in=6.0, r_max=12.0):
+    """Torus primitives at one Kerr-Schild point: rho, p, z^i, u^t."""
+    out = np.zeros(6)
+    lib().grmhd_fm_torus_point(spin, gamma, r_in, r_max, x, y, z, _p(out))
+    return out
+
+
 def multi_blast(n_inst=64, sizes=(32, 48, 64, 80, 96), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01):
     """Uneven instances: each a cubic blast-wave grid of side drawn from `sizes`,
     cut into n^3 blocks; all instances share one block list."""

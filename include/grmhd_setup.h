@@ -40,6 +40,22 @@ int grmhd_setup_blast_ks(const grmhd_grid* g, const grmhd_instance* inst, int32_
                          double p_in, double r_in, const double Bvec[3], double noise, uint64_t seed, double* state,
                          double* Bx, double* By, double* Bz, double* aux);
 
+/* Fishbone-Moncrief torus (inner edge r_in, pressure maximum r_max, rho = 1
+ * there) around a spin-a hole at the origin, Kerr-Schild metric and K_ij in
+ * every cell, atmosphere rho_atm/p_atm outside, p times (1 + noise U[-1,1]);
+ * single poloidal loop from A_phi = max(rho - 0.2, 0) by the discrete curl
+ * (divergence free), scaled to max p / max(b^2/2) = beta.
+ * info (or null): l = u^t u_phi, kappa (p = kappa rho^gamma), B scale,
+ * max b^2/2.  Returns 0, or nonzero for bad parameters / blocks. */
+int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
+                         const double lo[3], double spin, double r_in, double r_max, double beta, double rho_atm,
+                         double p_atm, double noise, uint64_t seed, double* state, double* Bx, double* By,
+                         double* Bz, double* aux, double info[4]);
+/* The torus primitives at one Cartesian Kerr-Schild point (no atmosphere):
+ * out = {rho, p, z^x, z^y, z^z, u^t}. */
+void grmhd_fm_torus_point(double spin, double gamma, double r_in, double r_max, double x, double y, double z,
+                          double out[6]);
+
 /* Seeded c2p test states (SoA outputs, see grmhd_abi.h for the layouts). */
 int grmhd_setup_c2p_states(int64_t n, uint64_t seed, double gamma, double spin, const double rho_rng[2],
                            const double eps_rng[2], const double W_rng[2], const double sig_rng[2],
