@@ -272,6 +272,62 @@ def c3_leg(ga, W, torch, dev, steps=3):
             "ms_per_step_default_shapes": 1e3 * t_default, "tuning": tune}
 
 
+def torus_leg(ga, W, torch, dev, nb=12, steps=5, warmup=2):
+    """Fishbone-Moncrief torus (r_in 6 M, r_max 12 M, Gamma 4/3, beta 100
+    loop) around a spin-0.9375 hole in the Kerr-Schild metric: one uniform
+    level of 16^3 blocks covering [-48, 48]^3 M (192^3 zones, the fine level
+    of the two-level forest everywhere), device CFL dt, graph RK3 steps with
+    the plain c2p and with the learned guess (network trained on the torus's
+    own cells)."""
+    log(f"[bench] torus leg: {nb}^3 blocks of 16^3")
+    wl = W.fm_torus(nb=(nb, nb, nb), n=16, extent=48.0)
+    mlp = ga.Mlp()
+    out = {"workload": "Fishbone-Moncrief torus a=0.9375 r_in=6 r_max=12 beta=100, Kerr-Schild (Cowling), "
+                       "192^3 on [-48,48]^3 M, 1728 blocks of 16^3, ngz 3, WENO-Z + HLLE + UCT, rho_atm 1e-10",
+           "zones": wl.zones, "steps": steps}
+    for mode in ("plain", "learned"):
+        ctx = ga.Context(wl.params, wl.grid, wl.instances)
+        st = torch.from_numpy(wl.state).to(dev)
+        B = [torch.from_numpy(b).to(dev) for b in wl.B]
+        aux = torch.from_numpy(wl.aux).to(dev)
+        if mode == "learned":
+            t0 = time.perf_counter()
+            rep = mlp.fit_cells(ctx, st, B, max_states=1 << 20)
+            out["fit_seconds"] = round(time.perf_counter() - t0, 3)
+            ctx.set_learned_c2p(mlp)
+        dtd = torch.zeros(1, dtype=torch.float64, device=dev)
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            sp = stream.cuda_stream
+            for _ in range(warmup):
+                ctx.compute_dt(st, aux, 0.25, dtd, stream=sp)
+                ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=True)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.compute_dt(st, aux, 0.25, dtd, stream=sp)
+                ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=True)
+            stream.synchronize()
+            wall = time.perf_counter() - t0
+            ctx.set_timing(1)
+            ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=False)
+            stream.synchronize()
+            c2p_ms = ctx.kernel_time_ms("c2p")[0]
+            ctx.set_timing(0)
+        ctx.compute_auxiliaries(st, B, aux)
+        torch.cuda.synchronize()
+        stt = ctx.c2p_stats()
+        out[mode] = {"ms_per_step": round(1e3 * wall / steps, 4), "zone_updates_per_s": wl.zones * steps / wall,
+                     "c2p_ms_per_step": round(c2p_ms, 4), "dt": float(dtd.item()),
+                     "finite": bool(torch.isfinite(st).all().item() and torch.isfinite(aux).all().item()),
+                     "root_evals_per_cell": round(stt["n_root_evals"] / max(stt["n_cells"], 1), 3),
+                     "bracket_evals_per_cell": round(stt["n_bracket_evals"] / max(stt["n_cells"], 1), 3),
+                     "atmosphere_fraction": round(stt["n_atmo_reset"] / max(stt["n_cells"], 1), 4)}
+        del ctx, st, B, aux
+    out["learned_vs_plain_step"] = round(out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"], 4)
+    return out
+
+
 def c1_leg(ga, W, torch, dev, t_final=0.4, cfl=0.25):
     """Balsara-1 to t = 0.4 (400 cells, GS CT, device CFL dt read back every
     step); the oracle replays the same dt sequence for the parity figure."""
@@ -332,6 +388,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-c2p", action="store_true", help="skip the learned-vs-plain c2p leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven-instance leg")
+    ap.add_argument("--no-torus", action="store_true", help="skip the Kerr-Schild torus leg")
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
@@ -426,6 +483,7 @@ def main():
         c2p_leg = c4_leg(ga, W, torch, dev)
     c1 = None if args.no_c1 else c1_leg(ga, W, torch, dev)
     c3 = None if args.no_c3 else c3_leg(ga, W, torch, dev)
+    torus = None if args.no_torus else torus_leg(ga, W, torch, dev)
 
     cpu = None
     if not args.no_cpu:
@@ -456,6 +514,7 @@ def main():
         "c2p_learned_vs_plain": c2p_leg,
         "balsara1_full_run": c1,
         "batched_uneven": c3,
+        "fm_torus": torus,
     }
     print(json.dumps(out), flush=True)
 

@@ -76,7 +76,8 @@ ABI_SYMBOLS = [
     # include/grmhd_mlp.h
     "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
     "grmhd_mlp_get_params", "grmhd_mlp_set_scaling", "grmhd_mlp_get_scaling", "grmhd_mlp_set_bracket",
-    "grmhd_mlp_get_bracket", "grmhd_mlp_set_newton", "grmhd_mlp_get_newton", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
+    "grmhd_mlp_get_bracket", "grmhd_mlp_set_newton", "grmhd_mlp_get_newton",
+    "grmhd_mlp_fit_cells", "grmhd_set_learned_c2p", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
     "grmhd_mlp_train_step", "grmhd_mlp_default_fit_opts", "grmhd_mlp_fit", "grmhd_c2p_batch_learned",
 ]
 
@@ -167,6 +168,8 @@ def lib():
     L.grmhd_mlp_default_fit_opts.restype = None
     L.grmhd_mlp_fit.argtypes = [P, P, i64, P, P, P, C.POINTER(MlpFitOpts), C.POINTER(MlpFitReport), P]
     L.grmhd_c2p_batch_learned.argtypes = [P, P, i64, P, P, P, P, P, P, P, P]
+    L.grmhd_mlp_fit_cells.argtypes = [P, P, P, P * 3, i64, C.POINTER(MlpFitOpts), C.POINTER(MlpFitReport), P]
+    L.grmhd_set_learned_c2p.argtypes = [P, P]
     for name in ABI_SYMBOLS:
         getattr(L, name)  # raises AttributeError if an exported symbol is missing
     _lib = L
@@ -278,6 +281,11 @@ class Context:
         fn = self.L.grmhd_rk3_step_graph if graph else self.L.grmhd_rk3_step
         check(fn(self.h, _ptr(dt_dev), _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)), "rk3_step")
 
+    def set_learned_c2p(self, mlp):
+        """Cell-pass c2p from the MLP's guess (mlp: Mlp or None for plain)."""
+        check(self.L.grmhd_set_learned_c2p(self.h, mlp.h if mlp is not None else None), "set_learned_c2p")
+        self._mlp = mlp   # keep it alive while the context uses it
+
     def autotune(self, state, B, aux, reps=3, stream=None):
         t = Tuning()
         check(self.L.grmhd_autotune(self.h, _ptr(state), ptr3(*B), _ptr(aux), int(reps), C.byref(t), _ptr(stream)),
@@ -384,6 +392,14 @@ class Mlp:
         for k, v in kw.items():
             setattr(o, k, v)
         return o
+
+    def fit_cells(self, ctx, state, B, max_states=1 << 20, opts=None, stream=None):
+        """Train on the c2p inputs of up to max_states interior cells of ctx's blocks."""
+        rep = MlpFitReport()
+        o = opts or self.default_fit_opts()
+        check(self.L.grmhd_mlp_fit_cells(ctx.h, self.h, _ptr(state), ptr3(*B), int(max_states), C.byref(o),
+                                         C.byref(rep), _ptr(stream)), "mlp_fit_cells")
+        return {k: getattr(rep, k) for k, _ in MlpFitReport._fields_}
 
     def fit(self, ctx, n, cons, met, r=None, opts=None, stream=None):
         rep = MlpFitReport()

@@ -960,14 +960,16 @@ __global__ __launch_bounds__(256) void k_stats_final(const DevStats* __restrict_
 // per block, see wg_pos), CS_SHELL (the ghost-shell cells of every block),
 // CS_LIST (explicit {block, cell} pairs).
 enum { CS_INTERIOR = 0, CS_SHELL = 1, CS_LIST = 2 };
-template <typename EOS, bool FLAT>
+// LEARNED: the root solves start from the MLP's prediction (ideal gas:
+// Newton finish, as k_c2p_batch<EOS, true, true>; other EOS: brackets).
+template <typename EOS, bool FLAT, bool LEARNED = false>
 __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EOS eos, Opts o, const DevInst* __restrict__ inst,
                                                    const DevBlk* __restrict__ blk, double* st, int snv,
                                                    const double* __restrict__ mstate, int mnv,
                                                    const double* __restrict__ Bx, const double* __restrict__ By,
                                                    const double* __restrict__ Bz, double* aux, double* ecen,
                                                    DevStats* stats, int cset, const int32_t* __restrict__ shell,
-                                                   const int2* __restrict__ cl, int ne, int wpb) {
+                                                   const int2* __restrict__ cl, int ne, int wpb, MlpDev mlp) {
     WgPos wp{0, 0};
     if (cset != CS_LIST) wp = wg_pos(wpb);
     const int e = cset == CS_LIST ? (int)(blockIdx.x * blockDim.x + threadIdx.x)
@@ -1012,9 +1014,35 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
     double pr[NPRIM];
     C2PCount cnt;
     C2POut out{false, false};
+    C2PGuess gs{0, 0, 0, 0, 0, 0};
+    if constexpr (LEARNED) {
+        // every thread of the block takes part (the forward holds barriers);
+        // inactive threads run on the valid cell ee = 0 and discard the result
+        __shared__ __align__(16) uint16_t xs[4][64 * MLP_KP];
+        __shared__ __align__(16) float ys[4][64 * 2];
+        float f[MLP_NF];
+        double lxhi = 0.0;
+        mlp_features(eos, m, cons, f, lxhi);
+        float y0, y1;
+        mlp_block_forward(mlp, f, xs[threadIdx.x >> 6], ys[threadIdx.x >> 6], y0, y1);
+        const double xe = (double)__expf((float)lxhi + y0);
+        const double z = (double)y1;
+        const double rs = (fabs(z) < 1e-3) ? 1.0 + z * (1.0 + z * (0.5 + z * (1.0 / 6.0))) : exp(z);
+        if constexpr (EOS::is_ideal) {
+            gs.newton = mlp.newton;
+            gs.e_c = xe;
+            gs.s_c = rs;
+        } else {
+            gs.e_lo = xe * mlp.fe_lo;
+            gs.e_hi = fmin(xe * mlp.fe_hi, 1.0);
+            gs.s_lo = rs * mlp.fs_lo;
+            gs.s_hi = rs * mlp.fs_hi;
+            gs.nexp = mlp.nexp;
+        }
+        gs.s_rel = 1;
+    }
     if (active) {
         A[BX_ * nc] = Bc0 / m.sqrtg; A[BY_ * nc] = Bc1 / m.sqrtg; A[BZ_ * nc] = Bc2 / m.sqrtg;
-        C2PGuess gs{0, 0, 0, 0, 0, 0};
         C2PRoots rt{-1, -1, -1, -1};
         out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, 1e300, err, false, cnt, gs, rt);
         A[RHO_ * nc] = pr[RHOL]; A[EPS_ * nc] = pr[EPSL]; A[PRESS_ * nc] = pr[PRESSL];
@@ -1200,6 +1228,7 @@ __global__ __launch_bounds__(256) void k_reset_diag(int64_t ncell, int64_t nq, d
 struct TimerRec { std::string fam; hipEvent_t a, b; };
 
 struct grmhd_ctx {
+    grmhd_mlp* mlp = nullptr;                 // learned c2p guess for the cell passes (not owned)
     grmhd_params P;
     DevGrid G;
     Opts O;
@@ -1628,12 +1657,18 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     const int ni = G.n[0] * G.n[1] * G.n[2];
     const int bs = c->tune.c2p;
     int64_t npart = 0;
+    const MlpDev md = c->mlp ? c->mlp->dev() : MlpDev{};
     auto launch = [&](int cset, int ne, dim3 grid, int wpb) {
         DevStats* part = with_stats ? c->stats_part + npart : nullptr;
         npart += (int64_t)grid.x * grid.y;
-#define C2PL(EOSV, FL) k_c2p_cells<decltype(EOSV), FL><<<grid, bs, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, wpb)
-        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true); else C2PL(c->ideal, false); }
-        else { if (flat) C2PL(c->hybrid, true); else C2PL(c->hybrid, false); }
+#define C2PL(EOSV, FL, LR) k_c2p_cells<decltype(EOSV), FL, LR><<<grid, bs, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, wpb, md)
+        if (c->mlp) {
+            if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, true); else C2PL(c->ideal, false, true); }
+            else { if (flat) C2PL(c->hybrid, true, true); else C2PL(c->hybrid, false, true); }
+        } else {
+            if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, false); else C2PL(c->ideal, false, false); }
+            else { if (flat) C2PL(c->hybrid, true, false); else C2PL(c->hybrid, false, false); }
+        }
 #undef C2PL
     };
     {
@@ -1790,6 +1825,66 @@ extern "C" grmhd_status grmhd_c2p_batch_learned(grmhd_ctx* c, grmhd_mlp* mlp, in
     }
     c->last_c2p_n = n;
     return launch_check("c2p_batch_learned");
+}
+
+static void drop_graph(grmhd_ctx* c);
+
+// c2p inputs (densitised conserved with cell-centred face-average B, and the
+// metric) of n interior cells spread evenly over the block list, as SoA in
+// the layout of grmhd_c2p_batch: sample s is interior cell s*stride.
+__global__ __launch_bounds__(256) void k_c2p_inputs(DevGrid G, const double* __restrict__ st, int snv,
+                                                   const double* __restrict__ Bx, const double* __restrict__ By,
+                                                   const double* __restrict__ Bz, int flat, int64_t n,
+                                                   int64_t stride, double* __restrict__ cons_soa,
+                                                   double* __restrict__ met) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t per = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    const int64_t gi = s * stride, q = gi / per, e = gi - q * per;
+    const int k = G.ng + (int)(e / ((int64_t)G.n[0] * G.n[1]));
+    const int j = G.ng + (int)((e / G.n[0]) % G.n[1]);
+    const int i = G.ng + (int)(e % G.n[0]);
+    const int64_t nc = G.ncell, c = cidx(G, i, j, k);
+    const double* S = st + q * (int64_t)snv * nc + c;
+    for (int v = 0; v < NHRSC; ++v) cons_soa[v * n + s] = S[v * nc];
+    const double* bx = Bx + q * G.nface[0]; const double* by = By + q * G.nface[1]; const double* bz = Bz + q * G.nface[2];
+    const int64_t fx = fidx<0>(G, i, j, k), fy = fidx<1>(G, i, j, k), fz = fidx<2>(G, i, j, k);
+    cons_soa[BSXL * n + s] = 0.5 * (bx[fx] + bx[fx + 1]);
+    cons_soa[BSYL * n + s] = 0.5 * (by[fy] + by[fy + G.N[0]]);
+    cons_soa[BSZL * n + s] = 0.5 * (bz[fz] + bz[fz + (int64_t)G.N[0] * G.N[1]]);
+    const int comps[10] = {GXX_, GXY_, GXZ_, GYY_, GYZ_, GZZ_, ALP_, BETAX_, BETAY_, BETAZ_};
+    const double flatv[10] = {1, 0, 0, 1, 0, 1, 1, 0, 0, 0};
+    for (int a = 0; a < 10; ++a) met[a * n + s] = flat ? flatv[a] : S[comps[a] * nc];
+}
+
+extern "C" grmhd_status grmhd_mlp_fit_cells(grmhd_ctx* c, grmhd_mlp* mlp, const double* state,
+                                            const double* const B[3], int64_t max_states,
+                                            const grmhd_mlp_fit_opts* opts, grmhd_mlp_fit_report* report,
+                                            grmhd_stream_t stream) {
+    if (!c || !mlp || !state || !B || !B[0] || !B[1] || !B[2] || max_states < 16) {
+        set_err("invalid mlp_fit_cells arguments"); return GRMHD_ERR_INVALID;
+    }
+    const DevGrid& G = c->G;
+    const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2] * G.nq;
+    const int64_t stride = std::max<int64_t>(1, (total + max_states - 1) / max_states);
+    const int64_t n = (total + stride - 1) / stride;
+    hipStream_t s = (hipStream_t)stream;
+    double* buf = nullptr;
+    hipError_t e = hipMalloc(&buf, sizeof(double) * 20 * n);
+    if (e != hipSuccess) { set_err("mlp_fit_cells: %s", hipGetErrorString(e)); return GRMHD_ERR_ALLOC; }
+    k_c2p_inputs<<<nblk(n), 256, 0, s>>>(G, state, NSTATE, B[0], B[1], B[2], c->P.flat_metric, n, stride, buf,
+                                          buf + 10 * n);
+    grmhd_status st = launch_check("c2p_inputs");
+    if (st == GRMHD_OK) st = grmhd_mlp_fit(c, mlp, n, buf, buf + 10 * n, nullptr, opts, report, stream);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    return st;
+}
+
+extern "C" grmhd_status grmhd_set_learned_c2p(grmhd_ctx* c, grmhd_mlp* mlp) {
+    if (!c) { set_err("null context"); return GRMHD_ERR_INVALID; }
+    if (c->mlp != mlp) { c->mlp = mlp; drop_graph(c); }
+    return GRMHD_OK;
 }
 
 template <typename EOS>

@@ -126,6 +126,22 @@ grmhd_status grmhd_mlp_fit(grmhd_ctx* ctx, grmhd_mlp* mlp, int64_t n, const doub
                            const double* metric_soa, const double* r_or_null, const grmhd_mlp_fit_opts* opts,
                            grmhd_mlp_fit_report* report, grmhd_stream_t stream);
 
+/* Trains on the c2p inputs of up to max_states interior cells of the
+ * context's block list (evenly spaced, state/B as for grmhd_rk3_step), as
+ * grmhd_mlp_fit does on a batch.  Allocates and frees a temporary device
+ * buffer of 20 doubles per state; synchronises the stream. */
+grmhd_status grmhd_mlp_fit_cells(grmhd_ctx* ctx, grmhd_mlp* mlp, const double* state, const double* const B[3],
+                                 int64_t max_states, const grmhd_mlp_fit_opts* opts,
+                                 grmhd_mlp_fit_report* report, grmhd_stream_t stream);
+
+/* Makes the c2p of the cell passes (grmhd_advance_substep,
+ * grmhd_compute_auxiliaries, grmhd_rk3_step[_graph]) start from mlp's
+ * prediction: the Newton finish for the ideal gas, brackets otherwise, the
+ * plain solve where they fail (as grmhd_c2p_batch_learned).  mlp = NULL
+ * restores the plain solve.  The context does not own mlp; a captured RK3
+ * graph is re-captured. */
+grmhd_status grmhd_set_learned_c2p(grmhd_ctx* ctx, grmhd_mlp* mlp);
+
 /* grmhd_c2p_batch with the learned guess: features, MLP and the FP64
  * Kastaun solves (Newton finish or brackets, grmhd_mlp_set_newton) in one
  * fused kernel.  Same outputs and statistics as grmhd_c2p_batch (no dry
