@@ -284,12 +284,14 @@ def torus_leg(ga, W, torch, dev, nb=12, steps=5, warmup=2):
     mlp = ga.Mlp()
     out = {"workload": "Fishbone-Moncrief torus a=0.9375 r_in=6 r_max=12 beta=100, Kerr-Schild (Cowling), "
                        "192^3 on [-48,48]^3 M, 1728 blocks of 16^3, ngz 3, WENO-Z + HLLE + UCT, rho_atm 1e-10",
+           "static_metric_cache": True,
            "zones": wl.zones, "steps": steps}
     for mode in ("plain", "learned"):
         ctx = ga.Context(wl.params, wl.grid, wl.instances)
         st = torch.from_numpy(wl.state).to(dev)
         B = [torch.from_numpy(b).to(dev) for b in wl.B]
         aux = torch.from_numpy(wl.aux).to(dev)
+        ctx.set_static_metric(st)   # Cowling: metric derivatives taken once, outside the timed steps
         if mode == "learned":
             t0 = time.perf_counter()
             rep = mlp.fit_cells(ctx, st, B, max_states=1 << 20)

@@ -77,7 +77,7 @@ ABI_SYMBOLS = [
     "grmhd_mlp_init_params", "grmhd_mlp_create", "grmhd_mlp_destroy", "grmhd_mlp_set_params",
     "grmhd_mlp_get_params", "grmhd_mlp_set_scaling", "grmhd_mlp_get_scaling", "grmhd_mlp_set_bracket",
     "grmhd_mlp_get_bracket", "grmhd_mlp_set_newton", "grmhd_mlp_get_newton",
-    "grmhd_mlp_fit_cells", "grmhd_set_learned_c2p", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
+    "grmhd_mlp_fit_cells", "grmhd_set_learned_c2p", "grmhd_set_static_metric", "grmhd_mlp_features", "grmhd_mlp_targets", "grmhd_mlp_forward", "grmhd_mlp_grad",
     "grmhd_mlp_train_step", "grmhd_mlp_default_fit_opts", "grmhd_mlp_fit", "grmhd_c2p_batch_learned",
 ]
 
@@ -170,6 +170,7 @@ def lib():
     L.grmhd_c2p_batch_learned.argtypes = [P, P, i64, P, P, P, P, P, P, P, P]
     L.grmhd_mlp_fit_cells.argtypes = [P, P, P, P * 3, i64, C.POINTER(MlpFitOpts), C.POINTER(MlpFitReport), P]
     L.grmhd_set_learned_c2p.argtypes = [P, P]
+    L.grmhd_set_static_metric.argtypes = [P, P, i32, P]
     for name in ABI_SYMBOLS:
         getattr(L, name)  # raises AttributeError if an exported symbol is missing
     _lib = L
@@ -280,6 +281,10 @@ class Context:
     def rk3_step(self, dt_dev, state, B, aux, stream=None, graph=False):
         fn = self.L.grmhd_rk3_step_graph if graph else self.L.grmhd_rk3_step
         check(fn(self.h, _ptr(dt_dev), _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)), "rk3_step")
+
+    def set_static_metric(self, state, enable=True, stream=None):
+        check(self.L.grmhd_set_static_metric(self.h, _ptr(state) if enable else None, int(bool(enable)),
+                                             _ptr(stream)), "set_static_metric")
 
     def set_learned_c2p(self, mlp):
         """Cell-pass c2p from the MLP's guess (mlp: Mlp or None for plain)."""

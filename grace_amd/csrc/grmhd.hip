@@ -482,6 +482,7 @@ struct UpdArgs {
     double* nB[3]; const double* aB[3]; const double* bB[3];
     const double* dt; double dtfact;
     const DevInst* inst; const DevBlk* blk;
+    const double* mder;              // cached metric derivatives (30 per interior cell) or null
 };
 
 GHD double fdder(const double* P, int64_t s, double ih, int order) {
@@ -536,6 +537,39 @@ GHD void geom_sources(const Met& m, const double K[6], const double da[3], const
     }
 }
 
+// Derivatives of the Cowling metric at a cell (fill_deriv_* of
+// grmhd.hh:232-235, with the reference's idx(0,q) in every direction):
+// da[d], db[c + 3d], dg[b + 6d].
+GHD void metric_ders(const double* M, int64_t nc, const int64_t sd[3], double ix, int fd_order, double da[3],
+                     double db[9], double dg[18]) {
+    for (int d = 0; d < 3; ++d) da[d] = fdder(M + ALP_ * nc, sd[d], ix, fd_order);
+    for (int d = 0; d < 3; ++d)
+        for (int cc = 0; cc < 3; ++cc) db[cc + 3 * d] = fdder(M + (BETAX_ + cc) * nc, sd[d], ix, fd_order);
+    for (int d = 0; d < 3; ++d)
+        for (int b = 0; b < 6; ++b) dg[b + 6 * d] = fdder(M + (GXX_ + b) * nc, sd[d], ix, fd_order);
+}
+
+// The 30 metric derivatives of every interior cell, [q][30][k][j][i].
+__global__ __launch_bounds__(256) void k_metric_der(DevGrid G, const DevInst* __restrict__ inst,
+                                                    const DevBlk* __restrict__ blk, const double* __restrict__ ms,
+                                                    int mnv, int fd_order, double* __restrict__ out) {
+    const int64_t per = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * G.nq) return;
+    const int64_t q = t / per, e = t - q * per;
+    const int i = G.ng + (int)(e % G.n[0]), j = G.ng + (int)((e / G.n[0]) % G.n[1]);
+    const int k = G.ng + (int)(e / ((int64_t)G.n[0] * G.n[1]));
+    const int64_t nc = G.ncell;
+    const double* M = ms + q * (int64_t)mnv * nc + cidx(G, i, j, k);
+    const int64_t sd[3] = {1, G.N[0], (int64_t)G.N[0] * G.N[1]};
+    double da[3], db[9], dg[18];
+    metric_ders(M, nc, sd, inst[blk[q].inst].idx[0], fd_order, da, db, dg);
+    double* D = out + q * 30 * per + e;
+    for (int d = 0; d < 3; ++d) D[d * per] = da[d];
+    for (int b = 0; b < 9; ++b) D[(3 + b) * per] = db[b];
+    for (int b = 0; b < 18; ++b) D[(12 + b) * per] = dg[b];
+}
+
 template <bool FLAT>
 __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order, int wpb) {
     // wpb workgroups per block over its (n+1)^3 sites (see wg_pos)
@@ -571,12 +605,16 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
             double B[3] = {X[BX_ * nc], X[BY_ * nc], X[BZ_ * nc]};
             double W = lorentz(m.g, z);
             double da[3], dg[18], db[9], K[6];
-            const int64_t sd[3] = {1, G.N[0], (int64_t)G.N[0] * G.N[1]};
-            for (int d = 0; d < 3; ++d) da[d] = fdder(M + ALP_ * nc, sd[d], ix, fd_order);
-            for (int d = 0; d < 3; ++d)
-                for (int cc = 0; cc < 3; ++cc) db[cc + 3 * d] = fdder(M + (BETAX_ + cc) * nc, sd[d], ix, fd_order);
-            for (int d = 0; d < 3; ++d)
-                for (int b = 0; b < 6; ++b) dg[b + 6 * d] = fdder(M + (GXX_ + b) * nc, sd[d], ix, fd_order);
+            if (a.mder) {   // the static metric's derivatives, computed once by k_metric_der
+                const int64_t per = (int64_t)G.n[0] * G.n[1] * G.n[2];
+                const double* D = a.mder + q * 30 * per + ((k - G.ng) * (int64_t)G.n[1] + (j - G.ng)) * G.n[0] + (i - G.ng);
+                for (int d = 0; d < 3; ++d) da[d] = D[d * per];
+                for (int b = 0; b < 9; ++b) db[b] = D[(3 + b) * per];
+                for (int b = 0; b < 18; ++b) dg[b] = D[(12 + b) * per];
+            } else {
+                const int64_t sd[3] = {1, G.N[0], (int64_t)G.N[0] * G.N[1]};
+                metric_ders(M, nc, sd, ix, fd_order, da, db, dg);
+            }
             for (int b = 0; b < 6; ++b) K[b] = M[(KXX_ + b) * nc];
             double stau, ss[3];
             geom_sources(m, K, da, dg, db, X[RHO_ * nc], X[PRESS_ * nc], X[EPS_ * nc], B, z, W, stau, ss);
@@ -1229,6 +1267,8 @@ struct TimerRec { std::string fam; hipEvent_t a, b; };
 
 struct grmhd_ctx {
     grmhd_mlp* mlp = nullptr;                 // learned c2p guess for the cell passes (not owned)
+    double* mder = nullptr;                   // cached derivatives of the static metric (grmhd_set_static_metric)
+    const double* mder_src = nullptr;         // the state array they were taken from
     grmhd_params P;
     DevGrid G;
     Opts O;
@@ -1338,7 +1378,7 @@ static void build_eos(grmhd_ctx* c) {
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
-    F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag);
+    F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1608,6 +1648,7 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     a.inst = c->inst_d; a.blk = c->blk_d;
     for (int d = 0; d < 3; ++d) a.flux[d] = c->flux[d];
     a.emf = c->emf;
+    a.mder = (c->mder && a.mstate == c->mder_src && a.mnv == NSTATE) ? c->mder : nullptr;
     site_launch(G, c->tune.update, [&](dim3 g, int wpb) {
         if (c->P.flat_metric) k_update<true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
         else k_update<false><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
@@ -1879,6 +1920,24 @@ extern "C" grmhd_status grmhd_mlp_fit_cells(grmhd_ctx* c, grmhd_mlp* mlp, const 
     (void)hipStreamSynchronize(s);
     (void)hipFree(buf);
     return st;
+}
+
+extern "C" grmhd_status grmhd_set_static_metric(grmhd_ctx* c, const double* state, int32_t enable,
+                                                grmhd_stream_t stream) {
+    if (!c || (enable && !state)) { set_err("invalid set_static_metric arguments"); return GRMHD_ERR_INVALID; }
+    drop_graph(c);
+    if (!enable) { c->mder_src = nullptr; return GRMHD_OK; }
+    if (c->P.flat_metric) { c->mder_src = nullptr; return GRMHD_OK; }   // flat kernels take no derivatives
+    const DevGrid& G = c->G;
+    const int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2] * G.nq;
+    if (!c->mder) {
+        hipError_t e = hipMalloc(&c->mder, sizeof(double) * 30 * n);
+        if (e != hipSuccess) { c->mder = nullptr; set_err("static metric cache: %s", hipGetErrorString(e)); return GRMHD_ERR_ALLOC; }
+    }
+    k_metric_der<<<nblk(n), 256, 0, (hipStream_t)stream>>>(G, c->inst_d, c->blk_d, state, NSTATE, c->P.fd_order,
+                                                            c->mder);
+    c->mder_src = state;
+    return launch_check("metric_der");
 }
 
 extern "C" grmhd_status grmhd_set_learned_c2p(grmhd_ctx* c, grmhd_mlp* mlp) {

@@ -181,6 +181,17 @@ grmhd_status grmhd_rk3_step(grmhd_ctx* ctx, const double* dt_dev, double* state,
 grmhd_status grmhd_rk3_step_graph(grmhd_ctx* ctx, const double* dt_dev, double* state,
                                   double* const B[3], double* aux, grmhd_stream_t stream);
 
+/* Cowling: the metric in `state` is static, so the 30 finite-difference
+ * metric derivatives of the geometric source terms (grmhd.hh:232-235,
+ * recomputed by the reference in every substep) can be taken once:
+ * enable = 1 computes them for the interior cells of `state` into a context
+ * buffer (30 doubles per interior cell), and the update kernels of calls
+ * whose metric source is that same array then read them instead (results
+ * bit-identical).  The cache is not refreshed automatically: call again
+ * after changing the metric in that array, or enable = 0 to drop it.
+ * No effect for flat_metric.  Invalidates a captured RK3 graph. */
+grmhd_status grmhd_set_static_metric(grmhd_ctx* ctx, const double* state, int32_t enable, grmhd_stream_t stream);
+
 /* Workgroup sizes (threads) of the kernel families, chosen by
  * grmhd_autotune; every entry is one of 64, 128, 256. */
 typedef struct {
