@@ -249,13 +249,14 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
         cmax = fmax(0., fmax(cpl, cpr));
     }
     if (cmin < 1e-12 && cmax < 1e-12) { cmin = 1; cmax = 1; }
+    const double ihll = 1.0 / (cmax + cmin);   // shared by the HLL fluxes of this face
     const int64_t fo = fidx<D>(G, i, j, k);
     const int64_t nf = G.nface[D];
     double* V = vbar + q * 4 * nf + fo;
     constexpr int J0 = D == 0 ? 1 : 0, J1 = D == 2 ? 1 : 2;
     if (o.emf == 0) {  // UCT: HLL-averaged transverse vtilde + wave speeds, slots strided by nf
-        V[0] = hll(vtl[J0], vtr[J0], 0, 0, cmin, cmax);
-        V[nf] = hll(vtl[J1], vtr[J1], 0, 0, cmin, cmax);
+        V[0] = hll_r(vtl[J0], vtr[J0], 0, 0, cmin, cmax, ihll);
+        V[nf] = hll_r(vtl[J1], vtr[J1], 0, 0, cmin, cmax, ihll);
         V[2 * nf] = cmin; V[3 * nf] = cmax;
         if (ring) return;
     }
@@ -298,29 +299,29 @@ __global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const 
     }
     double* F = flux + q * NHRSC * nf + fo;
     const double sg = mf.sqrtg;
-    F[DENS_ * nf] = sg * hll(Fl[0], Fr[0], Dl, Dr, cmin, cmax);
-    F[ENTROPYSTAR_ * nf] = sg * hll(Fl[6], Fr[6], Esl, Esr, cmin, cmax);
-    F[TAU_ * nf] = sg * hll(Fl[4], Fr[4], taul, taur, cmin, cmax);
-    F[SX_ * nf] = sg * hll(Fl[1], Fr[1], Sl[0], Sr[0], cmin, cmax);
-    F[SY_ * nf] = sg * hll(Fl[2], Fr[2], Sl[1], Sr[1], cmin, cmax);
-    F[SZ_ * nf] = sg * hll(Fl[3], Fr[3], Sl[2], Sr[2], cmin, cmax);
-    F[YESTAR_ * nf] = sg * hll(Fl[5], Fr[5], pL[YEL] * Dl, pR[YEL] * Dr, cmin, cmax);
+    F[DENS_ * nf] = sg * hll_r(Fl[0], Fr[0], Dl, Dr, cmin, cmax, ihll);
+    F[ENTROPYSTAR_ * nf] = sg * hll_r(Fl[6], Fr[6], Esl, Esr, cmin, cmax, ihll);
+    F[TAU_ * nf] = sg * hll_r(Fl[4], Fr[4], taul, taur, cmin, cmax, ihll);
+    F[SX_ * nf] = sg * hll_r(Fl[1], Fr[1], Sl[0], Sr[0], cmin, cmax, ihll);
+    F[SY_ * nf] = sg * hll_r(Fl[2], Fr[2], Sl[1], Sr[1], cmin, cmax, ihll);
+    F[SZ_ * nf] = sg * hll_r(Fl[3], Fr[3], Sl[2], Sr[2], cmin, cmax, ihll);
+    F[YESTAR_ * nf] = sg * hll_r(Fl[5], Fr[5], pL[YEL] * Dl, pR[YEL] * Dr, cmin, cmax, ihll);
     if (o.emf == 1) {  // GS face EMFs (grmhd.hh:937-969)
         if (D == 0) {
             double a = -Bl[0] * vtl[2] + Bl[2] * vtl[0], b = -Br[0] * vtr[2] + Br[2] * vtr[0];
-            V[0] = sg * hll(a, b, Bl[2], Br[2], cmin, cmax);
+            V[0] = sg * hll_r(a, b, Bl[2], Br[2], cmin, cmax, ihll);
             a = -Bl[0] * vtl[1] + Bl[1] * vtl[0]; b = -Br[0] * vtr[1] + Br[1] * vtr[0];
-            V[nf] = -sg * hll(a, b, Bl[1], Br[1], cmin, cmax);
+            V[nf] = -sg * hll_r(a, b, Bl[1], Br[1], cmin, cmax, ihll);
         } else if (D == 1) {
             double a = -Bl[1] * vtl[2] + Bl[2] * vtl[1], b = -Br[1] * vtr[2] + Br[2] * vtr[1];
-            V[0] = -sg * hll(a, b, Bl[2], Br[2], cmin, cmax);
+            V[0] = -sg * hll_r(a, b, Bl[2], Br[2], cmin, cmax, ihll);
             a = Bl[0] * vtl[1] - Bl[1] * vtl[0]; b = Br[0] * vtr[1] - Br[1] * vtr[0];
-            V[nf] = sg * hll(a, b, Bl[0], Br[0], cmin, cmax);
+            V[nf] = sg * hll_r(a, b, Bl[0], Br[0], cmin, cmax, ihll);
         } else {
             double a = Bl[1] * vtl[2] - Bl[2] * vtl[1], b = Br[1] * vtr[2] - Br[2] * vtr[1];
-            V[0] = sg * hll(a, b, Bl[1], Br[1], cmin, cmax);
+            V[0] = sg * hll_r(a, b, Bl[1], Br[1], cmin, cmax, ihll);
             a = Bl[0] * vtl[2] - Bl[2] * vtl[0]; b = Br[0] * vtr[2] - Br[2] * vtr[0];
-            V[nf] = -sg * hll(a, b, Bl[0], Br[0], cmin, cmax);
+            V[nf] = -sg * hll_r(a, b, Bl[0], Br[0], cmin, cmax, ihll);
         }
     }
 }

@@ -944,37 +944,49 @@ GHD C2POut c2p(double cons[NCONS], double pr[NPRIM], const Met& m, const EOS& eo
 
 // ---- reconstruction ----------------------------------------------------------
 // u0..u5 = u(i-3..i+2); uL at i-1/2-eps, uR at i-1/2+eps
-// WENO-Z (weno_reconstruction.hh:172-278).  The three ratios t5/(b_k+eps)
-// of each side come from one division, t5/(d0 d1 d2) times the other two
-// d's (d_k >= 1e-42, so the product cannot underflow); this differs from the
-// reference's three divisions by a few ulp.
-GHD void wenoz_side(double b0, double b1, double b2, double p0, double p1, double p2, double w0, double w1,
-                    double w2, double& u) {
+// WENO-Z (weno_reconstruction.hh:172-278).  Per side, the three ratios
+// t5/(b_k+eps) come from one reciprocal of d0 d1 d2 (d_k >= 1e-42, so the
+// product cannot underflow), and the two sides of a face share their
+// reciprocals: 1/(P_R P_L) gives 1/P_R and 1/P_L, 1/(S_R S_L) the two
+// normalisations (guarded against overflow of the products, which falls back
+// to separate divisions).  This differs from the reference's three divisions
+// per weight by a few ulp.
+struct WenoSide { double q0, q1, q2; double P; };
+GHD void weno_betas(double b0, double b1, double b2, double& d0, double& d1, double& d2, double& t5) {
     const double eps = 1.0e-42;
-    const double d0 = b0 + eps, d1 = b1 + eps, d2 = b2 + eps;
-    const double d01 = d0 * d1;
-    const double r = fabs(b0 - b2) / (d01 * d2);
-    const double q0 = d1 * d2 * r, q1 = d0 * d2 * r, q2 = d01 * r;
-    const double a0 = w0 * (1.0 + q0 * q0), a1 = w1 * (1.0 + q1 * q1), a2 = w2 * (1.0 + q2 * q2);
-    u = (p0 * a0 + p1 * a1 + p2 * a2) / (6.0 * (a0 + a1 + a2));
+    d0 = b0 + eps; d1 = b1 + eps; d2 = b2 + eps;
+    t5 = fabs(b0 - b2);
 }
 GHD void wenoz(double um3, double um2, double um1, double u0, double up1, double up2, double& uL, double& uR) {
     const double c0 = 13.0 / 12.0, c1 = 0.25;
-    {
-        double b0 = c0 * sq(um2 + u0 - 2.0 * um1) + c1 * sq(um2 + 3.0 * u0 - 4.0 * um1);
-        double b1 = c0 * sq(um1 + up1 - 2.0 * u0) + c1 * sq(um1 - up1);
-        double b2 = c0 * sq(up2 + u0 - 2.0 * up1) + c1 * sq(up2 + 3.0 * u0 - 4.0 * up1);
-        // a0 pairs with i2, a2 with i0 on this side
-        wenoz_side(b2, b1, b0, (2.0 * up2 - 7.0 * up1 + 11.0 * u0), (-1.0 * up1 + 5.0 * u0 + 2.0 * um1),
-                   (2.0 * u0 + 5.0 * um1 - um2), 0.1, 0.6, 0.3, uR);
-    }
-    {
-        double b0 = c0 * sq(um3 + um1 - 2.0 * um2) + c1 * sq(um3 + 3.0 * um1 - 4.0 * um2);
-        double b1 = c0 * sq(um2 + u0 - 2.0 * um1) + c1 * sq(um2 - u0);
-        double b2 = c0 * sq(up1 + um1 - 2.0 * u0) + c1 * sq(up1 + 3.0 * um1 - 4.0 * u0);
-        wenoz_side(b0, b1, b2, (2.0 * um3 - 7.0 * um2 + 11.0 * um1), (-1.0 * um2 + 5.0 * um1 + 2.0 * u0),
-                   (2.0 * um1 + 5.0 * u0 - up1), 0.1, 0.6, 0.3, uL);
-    }
+    // right state (cell i stencils; a0 pairs with i2, a2 with i0 on this side)
+    const double rb0 = c0 * sq(um2 + u0 - 2.0 * um1) + c1 * sq(um2 + 3.0 * u0 - 4.0 * um1);
+    const double rb1 = c0 * sq(um1 + up1 - 2.0 * u0) + c1 * sq(um1 - up1);
+    const double rb2 = c0 * sq(up2 + u0 - 2.0 * up1) + c1 * sq(up2 + 3.0 * u0 - 4.0 * up1);
+    // left state (cell i-1 stencils)
+    const double lb0 = c0 * sq(um3 + um1 - 2.0 * um2) + c1 * sq(um3 + 3.0 * um1 - 4.0 * um2);
+    const double lb1 = c0 * sq(um2 + u0 - 2.0 * um1) + c1 * sq(um2 - u0);
+    const double lb2 = c0 * sq(up1 + um1 - 2.0 * u0) + c1 * sq(up1 + 3.0 * um1 - 4.0 * u0);
+    double rd0, rd1, rd2, rt, ld0, ld1, ld2, lt;
+    weno_betas(rb2, rb1, rb0, rd0, rd1, rd2, rt);
+    weno_betas(lb0, lb1, lb2, ld0, ld1, ld2, lt);
+    const double rP01 = rd0 * rd1, rP = rP01 * rd2, lP01 = ld0 * ld1, lP = lP01 * ld2;
+    double rr, lr;
+    const double PP = rP * lP;
+    if (PP < 1e300) { const double inv = 1.0 / PP; rr = rt * lP * inv; lr = lt * rP * inv; }
+    else { rr = rt / rP; lr = lt / lP; }
+    const double rq0 = rd1 * rd2 * rr, rq1 = rd0 * rd2 * rr, rq2 = rP01 * rr;
+    const double lq0 = ld1 * ld2 * lr, lq1 = ld0 * ld2 * lr, lq2 = lP01 * lr;
+    const double ra0 = 0.1 * (1.0 + rq0 * rq0), ra1 = 0.6 * (1.0 + rq1 * rq1), ra2 = 0.3 * (1.0 + rq2 * rq2);
+    const double la0 = 0.1 * (1.0 + lq0 * lq0), la1 = 0.6 * (1.0 + lq1 * lq1), la2 = 0.3 * (1.0 + lq2 * lq2);
+    const double rn = (2.0 * up2 - 7.0 * up1 + 11.0 * u0) * ra0 + (-1.0 * up1 + 5.0 * u0 + 2.0 * um1) * ra1 +
+                      (2.0 * u0 + 5.0 * um1 - um2) * ra2;
+    const double ln = (2.0 * um3 - 7.0 * um2 + 11.0 * um1) * la0 + (-1.0 * um2 + 5.0 * um1 + 2.0 * u0) * la1 +
+                      (2.0 * um1 + 5.0 * u0 - up1) * la2;
+    const double rS = 6.0 * (ra0 + ra1 + ra2), lS = 6.0 * (la0 + la1 + la2);
+    const double SS = rS * lS;
+    if (SS < 1e300) { const double inv = 1.0 / SS; uR = rn * lS * inv; uL = ln * rS * inv; }
+    else { uR = rn / rS; uL = ln / lS; }
 }
 GHD double mc2(double sL, double sR) {
     double sC = 0.5 * (sR + sL), sg = copysign(1.0, sR);
@@ -998,6 +1010,10 @@ GHD void recon(double um3, double um2, double um1, double u0, double up1, double
 }
 GHD double hll(double fL, double fR, double uL, double uR, double cmin, double cmax) {
     return (cmax * fL + cmin * fR - cmax * cmin * (uR - uL)) / (cmax + cmin);
+}
+// hll with the reciprocal of cmax + cmin shared by the fluxes of one face
+GHD double hll_r(double fL, double fR, double uL, double uR, double cmin, double cmax, double inv) {
+    return (cmax * fL + cmin * fR - cmax * cmin * (uR - uL)) * inv;
 }
 
 }  // namespace gm
