@@ -571,7 +571,9 @@ __global__ __launch_bounds__(256) void k_metric_der(DevGrid G, const DevInst* __
     for (int b = 0; b < 18; ++b) D[(12 + b) * per] = dg[b];
 }
 
-template <bool FLAT>
+// MDER: the metric derivatives come from the static-metric cache a.mder
+// (the finite differences are not compiled in, which frees registers).
+template <bool FLAT, bool MDER = false>
 __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_order, int wpb) {
     // wpb workgroups per block over its (n+1)^3 sites (see wg_pos)
     const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
             double B[3] = {X[BX_ * nc], X[BY_ * nc], X[BZ_ * nc]};
             double W = lorentz(m.g, z);
             double da[3], dg[18], db[9], K[6];
-            if (a.mder) {   // the static metric's derivatives, computed once by k_metric_der
+            if constexpr (MDER) {   // the static metric's derivatives, computed once by k_metric_der
                 const int64_t per = (int64_t)G.n[0] * G.n[1] * G.n[2];
                 const double* D = a.mder + q * 30 * per + ((k - G.ng) * (int64_t)G.n[1] + (j - G.ng)) * G.n[0] + (i - G.ng);
                 for (int d = 0; d < 3; ++d) da[d] = D[d * per];
@@ -1652,6 +1654,7 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     a.mder = (c->mder && a.mstate == c->mder_src && a.mnv == NSTATE) ? c->mder : nullptr;
     site_launch(G, c->tune.update, [&](dim3 g, int wpb) {
         if (c->P.flat_metric) k_update<true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
+        else if (a.mder) k_update<false, true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
         else k_update<false><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
     });
 }
