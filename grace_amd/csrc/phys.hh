@@ -641,15 +641,17 @@ GHD bool guess_bracket(F& f, double lo, double hi, double umax, int nexp, double
 // narrows the bracket [lo, hi] (initially (0, umax]), and a step that leaves
 // it (or a non-finite step) is replaced by bisection.  It stops after a
 // Newton step dx after which the new iterate's error |f''/(2 f')| dx^2
-// (Taylor) is within NEWTON_ACCEPT of it: either |dx| <= NEWTON_TINY x, or
+// (Taylor) is within acc of it: either |dx| <= NEWTON_TINY x, or
 // |dx| <= NEWTON_RTOL x with f'' estimated from the slopes of the last two
 // iterates (both reached by Newton steps).  Otherwise it iterates on, up to
-// maxit evaluations.
+// maxit evaluations.  The callers pass acc = NEWTON_ACCEPT / (1 + r^2/h0^2):
+// a relative error e of mu moves W by up to W^2 e, and W^2 <= 1 + r^2/h0^2
+// on the master function's domain.
 constexpr double NEWTON_RTOL = 1e-7;
 constexpr double NEWTON_TINY = 1e-10;
 constexpr double NEWTON_ACCEPT = 1e-13;
 template <typename F>
-GHD bool newton_guess(F& f, double x, double umax, int maxit, double& root) {
+GHD bool newton_guess(F& f, double x, double umax, int maxit, double acc, double& root) {
     double lo = 0.0, hi = umax;
     if (!(x > lo && x < hi)) x = 0.5 * (lo + hi);
     double xp = 0.0, dfp = 0.0;
@@ -666,7 +668,7 @@ GHD bool newton_guess(F& f, double x, double umax, int maxit, double& root) {
         if (newton) {
             if (fabs(dx) <= NEWTON_TINY * xn) { root = xn; return true; }
             if (prev_newton && fabs(dx) <= NEWTON_RTOL * xn &&
-                fabs(0.5 * (df - dfp) / ((x - xp) * df)) * dx * dx <= NEWTON_ACCEPT * xn) {
+                fabs(0.5 * (df - dfp) / ((x - xp) * df)) * dx * dx <= acc * xn) {
                 root = xn;
                 return true;
             }
@@ -685,7 +687,8 @@ GHD double kastaun(const EOS& eos, const C2PIn& c, double pr[NPRIM], unsigned& s
     double mu = -1.0;
     if constexpr (EOS::is_ideal) {
         double r;
-        if (gs.newton > 0 && gs.e_c > 0.0 && newton_guess(f, gs.e_c / h0, 1.0 / h0, gs.newton, r)) {
+        if (gs.newton > 0 && gs.e_c > 0.0 &&
+            newton_guess(f, gs.e_c / h0, 1.0 / h0, gs.newton, NEWTON_ACCEPT / (1.0 + c.r2 / (h0 * h0)), r)) {
             mu = r; cnt.guess_used |= 1;
         }
     }
@@ -726,7 +729,8 @@ GHD double kastaun_ent(const EOS& eos, const C2PIn& c, double pr[NPRIM], unsigne
     double mu = -1.0;
     if constexpr (EOS::is_ideal) {
         double r;
-        if (gs.newton > 0 && gs.s_c > 0.0 && newton_guess(f, gs.s_c / h0, 1.0 / h0, gs.newton, r)) {
+        if (gs.newton > 0 && gs.s_c > 0.0 &&
+            newton_guess(f, gs.s_c / h0, 1.0 / h0, gs.newton, NEWTON_ACCEPT / (1.0 + c.r2 / (h0 * h0)), r)) {
             mu = r; cnt.guess_used |= 2;
         }
     }

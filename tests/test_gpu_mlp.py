@@ -236,3 +236,42 @@ def test_gpu_learned_c2p_in_rk3_torus(dev):
         assert linf_rel(face_interior(wl.grid, Bl[d], d), face_interior(wl.grid, Bp[d], d)) < 1e-9, d
     assert np.array_equal(al[:, ga.C2P_ERR], ap[:, ga.C2P_ERR])
     assert stl["n_guess_used"] > 0.5 * stl["n_cells"]
+
+
+@pytest.mark.parametrize("case", ["sigma", "cold", "dense", "near_hole"])
+def test_gpu_learned_c2p_out_of_distribution(dev, case):
+    """A network trained on the C4 distribution, run on states outside it
+    (magnetisation to 1e4, eps down to 1e-10, rho to 1e2, r down to 1.1 M):
+    the Newton finish either converges to the root or hands the state to
+    the plain solve, so the error bits match and the primitives stay within
+    1e-10 of the plain FP64 solve."""
+    import torch
+    from bench import state_residual
+    kw = {"sigma": dict(sigma=(1e-2, 1e4)), "cold": dict(eps=(1e-10, 1e-4)), "dense": dict(rho=(1e-4, 1e2)),
+          "near_hole": dict(r=(1.1, 2.0))}[case]
+    ntr, n = 1 << 17, 1 << 15
+    tr = W.c2p_states(ntr, seed=W.SEED + 7)
+    ctx = single_ctx(tr["params"])
+    mlp = ga.Mlp()
+    mlp.fit(ctx, ntr, to_dev(tr["cons"], dev), to_dev(tr["met"], dev), to_dev(tr["rad"], dev),
+            opts=ga.Mlp.default_fit_opts(epochs=40))
+    te = W.c2p_states(n, seed=W.SEED + 77, **kw)
+    say(f"[learned c2p, out of distribution] {case} {kw}")
+    res = {}
+    for mode in ("plain", "learned"):
+        cons = to_dev(te["cons"], dev)
+        prims = torch.zeros((13, n), dtype=torch.float64, device=dev)
+        err = torch.zeros(n, dtype=torch.int64, device=dev)
+        fl = torch.zeros(n, dtype=torch.uint8, device=dev)
+        if mode == "plain":
+            ctx.c2p_batch(n, cons, to_dev(te["met"], dev), prims, err, fl, r=to_dev(te["rad"], dev))
+        else:
+            ga.c2p_batch_learned(ctx, mlp, n, cons, to_dev(te["met"], dev), prims, err, fl, r=to_dev(te["rad"], dev))
+        torch.cuda.synchronize()
+        res[mode] = (prims, err, ctx.c2p_stats())
+    same = res["plain"][1] == res["learned"][1]
+    assert bool(same.all())
+    r = state_residual(res["learned"][0], res["plain"][0])
+    say(f"  max state-normalised difference {r.max().item():.3e}, stats {res['learned'][2]}")
+    assert bool(torch.isfinite(res["learned"][0]).all())
+    assert r.max().item() < 1e-10
