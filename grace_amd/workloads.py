@@ -89,7 +89,8 @@ def blast_ks(nb=(2, 2, 2), n=16, ngz=3, spin=0.9, center=(8.0, 0.0, 0.0), size=4
 
 
 def fm_torus(nb=(4, 4, 4), n=16, ngz=3, extent=48.0, spin=0.9375, r_in=6.0, r_max=12.0, beta=100.0,
-             rho_atm=1e-10, p_atm=1e-14, gamma=4.0 / 3.0, noise=0.01, seed=SEED, emf=EMF_UCT, excision_lapse=0.55):
+             rho_atm=1e-10, p_atm=1e-14, gamma=4.0 / 3.0, noise=0.01, seed=SEED, emf=EMF_UCT, excision_lapse=0.55,
+             fofc=0):
     """Fishbone-Moncrief torus (r_in 6 M, pressure maximum 12 M, Gamma 4/3,
     single poloidal loop with max p / max p_mag = 100) around a spin-0.9375
     hole at the centre of the cube [-extent, extent]^3, Kerr-Schild metric
@@ -107,7 +108,7 @@ def fm_torus(nb=(4, 4, 4), n=16, ngz=3, extent=48.0, spin=0.9375, r_in=6.0, r_ma
     rc = lib().grmhd_setup_fm_torus(grid, inst, 1, gamma, _p(lo), spin, r_in, r_max, beta, rho_atm, p_atm, noise,
                                     seed, _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux), _p(info))
     assert rc == 0, rc
-    params = default_params(gamma=gamma, emf=emf, flat_metric=0)
+    params = default_params(gamma=gamma, emf=emf, flat_metric=0, fofc=fofc)
     params.rho_fl = rho_atm
     # atmosphere temperature p/rho: a zero floor would give the reset cells
     # eps = 0 and an entropy of -inf (ideal_gas_eos.hh:78-81)

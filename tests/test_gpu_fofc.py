@@ -60,3 +60,36 @@ def test_gpu_fofc_rejects_uct_and_few_ghosts():
     wl = W.blast(nb=(1, 1, 1), n=8, ngz=3, emf=ga.EMF_GS, fofc=1)
     with pytest.raises(RuntimeError, match="ngz"):
         ga.Context(wl.params, wl.grid, wl.instances)
+
+
+def test_gpu_fofc_torus_curved(dev):
+    """FOFC in the curved metric: the Fishbone-Moncrief torus around the
+    spin-0.9375 hole (GS, ngz 4), where the torus edge, the atmosphere and
+    the excised region flag many cells; one substep and one RK3 step
+    against the oracle."""
+    import torch
+    wl = W.fm_torus(nb=(2, 2, 2), n=16, ngz=4, extent=24.0, emf=ga.EMF_GS, fofc=1)
+    say("[fofc torus] FM torus 32^3, GS, ngz 4")
+    dt = 0.25 * wl.extra["dx"]
+    st, B, aux = wl.copy_arrays()
+    new_o, nB_o = st.copy(), [b.copy() for b in B]
+    stepper = O.Stepper(wl)
+    stepper.substep(dt, 1.0, new_o, st, nB_o, B, aux)
+    say(f"  oracle flagged {stepper.fofc_flagged()} cells")
+    assert stepper.fofc_flagged() > 1000
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    sg, Bg, ag = dev_arrays(wl, dev)
+    ng, nBg = sg.clone(), [b.clone() for b in Bg]
+    ctx.advance_substep(torch.tensor([dt], dtype=torch.float64, device=dev), 1.0, ng, sg, nBg, Bg, ag)
+    torch.cuda.synchronize()
+    gn = to_host(ng)
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
+        assert e < 1e-12, (v, e)
+    for d in range(3):
+        e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
+        assert e < 1e-12, (d, e)
+    ref, got, _ = run_rk3_pair(wl, dev, steps=1)
+    check_close(wl, ref, got, 1e-12)
+    bmax = max(np.abs(b).max() for b in got[1])
+    assert np.max(np.abs(interior(wl.grid, got[2][:, ga.BDIV]))) * wl.extra["dx"] < 1e-12 * bmax
