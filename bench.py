@@ -391,6 +391,7 @@ def main():
     ap.add_argument("--no-c2p", action="store_true", help="skip the learned-vs-plain c2p leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven-instance leg")
     ap.add_argument("--no-torus", action="store_true", help="skip the Kerr-Schild torus leg")
+    ap.add_argument("--no-tune", action="store_true", help="default workgroup sizes instead of grmhd_autotune")
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
@@ -430,7 +431,13 @@ def main():
         ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
         ctx.rk3_step(dt, st, B, aux, stream=sp, graph=use_graph)
 
+    tuning = None
     with torch.cuda.stream(stream):
+        if not args.no_tune:
+            # workgroup sizes per kernel family picked by timing (inputs are
+            # only read), before the warm-up and the timed steps
+            tuning = ctx.autotune(st, B, aux, reps=3, stream=sp)
+            stream.synchronize()
         log(f"[bench] warmup {args.warmup}")
         for _ in range(args.warmup):
             step()
@@ -510,7 +517,7 @@ def main():
         "data": "synthetic (seeded blast-wave initial data, seed 20261202)",
         "config": {"workload": "blast3d_128cube_512x16cube_blocks_ngz3_wenoz_hlle_uct_rk3_plain_c2p",
                    "zones": wl.zones, "blocks": int(wl.grid.nq), "c2p": "plain Kastaun (FP64)",
-                   "graph": use_graph, "gpu_event_ms_per_step": gpu_ms / args.steps},
+                   "graph": use_graph, "gpu_event_ms_per_step": gpu_ms / args.steps, "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
         "c2p_learned_vs_plain": c2p_leg,
