@@ -1270,6 +1270,7 @@ struct TimerRec { std::string fam; hipEvent_t a, b; };
 
 struct grmhd_ctx {
     grmhd_mlp* mlp = nullptr;                 // learned c2p guess for the cell passes (not owned)
+    MlpDev gmlp{};                            // its kernel arguments in the captured graph
     double* mder = nullptr;                   // cached derivatives of the static metric (grmhd_set_static_metric)
     const double* mder_src = nullptr;         // the state array they were taken from
     grmhd_params P;
@@ -2108,6 +2109,11 @@ extern "C" grmhd_status grmhd_rk3_step(grmhd_ctx* c, const double* dt_dev, doubl
     return launch_check("rk3_step");
 }
 
+static bool same_mlp(const MlpDev& a, const MlpDev& b) {
+    return a.frag == b.frag && a.bias == b.bias && a.norm == b.norm && a.fe_lo == b.fe_lo && a.fe_hi == b.fe_hi &&
+           a.fs_lo == b.fs_lo && a.fs_hi == b.fs_hi && a.nexp == b.nexp && a.newton == b.newton;
+}
+
 extern "C" grmhd_status grmhd_rk3_step_graph(grmhd_ctx* c, const double* dt_dev, double* state, double* const B[3],
                                              double* aux, grmhd_stream_t stream) {
     if (!c || !dt_dev || !state || !B || !aux) { set_err("null argument"); return GRMHD_ERR_INVALID; }
@@ -2116,7 +2122,11 @@ extern "C" grmhd_status grmhd_rk3_step_graph(grmhd_ctx* c, const double* dt_dev,
     if (r != GRMHD_OK) return r;
     hipStream_t s = (hipStream_t)stream;
     const void* key[8] = {dt_dev, state, B[0], B[1], B[2], aux, nullptr, nullptr};
-    bool same = c->gexec && std::memcmp(key, c->gkey, sizeof(key)) == 0;
+    // the learned c2p's bracket/Newton settings are kernel arguments, so a
+    // change of them (grmhd_mlp_set_bracket/set_newton, a new fit) needs a
+    // new capture; its weights are read through pointers
+    const MlpDev md = c->mlp ? c->mlp->dev() : MlpDev{};
+    bool same = c->gexec && std::memcmp(key, c->gkey, sizeof(key)) == 0 && same_mlp(md, c->gmlp);
     if (!same) {
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
@@ -2129,6 +2139,7 @@ extern "C" grmhd_status grmhd_rk3_step_graph(grmhd_ctx* c, const double* dt_dev,
         if (e != hipSuccess) { set_err("capture: %s", hipGetErrorString(e)); return GRMHD_ERR_HIP; }
         HIP_TRY(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
         std::memcpy(c->gkey, key, sizeof(key));
+        c->gmlp = md;
     }
     Scope sc(c, "step", s);
     HIP_TRY(hipGraphLaunch(c->gexec, s));

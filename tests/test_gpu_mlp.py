@@ -275,3 +275,36 @@ def test_gpu_learned_c2p_out_of_distribution(dev, case):
     say(f"  max state-normalised difference {r.max().item():.3e}, stats {res['learned'][2]}")
     assert bool(torch.isfinite(res["learned"][0]).all())
     assert r.max().item() < 1e-10
+
+
+def test_gpu_learned_rk3_graph_follows_settings(dev):
+    """A captured RK3 graph with the learned c2p replays bit-identically to
+    the eager step, and is captured again when the network's settings change
+    (here the Newton evaluation limit: 0 switches to the brackets)."""
+    import torch
+    wl = W.fm_torus(nb=(2, 2, 2), n=16, extent=24.0)
+    say("[learned rk3 graph] FM torus 32^3")
+    dt = torch.tensor([0.25 * wl.extra["dx"]], dtype=torch.float64, device=dev)
+    mlp = ga.Mlp()
+    out = {}
+    for mode in ("eager", "graph"):
+        ctx = ga.Context(wl.params, wl.grid, wl.instances)
+        st = to_dev(wl.state, dev)
+        B = [to_dev(b, dev) for b in wl.B]
+        aux = to_dev(wl.aux, dev)
+        if mode == "eager":
+            mlp.fit_cells(ctx, st, B, max_states=1 << 15, opts=ga.Mlp.default_fit_opts(epochs=20))
+        ctx.set_learned_c2p(mlp)
+        s = torch.cuda.Stream()
+        res = []
+        with torch.cuda.stream(s):
+            for newton in (8, 0):
+                mlp.set_newton(newton)
+                ctx.rk3_step(dt, st, B, aux, stream=s.cuda_stream, graph=(mode == "graph"))
+                s.synchronize()
+                res.append((to_host(st), to_host(aux)))
+        mlp.set_newton(8)
+        out[mode] = res
+    for (se, ae), (sg, ag) in zip(out["eager"], out["graph"]):
+        assert np.array_equal(se, sg)
+        assert np.array_equal(ae, ag)
