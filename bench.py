@@ -135,8 +135,10 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
                      ga.make_instances([dict(nb=(1, 1, 1), q0=0, bc=(0,) * 6, idx=(1.0, 1.0, 1.0))]))
     mlp = ga.Mlp()
     t0 = time.perf_counter()
+    # 120 epochs (2.4 s): median log error 1.7e-3 instead of 2.7e-3 at the
+    # default 30, 3.8 instead of 4.4 Newton evaluations per state
     rep = mlp.fit(ctx, n_train, torch.from_numpy(tr["cons"]).to(dev), torch.from_numpy(tr["met"]).to(dev),
-                  torch.from_numpy(tr["rad"]).to(dev))
+                  torch.from_numpy(tr["rad"]).to(dev), opts=ga.Mlp.default_fit_opts(epochs=120))
     torch.cuda.synchronize()
     fit_s = time.perf_counter() - t0
     del tr
