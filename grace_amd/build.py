@@ -70,12 +70,30 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j4"])
 
 
+HOST_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp")
+HOST_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_host_api")
+
+
+def build_host_test():
+    """The C++ test of grace_amd/host/grace_mi355x.hh (links the library and
+    the CPU oracle; run on the GPU by tests/test_gpu_cpp_host.py).  Needs the
+    oracle built first."""
+    os.makedirs(os.path.dirname(HOST_TEST_BIN), exist_ok=True)
+    tmp = HOST_TEST_BIN + ".tmp"
+    _run([HIPCC, "-std=c++17", "-O2", f"--offload-arch={ARCH}", "-o", tmp, "-x", "hip", HOST_TEST_SRC,
+          "-L" + HERE, "-lgrmhd_mi355x", "-L" + os.path.join(ROOT, "oracle", "_build"), "-loracle",
+          "-Wl,-rpath,$ORIGIN/../../../grace_amd", "-Wl,-rpath,$ORIGIN/../../../oracle/_build"])
+    os.replace(tmp, HOST_TEST_BIN)
+    return HOST_TEST_BIN
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     build_lib()
     build_io()
     if "--oracle" in argv:
         build_oracle()
+        build_host_test()
 
 
 if __name__ == "__main__":

@@ -50,6 +50,8 @@ enum { GRMHD_RIEMANN_HLLE = 0, GRMHD_RIEMANN_LLF = 1 };
 enum { GRMHD_EMF_UCT = 0, GRMHD_EMF_GS = 1 };
 enum { GRMHD_THERMO_TEMP = 0, GRMHD_THERMO_PRESS = 1 };
 enum { GRMHD_BC_OUTFLOW = 0, GRMHD_BC_PERIODIC = 1 };
+/* variables per block of the state and aux arrays (layout above) */
+enum { GRMHD_NSTATE = 23, GRMHD_NAUX = 15, GRMHD_NHRSC = 7 };
 
 /* Runtime + compile-time options of the reference's hot path.  Defaults
  * (grmhd_default_params) are the reference's parameter-schema defaults,
