@@ -8,6 +8,8 @@ back to a CPU implementation: if the library is missing, every call raises.
 import ctypes as C
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgrmhd_mi355x.so")
 
@@ -58,6 +60,26 @@ class Instance(C.Structure):
     _fields_ = [("nb", C.c_int32 * 3), ("q0", C.c_int64), ("bc", C.c_int32 * 6), ("idx", C.c_double * 3)]
 
 
+class Forest(C.Structure):
+    """grmhd_forest: two-level block forest (see include/grmhd_abi.h)."""
+    _fields_ = [("nb", C.c_int32 * 3), ("bc", C.c_int32 * 6), ("idx", C.c_double * 3), ("n_refined", C.c_int32),
+                ("refined", C.POINTER(C.c_int32))]
+
+
+def make_forest(nb, bc, idx, refined):
+    """A Forest struct plus the int32 array it points to (keep both alive)."""
+    ref = np.ascontiguousarray(sorted(set(int(r) for r in refined)), dtype=np.int32)
+    f = Forest()
+    for d in range(3):
+        f.nb[d] = int(nb[d]); f.idx[d] = float(idx[d])
+    for d in range(6):
+        f.bc[d] = int(bc[d])
+    f.n_refined = len(ref)
+    f.refined = ref.ctypes.data_as(C.POINTER(C.c_int32)) if len(ref) else C.POINTER(C.c_int32)()
+    f._keep = ref
+    return f
+
+
 class C2PStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("n_cells", "n_fatal", "n_atmo_reset", "n_ent_backup", "n_guess_used",
                                           "n_root_evals", "n_bracket_evals", "max_root_evals")]
@@ -67,6 +89,7 @@ _lib = None
 
 ABI_SYMBOLS = [
     "grmhd_default_params", "grmhd_ctx_create", "grmhd_ctx_destroy", "grmhd_last_error",
+    "grmhd_ctx_create_forest", "grmhd_forest_block", "grmhd_setup_forest_wave",
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
@@ -120,6 +143,10 @@ def lib():
     L.grmhd_default_params.argtypes = [C.POINTER(Params)]
     L.grmhd_ctx_create.argtypes = [C.POINTER(Params), C.POINTER(Grid), C.POINTER(Instance), i32, C.POINTER(P)]
     L.grmhd_ctx_destroy.argtypes = [P]
+    L.grmhd_ctx_create_forest.argtypes = [C.POINTER(Params), C.POINTER(Grid), C.POINTER(Forest), C.POINTER(P)]
+    L.grmhd_forest_block.argtypes = [P, i64, C.POINTER(i32), P]
+    L.grmhd_setup_forest_wave.argtypes = [C.POINTER(Grid), C.POINTER(Forest), d, P, d, d, d, d, d, P, P, P, d,
+                                          P, P, P, P, P]
     L.grmhd_last_error.restype = C.c_char_p
     L.grmhd_advance_substep.argtypes = [P, P, d, P, P, P * 3, P * 3, P, P]
     L.grmhd_fill_ghosts.argtypes = [P, P, P * 3, P]
@@ -228,15 +255,27 @@ def make_instances(specs):
 class Context:
     """Owns a grmhd_ctx; methods take torch device tensors."""
 
-    def __init__(self, params, grid, instances):
+    def __init__(self, params, grid, instances, forest=None):
         self.L = lib()
         self.params = params
         self.grid = grid
         self.instances = instances
+        self.forest = forest
         h = C.c_void_p()
-        check(self.L.grmhd_ctx_create(C.byref(params), C.byref(grid), instances, len(instances), C.byref(h)),
-              "ctx_create")
+        if forest is not None:
+            check(self.L.grmhd_ctx_create_forest(C.byref(params), C.byref(grid), C.byref(forest), C.byref(h)),
+                  "ctx_create_forest")
+        else:
+            check(self.L.grmhd_ctx_create(C.byref(params), C.byref(grid), instances, len(instances), C.byref(h)),
+                  "ctx_create")
         self.h = h
+
+    def forest_block(self, q):
+        """(level, (cx, cy, cz)) of block q of a forest context."""
+        lev = C.c_int32()
+        crd = np.zeros(3, dtype=np.int32)
+        check(self.L.grmhd_forest_block(self.h, int(q), C.byref(lev), crd.ctypes.data), "forest_block")
+        return lev.value, tuple(int(x) for x in crd)
 
     def close(self):
         if self.h:

@@ -10,6 +10,8 @@
 //                  divergence + CT face update (tstep_utils.hh:140-220,
 //                  evolve.cpp:1342-1479)
 //   k_gather       block ghost-zone gather (boundary_conditions.cpp:55-81)
+//   k_amr_*        coarse-fine ghosts and refluxing of a two-level forest
+//                  (forest.hh; prolongation/restriction kernels, refluxing.cpp)
 //   k_c2p_cells    c2p over cells + conserved rewrite + BDIV + E_center
 //                  (auxiliaries.cpp:81-180, grmhd.hh:306-434)
 //   k_c2p_batch    c2p over a flat batch (c2p.cpp:188-414)
@@ -24,6 +26,7 @@
 #include <map>
 #include "phys.hh"
 #include "mlp.hh"
+#include "forest.hh"
 #include "../../include/grmhd_abi.h"
 #include "../../include/grmhd_mlp.h"
 
@@ -842,7 +845,7 @@ struct GatherArgs {
     double* B[3];                            // face arrays (null: skip faces)
     int cell_mode;
 };
-constexpr int GSRC_OWN = 1 << 30;
+// GSRC_OWN / GSRC_AMR: forest.hh
 
 // Source of ghost element `loc` (face direction fd, -1 for cells) of block b.
 static void ghost_source(const DevGrid& G, const DevInst& I, const DevBlk& b, int fd, const int loc[3], int& qs,
@@ -882,8 +885,8 @@ __global__ __launch_bounds__(256) void k_gather_cells(const int32_t* __restrict_
     const int64_t q = wp.q;
     const int2 sr = src[q * per + e];
     const bool own = (sr.x & GSRC_OWN) != 0;
-    if (mode >= 2 && (mode == 2) != own) return;
-    const int64_t qs = sr.x & (GSRC_OWN - 1);
+    if ((sr.x & GSRC_AMR) || (mode >= 2 && (mode == 2) != own)) return;
+    const int64_t qs = sr.x & (GSRC_AMR - 1);
     const int off = list[e];
     for (int c0 = 0; c0 < ch.nch; c0 += 4) {
         double v[4];
@@ -928,6 +931,7 @@ __global__ __launch_bounds__(256) void k_gather_faces(const int32_t* __restrict_
     if (e < ncg + nfg0) { B = B0; nf = nf0; }
     else if (e < ncg + nfg0 + nfg1) { B = B1; nf = nf1; }
     else { B = B2; nf = nf2; }
+    if (sr.x & GSRC_AMR) return;
     B[q * nf + list[e]] = B[(int64_t)sr.x * nf + sr.y];
 }
 
@@ -1311,6 +1315,18 @@ struct grmhd_ctx {
     const void* gkey[8] = {0};
     // workgroup sizes picked by grmhd_autotune (all 256 until then)
     grmhd_tuning tune{{256, 256, 256}, 256, 256, 256, 256, 256};
+    // two-level forest (grmhd_ctx_create_forest)
+    bool forest = false;
+    FForest ff{};                             // device tables
+    std::vector<int32_t> cmap_h;
+    std::vector<int4> fblk_h;
+    int32_t* cmap_d = nullptr;
+    int4* fblk_d = nullptr;
+    int2* amr_cells = nullptr; int64_t n_amr_cells = 0;
+    int2* amr_faces = nullptr; int64_t n_amr_faces = 0;
+    RefluxFace* rflux = nullptr; int64_t n_rflux = 0;
+    RefluxEdge* remf = nullptr; int64_t n_remf = 0;
+    int64_t n_own_gather = 0;                 // own-c2p ghosts that copy a clamped source
 };
 
 static hipEvent_t pool_event(grmhd_ctx* c) {
@@ -1383,6 +1399,7 @@ static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
+    F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->rflux); F(c->remf);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1390,8 +1407,15 @@ static void free_ctx(grmhd_ctx* c) {
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
 }
 
-extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid* g, const grmhd_instance* inst,
-                                         int32_t n_inst, grmhd_ctx** out) {
+// Host description of a two-level forest (grmhd_ctx_create_forest).
+struct ForestIn {
+    int nb[3], bc[6];
+    std::vector<int32_t> cmap;
+    std::vector<int4> fblk;
+};
+
+static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, const grmhd_instance* inst,
+                                    int32_t n_inst, const ForestIn* fin, grmhd_ctx** out) {
     if (!p || !g || !out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
     *out = nullptr;
     if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->ngz < 3 || g->nq < 1) {
@@ -1510,6 +1534,24 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
         // per-block sources of every ghost element
         const int64_t per = (int64_t)lst.size();
         std::vector<int2> src((size_t)(per * G.nq));
+        FForest ffh{};   // the forest's lookup over host copies of its tables
+        if (fin) {
+            c->forest = true;
+            c->cmap_h = fin->cmap;
+            c->fblk_h = fin->fblk;
+            for (int d = 0; d < 3; ++d) { ffh.nb[d] = fin->nb[d]; ffh.n[d] = G.n[d]; ffh.N[d] = G.N[d]; ffh.nface[d] = G.nface[d]; }
+            for (int d = 0; d < 6; ++d) ffh.bc[d] = fin->bc[d];
+            ffh.ng = G.ng; ffh.ncell = G.ncell; ffh.nq = G.nq;
+            ffh.cmap = c->cmap_h.data(); ffh.blk = c->fblk_h.data();
+            if ((e = hipMalloc(&c->cmap_d, sizeof(int32_t) * c->cmap_h.size())) != hipSuccess) return fail("forest map", e);
+            if ((e = hipMemcpy(c->cmap_d, c->cmap_h.data(), sizeof(int32_t) * c->cmap_h.size(), hipMemcpyHostToDevice)) != hipSuccess)
+                return fail("forest map copy", e);
+            if ((e = hipMalloc(&c->fblk_d, sizeof(int4) * c->fblk_h.size())) != hipSuccess) return fail("forest blocks", e);
+            if ((e = hipMemcpy(c->fblk_d, c->fblk_h.data(), sizeof(int4) * c->fblk_h.size(), hipMemcpyHostToDevice)) != hipSuccess)
+                return fail("forest blocks copy", e);
+            c->ff = ffh;
+            c->ff.cmap = c->cmap_d; c->ff.blk = c->fblk_d;
+        }
         for (int64_t q = 0; q < G.nq; ++q) {
             const DevBlk& b = blk[q];
             const DevInst& I = c->inst_h[b.inst];
@@ -1519,35 +1561,139 @@ extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid
                 int off = lst[x];
                 int loc[3] = {off % E0, (off / E0) % E1, off / (E0 * E1)};
                 int qs, so;
-                ghost_source(G, I, b, fd, loc, qs, so);
-                if (fd < 0 && ghost_needs_own_c2p(G, I, b, loc[0], loc[1], loc[2])) qs |= GSRC_OWN;
+                if (fin) {
+                    forest_source(ffh, (int)q, fd, loc, qs, so);
+                } else {
+                    ghost_source(G, I, b, fd, loc, qs, so);
+                    if (fd < 0 && ghost_needs_own_c2p(G, I, b, loc[0], loc[1], loc[2])) qs |= GSRC_OWN;
+                }
                 src[(size_t)(q * per + x)] = make_int2(qs, so);
             }
         }
-        std::vector<int2> own;
+        std::vector<int2> own, amr_c, amr_f;
         std::vector<int4> owng;
         for (int64_t q = 0; q < G.nq; ++q)
-            for (int64_t x = 0; x < c->ncg; ++x) {
+            for (int64_t x = 0; x < per; ++x) {
                 const int2 sr = src[(size_t)(q * per + x)];
-                if (sr.x & GSRC_OWN) {
-                    own.push_back(make_int2((int)q, lst[x]));
-                    owng.push_back(make_int4((int)q, lst[x], sr.x & (GSRC_OWN - 1), sr.y));
+                if (x >= c->ncg) {
+                    if (sr.x & GSRC_AMR) {
+                        const int fd = x < c->ncg + c->nfg[0] ? 0 : (x < c->ncg + c->nfg[0] + c->nfg[1] ? 1 : 2);
+                        amr_f.push_back(make_int2((int)q, (fd << 24) | lst[x]));
+                    }
+                    continue;
                 }
+                if (sr.x & GSRC_OWN) own.push_back(make_int2((int)q, lst[x]));
+                if (sr.x & GSRC_AMR) amr_c.push_back(make_int2((int)q, lst[x]));
+                else if (sr.x & GSRC_OWN) owng.push_back(make_int4((int)q, lst[x], sr.x & (GSRC_AMR - 1), sr.y));
             }
         c->n_own = (int64_t)own.size();
-        if (!own.empty()) {
-            if ((e = hipMalloc(&c->own_cells, sizeof(int2) * own.size())) != hipSuccess) return fail("own cells", e);
-            if ((e = hipMemcpy(c->own_cells, own.data(), sizeof(int2) * own.size(), hipMemcpyHostToDevice)) != hipSuccess)
-                return fail("own cells copy", e);
-            if ((e = hipMalloc(&c->own_gather, sizeof(int4) * owng.size())) != hipSuccess) return fail("own gather", e);
-            if ((e = hipMemcpy(c->own_gather, owng.data(), sizeof(int4) * owng.size(), hipMemcpyHostToDevice)) != hipSuccess)
-                return fail("own gather copy", e);
+        c->n_own_gather = (int64_t)owng.size();
+        c->n_amr_cells = (int64_t)amr_c.size();
+        c->n_amr_faces = (int64_t)amr_f.size();
+        auto upload = [&](auto** dst, const auto& v, const char* what) -> bool {
+            if (v.empty()) return true;
+            const size_t bytes = sizeof(v[0]) * v.size();
+            if ((e = hipMalloc((void**)dst, bytes)) != hipSuccess) return false;
+            if ((e = hipMemcpy(*dst, v.data(), bytes, hipMemcpyHostToDevice)) != hipSuccess) return false;
+            (void)what;
+            return true;
+        };
+        if (!upload(&c->own_cells, own, "own cells")) return fail("own cells", e);
+        if (!upload(&c->own_gather, owng, "own gather")) return fail("own gather", e);
+        if (!upload(&c->amr_cells, amr_c, "amr cells")) return fail("coarse-fine cells", e);
+        if (!upload(&c->amr_faces, amr_f, "amr faces")) return fail("coarse-fine faces", e);
+        if (fin) {
+            std::vector<RefluxFace> rf;
+            std::vector<RefluxEdge> re;
+            forest_reflux(ffh, rf, re);
+            c->n_rflux = (int64_t)rf.size();
+            c->n_remf = (int64_t)re.size();
+            if (!upload(&c->rflux, rf, "reflux faces")) return fail("reflux faces", e);
+            if (!upload(&c->remf, re, "reflux edges")) return fail("reflux edges", e);
         }
         if ((e = hipMalloc(&c->ghost_src, sizeof(int2) * src.size())) != hipSuccess) return fail("ghost sources", e);
         if ((e = hipMemcpy(c->ghost_src, src.data(), sizeof(int2) * src.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail("ghost sources copy", e);
     }
     *out = c;
+    return GRMHD_OK;
+}
+
+extern "C" grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid* g, const grmhd_instance* inst,
+                                         int32_t n_inst, grmhd_ctx** out) {
+    return ctx_create_impl(p, g, inst, n_inst, nullptr, out);
+}
+
+extern "C" grmhd_status grmhd_ctx_create_forest(const grmhd_params* p, const grmhd_grid* g, const grmhd_forest* f,
+                                                grmhd_ctx** out) {
+    if (!p || !g || !f || !out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d)
+        if (f->nb[d] < 1 || !(f->idx[d] > 0)) { set_err("forest: nb must be >= 1 and idx > 0"); return GRMHD_ERR_INVALID; }
+    for (int d = 0; d < 6; ++d) if (f->bc[d] != 0 && f->bc[d] != 1) { set_err("forest: bc kind"); return GRMHD_ERR_INVALID; }
+    const int64_t nc = (int64_t)f->nb[0] * f->nb[1] * f->nb[2];
+    if (nc > (1 << 24)) { set_err("forest: too many coarse blocks"); return GRMHD_ERR_INVALID; }
+    if (f->n_refined < 0 || f->n_refined > nc || (f->n_refined > 0 && !f->refined)) {
+        set_err("forest: bad refined list"); return GRMHD_ERR_INVALID;
+    }
+    std::vector<char> ref((size_t)nc, 0);
+    for (int r = 0; r < f->n_refined; ++r) {
+        const int32_t id = f->refined[r];
+        if (id < 0 || id >= nc || ref[id]) { set_err("forest: refined id %d out of range or repeated", id); return GRMHD_ERR_INVALID; }
+        ref[id] = 1;
+    }
+    if (g->nx % 2 || g->ny % 2 || g->nz % 2 || g->ngz % 2) {
+        set_err("forest: block extents and ngz must be even (coarse-fine operators work on 2x2x2 children)");
+        return GRMHD_ERR_INVALID;
+    }
+    // grace_initialize.cpp:243-248; the divergence-preserving prolongation
+    // (prolongation_kernels.hh:330-381) is written for cubic cells
+    if (g->nx != g->ny || g->ny != g->nz || f->idx[0] != f->idx[1] || f->idx[1] != f->idx[2]) {
+        set_err("forest: blocks must be cubic with equal spacing in x, y and z");
+        return GRMHD_ERR_INVALID;
+    }
+    if (g->nq != nc + 7 * (int64_t)f->n_refined) {
+        set_err("forest: grid.nq must be %lld (coarse blocks + 7 per refined block)", (long long)(nc + 7 * (int64_t)f->n_refined));
+        return GRMHD_ERR_INVALID;
+    }
+    if ((int64_t)(g->nx + 2 * g->ngz + 1) * (g->ny + 2 * g->ngz + 1) * (g->nz + 2 * g->ngz + 1) >= (1 << 24)) {
+        set_err("forest: blocks too large"); return GRMHD_ERR_INVALID;
+    }
+    if (p->fofc) { set_err("FOFC on a forest is not supported"); return GRMHD_ERR_UNSUPPORTED; }
+    ForestIn fin;
+    for (int d = 0; d < 3; ++d) fin.nb[d] = f->nb[d];
+    for (int d = 0; d < 6; ++d) fin.bc[d] = f->bc[d];
+    fin.cmap.assign((size_t)nc, 0);
+    int32_t nq = 0;
+    for (int64_t b = 0; b < nc; ++b)
+        if (!ref[b]) {
+            fin.cmap[b] = nq++;
+            fin.fblk.push_back(make_int4(0, (int)(b % f->nb[0]), (int)((b / f->nb[0]) % f->nb[1]), (int)(b / ((int64_t)f->nb[0] * f->nb[1]))));
+        }
+    for (int64_t b = 0; b < nc; ++b)
+        if (ref[b]) {
+            fin.cmap[b] = -(1 + nq);
+            const int cb[3] = {(int)(b % f->nb[0]), (int)((b / f->nb[0]) % f->nb[1]), (int)(b / ((int64_t)f->nb[0] * f->nb[1]))};
+            for (int ch = 0; ch < 8; ++ch, ++nq)
+                fin.fblk.push_back(make_int4(1, 2 * cb[0] + (ch & 1), 2 * cb[1] + ((ch >> 1) & 1), 2 * cb[2] + ((ch >> 2) & 1)));
+        }
+    std::vector<grmhd_instance> inst((size_t)nq);
+    for (int32_t q = 0; q < nq; ++q) {
+        grmhd_instance& I = inst[q];
+        const double s = fin.fblk[q].x ? 2.0 : 1.0;
+        for (int d = 0; d < 3; ++d) { I.nb[d] = 1; I.idx[d] = f->idx[d] * s; }
+        for (int d = 0; d < 6; ++d) I.bc[d] = f->bc[d];
+        I.q0 = q;
+    }
+    return ctx_create_impl(p, g, inst.data(), nq, &fin, out);
+}
+
+extern "C" grmhd_status grmhd_forest_block(const grmhd_ctx* c, int64_t q, int32_t* level, int32_t coords[3]) {
+    if (!c || !level || !coords) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    if (!c->forest) { set_err("not a forest context"); return GRMHD_ERR_INVALID; }
+    if (q < 0 || q >= c->G.nq) { set_err("block %lld out of range", (long long)q); return GRMHD_ERR_INVALID; }
+    const int4 b = c->fblk_h[(size_t)q];
+    *level = b.x; coords[0] = b.y; coords[1] = b.z; coords[2] = b.w;
     return GRMHD_OK;
 }
 
@@ -1669,6 +1815,8 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
         k_gather_faces<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_faces, 0, s>>>(
             c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0], (int)c->nfg[1], (int)c->nfg[2], wpb,
             a.B[0], a.B[1], a.B[2], G.nface[0], G.nface[1], G.nface[2]);
+        if (c->n_amr_faces > 0)
+            k_amr_faces<<<nblk(c->n_amr_faces), 256, 0, s>>>(c->amr_faces, (int)c->n_amr_faces, a.B[0], a.B[1], a.B[2], c->ff);
     }
     if (a.cell_mode == 0) return;
     Scope sc(c, "ghost", s);
@@ -1682,14 +1830,30 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     if (a.ecen)
         for (int v = 0; v < 3; ++v) add(a.ecen, 3, v);
     if (ch.nch == 0) return;
+    // coarse-fine ghost cells of the state (they run their own c2p)
+    const bool amr = a.st && a.cell_mode <= 2 && c->n_amr_cells > 0;
     if (a.cell_mode == 2) {
-        if (c->n_own > 0)
-            k_gather_list<<<dim3(nblk(c->n_own), 1, ch.nch), 256, 0, s>>>(c->own_gather, (int)c->n_own, ch);
+        if (c->n_own_gather > 0)
+            k_gather_list<<<dim3(nblk(c->n_own_gather), 1, ch.nch), 256, 0, s>>>(c->own_gather, (int)c->n_own_gather, ch);
+        if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
         return;
     }
     const int wpb = (int)nblk_bs(c->ncg, c->tune.gather_cells);
     k_gather_cells<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per,
                                                                                  (int)c->ncg, wpb, a.cell_mode, ch);
+    if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
+}
+
+// reflux_correct_fluxes / reflux_correct_emfs (refluxing.cpp:547-806): after
+// the edge EMFs, before the update that consumes both.
+static void launch_reflux(grmhd_ctx* c, hipStream_t s) {
+    if (!c->forest) return;
+    Scope sc(c, "reflux", s);
+    const DevGrid& G = c->G;
+    if (c->n_rflux > 0)
+        k_reflux_flux<<<dim3(nblk(c->n_rflux), NHRSC), 256, 0, s>>>(c->rflux, (int)c->n_rflux, c->flux[0], c->flux[1],
+                                                                    c->flux[2], G.nface[0], G.nface[1], G.nface[2]);
+    if (c->n_remf > 0) k_reflux_emf<<<nblk(c->n_remf), 256, 0, s>>>(c->remf, (int)c->n_remf, c->emf);
 }
 
 // c2p over the cells of every block: the interior cells, plus either every
@@ -1775,6 +1939,7 @@ extern "C" grmhd_status grmhd_advance_substep(grmhd_ctx* c, const double* dt_dev
     launch_emf(c, old_B, s);
     if (c->P.fofc)
         launch_fofc(c, new_state, NSTATE, nullptr, 0, 1.0, 0.0, old_state, old_B, aux, dt_dev, dtfact, s);
+    launch_reflux(c, s);
     UpdArgs a{};
     a.nst = new_state; a.nnv = NSTATE;
     a.ast = new_state; a.anv = NSTATE; a.bst = nullptr; a.bnv = 0; a.ca = 1.0; a.cb = 0.0;
@@ -2061,6 +2226,7 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         launch_emf(c, S.oB, s);
         if (c->P.fofc)
             launch_fofc(c, S.ast, S.anv, S.bst, S.bnv, S.ca, S.cb, state, S.oB, aux, dt_dev, S.dtf, s);
+        launch_reflux(c, s);
         UpdArgs a{};
         a.nst = S.nst; a.nnv = S.nnv; a.ast = S.ast; a.anv = S.anv; a.bst = S.bst; a.bnv = S.bnv;
         a.ca = S.ca; a.cb = S.cb; a.mstate = state; a.mnv = NSTATE; a.aux = aux;

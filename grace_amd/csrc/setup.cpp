@@ -252,6 +252,82 @@ FMPoint fm_point(const FMTorus& T, double lnh_in, double kappa, double gamma, do
 
 extern "C" {
 
+int grmhd_setup_forest_wave(const grmhd_grid* g, const grmhd_forest* f, double gamma, const double lo[3],
+                            double rho0, double amp, double p0, double p_in, double r_in, const double center[3],
+                            const double zvel[3], const double Bvec[3], double b_loop, double* state, double* Bx,
+                            double* By, double* Bz, double* aux) {
+    if (!g || !f || !(gamma > 1.0) || !(rho0 > 0) || !(p0 > 0)) return 1;
+    const int64_t nc = (int64_t)f->nb[0] * f->nb[1] * f->nb[2];
+    if (g->nq != nc + 7 * (int64_t)f->n_refined) return 1;
+    std::vector<char> ref((size_t)nc, 0);
+    for (int r = 0; r < f->n_refined; ++r) {
+        if (f->refined[r] < 0 || f->refined[r] >= nc) return 1;
+        ref[f->refined[r]] = 1;
+    }
+    // block order: coarse leaves, then 8 children per refined block
+    std::vector<int> lev, crd;
+    for (int64_t b = 0; b < nc; ++b)
+        if (!ref[b]) { lev.push_back(0); crd.push_back((int)(b % f->nb[0])); crd.push_back((int)((b / f->nb[0]) % f->nb[1])); crd.push_back((int)(b / ((int64_t)f->nb[0] * f->nb[1]))); }
+    for (int64_t b = 0; b < nc; ++b)
+        if (ref[b]) {
+            const int cb[3] = {(int)(b % f->nb[0]), (int)((b / f->nb[0]) % f->nb[1]), (int)(b / ((int64_t)f->nb[0] * f->nb[1]))};
+            for (int ch = 0; ch < 8; ++ch) {
+                lev.push_back(1);
+                for (int d = 0; d < 3; ++d) crd.push_back(2 * cb[d] + ((ch >> d) & 1));
+            }
+        }
+    Idx X(g);
+    double* B[3] = {Bx, By, Bz};
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const double L[3] = {f->nb[0] * n[0] / f->idx[0], f->nb[1] * n[1] / f->idx[1], f->nb[2] * n[2] / f->idx[2]};
+    const double kx = 2.0 * M_PI / L[0], ky = 2.0 * M_PI / L[1];
+    const double gm1 = gamma - 1.0;
+    auto Az = [&](double x, double y) { return b_loop / kx * std::sin(kx * (x - lo[0])) * std::sin(ky * (y - lo[1])); };
+    parallel_blocks(g->nq, [&](int64_t q0, int64_t q1, int) {
+        for (int64_t q = q0; q < q1; ++q) {
+            std::memset(aux + q * NAUX * X.ncell, 0, sizeof(double) * NAUX * X.ncell);
+            const double s = lev[q] ? 2.0 : 1.0;
+            double dx[3], org[3], idq[3];
+            for (int d = 0; d < 3; ++d) {
+                idq[d] = f->idx[d] * s;
+                dx[d] = 1.0 / idq[d];
+                org[d] = lo[d] + (double)crd[3 * q + d] * n[d] * dx[d];
+            }
+            auto pos = [&](int d, int l, double off) { return org[d] + (l - g->ngz + off) * dx[d]; };
+            for (int k = 0; k < X.N[2]; ++k)
+                for (int j = 0; j < X.N[1]; ++j)
+                    for (int i = 0; i < X.N[0]; ++i) {
+                        const double x = pos(0, i, 0.5), y = pos(1, j, 0.5), z = pos(2, k, 0.5);
+                        const double r = std::sqrt((x - center[0]) * (x - center[0]) + (y - center[1]) * (y - center[1]) +
+                                                   (z - center[2]) * (z - center[2]));
+                        const double rho = rho0 * (1.0 + amp * std::sin(kx * (x - lo[0])) * std::sin(ky * (y - lo[1])));
+                        const double p = (r_in > 0 && r < r_in) ? p_in : p0;
+                        const int64_t c = X.c(i, j, k);
+                        flat_metric(state, X, q, c);
+                        auto A = [&](int v) -> double& { return aux[(q * NAUX + v) * X.ncell + c]; };
+                        const double eps = p / (gm1 * rho);
+                        A(RHO_) = rho; A(PRESS_) = p; A(EPS_) = eps; A(TEMP_) = eps * gm1;
+                        A(ENTROPY_) = std::log(eps) / gm1 - std::log(rho);
+                        A(ZVECX_) = zvel[0]; A(ZVECY_) = zvel[1]; A(ZVECZ_) = zvel[2]; A(YE_) = 0.0;
+                    }
+            // B_x = dA_z/dy, B_y = -dA_z/dx on the faces, from A_z at the z-edges
+            for (int k = 0; k < X.N[2]; ++k)
+                for (int j = 0; j < X.N[1] + 1; ++j)
+                    for (int i = 0; i < X.N[0] + 1; ++i) {
+                        if (j < X.N[1])
+                            B[0][q * X.nface[0] + X.f(0, i, j, k)] =
+                                Bvec[0] + (Az(pos(0, i, 0.0), pos(1, j + 1, 0.0)) - Az(pos(0, i, 0.0), pos(1, j, 0.0))) * idq[1];
+                        if (i < X.N[0])
+                            B[1][q * X.nface[1] + X.f(1, i, j, k)] =
+                                Bvec[1] - (Az(pos(0, i + 1, 0.0), pos(1, j, 0.0)) - Az(pos(0, i, 0.0), pos(1, j, 0.0))) * idq[0];
+                    }
+            for (int64_t fz = 0; fz < X.nface[2]; ++fz) B[2][q * X.nface[2] + fz] = Bvec[2];
+            finish_cells(g, X, state, B, aux, idq, q);
+        }
+    });
+    return 0;
+}
+
 int grmhd_setup_blast(const grmhd_grid* g, const grmhd_instance* inst, int32_t n_inst, double gamma,
                       const double lo[3], const double hi[3], double rho_amb, double p_amb, double p_in, double r_in,
                       const double Bvec[3], double noise, uint64_t seed, double* state, double* Bx, double* By,

@@ -6,7 +6,7 @@ CPU checker receive bit-identical inputs.  Seed 20261202 unless stated.
 """
 import numpy as np
 
-from . import (lib, Grid, make_instances, default_params, NSTATE, NAUX, BC_OUTFLOW, BC_PERIODIC,
+from . import (lib, Grid, make_instances, make_forest, default_params, NSTATE, NAUX, BC_OUTFLOW, BC_PERIODIC,
                EMF_GS, EMF_UCT)
 
 SEED = 20261202
@@ -26,6 +26,7 @@ class Workload:
         self.aux = aux
         self.zones = zones
         self.extra = extra or {}
+        self.forest = None   # grmhd_forest of a two-level forest workload
 
     def copy_arrays(self):
         return self.state.copy(), [b.copy() for b in self.B], self.aux.copy()
@@ -42,6 +43,44 @@ def alloc(grid):
 
 def _p(a):
     return a.ctypes.data
+
+
+def forest_wave(nb=(4, 4, 4), n=8, ngz=4, refined=None, bc=BC_PERIODIC, extent=1.0, gamma=4.0 / 3.0,
+                rho0=1.0, amp=0.2, p0=1.0, p_in=0.0, r_in=0.0, zvel=(0.3, 0.2, 0.1), Bvec=(0.2, 0.1, 0.05),
+                b_loop=0.1, emf=EMF_UCT, recon=0, riemann=0):
+    """Flat-space data on a two-level forest: nb coarse blocks of n^3 cubic
+    cells of side extent / (nb[0] n), the box centred on the origin, the
+    coarse blocks in `refined` (coarse ids
+    bi + nb0 (bj + nb1 bk); default: the central 2x2x2, or the first block when
+    nb has a side < 2) refined once.  A density wave rho0 (1 + amp sin sin),
+    pressure p0 (p_in inside r < r_in at the centre), uniform z = W v, face B =
+    Bvec plus a divergence-free loop of strength b_loop."""
+    if refined is None:
+        if min(nb) >= 2:
+            c = [nb[d] // 2 - 1 for d in range(3)]
+            refined = [(c[0] + a) + nb[0] * ((c[1] + b) + nb[1] * (c[2] + e))
+                       for e in range(2) for b in range(2) for a in range(2)]
+        else:
+            refined = [0]
+    bcs = (bc,) * 6 if isinstance(bc, int) else tuple(bc)
+    dx = extent / (nb[0] * n)
+    idx = (1.0 / dx,) * 3
+    forest = make_forest(nb, bcs, idx, refined)
+    nq = nb[0] * nb[1] * nb[2] + 7 * forest.n_refined
+    grid = Grid(n, n, n, ngz, nq)
+    state, B, aux = alloc(grid)
+    lo = np.array([-0.5 * nb[d] * n * dx for d in range(3)])
+    ctr = np.zeros(3)
+    zv = np.array(zvel, dtype=np.float64)
+    Bv = np.array(Bvec, dtype=np.float64)
+    rc = lib().grmhd_setup_forest_wave(grid, forest, gamma, _p(lo), rho0, amp, p0, p_in, r_in, _p(ctr), _p(zv),
+                                       _p(Bv), b_loop, _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux))
+    assert rc == 0, rc
+    params = default_params(gamma=gamma, emf=emf, recon=recon, riemann=riemann, flat_metric=1)
+    wl = Workload("forest_wave", params, grid, [], state, B, aux, zones=nq * n ** 3,
+                  extra=dict(dx=dx, extent=extent))
+    wl.forest = forest
+    return wl
 
 
 def blast(nb=(8, 8, 8), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01, emf=EMF_UCT, recon=0, riemann=0,

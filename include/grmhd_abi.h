@@ -111,6 +111,41 @@ void grmhd_default_params(grmhd_params* p);
  * the variable_list singleton of the reference. */
 grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid* grid,
                               const grmhd_instance* inst, int32_t n_inst, grmhd_ctx** out);
+/* Two-level block forest (the p4est forest of include/grace/amr/forest.hh
+ * restricted to one refinement level): nb[0]*nb[1]*nb[2] coarse blocks of
+ * spacing 1/idx, of which the n_refined listed (coarse ids
+ * bi + nb0*(bj + nb1*bk), any order, no repeats) are refined once into 8
+ * blocks of half the spacing.  bc[] per domain face as for an instance.
+ * Block order q: the coarse leaves in ascending coarse id, then the 8
+ * children (p4est Morton id cx + 2cy + 4cz) of each refined block, refined
+ * blocks in ascending coarse id; grmhd_forest_block reports where each lies.
+ * grid.nq must be nb0*nb1*nb2 + 7*n_refined; blocks cubic with even extent,
+ * ngz even and idx equal in x, y, z (grace_initialize.cpp:243-248; the
+ * divergence-preserving prolongation is written for cubic cells).
+ * On such a context the ghost fill also prolongs coarse data into fine ghosts
+ * and restricts fine data into coarse ghosts (minmod prolongation and 2x2x2
+ * averages for the hydro conserved, prolongation_kernels.hh:59-106 /
+ * pr_helpers.hh:71-212; Toth-Roe divergence-preserving prolongation and area
+ * averages for face B, prolongation_kernels.hh:111-386 /
+ * restrict_kernels.hh:125-164), and every substep refluxes: the coarse flux
+ * of each coarse-fine face becomes the average of the four fine fluxes and
+ * each coarse edge EMF touched by a fine cell the average of the two fine
+ * EMFs along it (refluxing.cpp:547-806, 959-1173).  The Cowling metric of the
+ * ghost cells is the caller's (initial data), as on uniform grids.  FOFC is
+ * not supported on a forest (GRMHD_ERR_UNSUPPORTED). */
+typedef struct {
+    int32_t nb[3];
+    int32_t bc[6];
+    double idx[3];            /* inverse coarse cell spacings */
+    int32_t n_refined;
+    const int32_t* refined;   /* host array of coarse ids */
+} grmhd_forest;
+
+grmhd_status grmhd_ctx_create_forest(const grmhd_params* p, const grmhd_grid* grid, const grmhd_forest* forest,
+                                     grmhd_ctx** out);
+/* Level (0 coarse, 1 fine) and block coordinates at that level of block q. */
+grmhd_status grmhd_forest_block(const grmhd_ctx* ctx, int64_t q, int32_t* level, int32_t coords[3]);
+
 void grmhd_ctx_destroy(grmhd_ctx* ctx);
 const char* grmhd_last_error(void);
 

@@ -56,6 +56,19 @@ int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_
 void grmhd_fm_torus_point(double spin, double gamma, double r_in, double r_max, double x, double y, double z,
                           double out[6]);
 
+/* Flat-space data on a two-level forest (block order of
+ * grmhd_ctx_create_forest; the forest's nb, idx and refined list) covering
+ * the box from lo: rho = rho0 (1 + amp sin(2 pi x/Lx) sin(2 pi y/Ly)),
+ * p = p0, or p_in inside the sphere of radius r_in around center, z = W v
+ * = zvel, face B = Bvec plus the discrete curl of
+ * A_z = b_loop Lx/(2 pi) sin(2 pi x/Lx) sin(2 pi y/Ly) sampled at the edges
+ * (divergence free in every block, and a coarse face holds the average of
+ * the four fine faces covering it).  Ghost cells included.  Ideal gas. */
+int grmhd_setup_forest_wave(const grmhd_grid* g, const grmhd_forest* f, double gamma, const double lo[3],
+                            double rho0, double amp, double p0, double p_in, double r_in, const double center[3],
+                            const double zvel[3], const double Bvec[3], double b_loop, double* state, double* Bx,
+                            double* By, double* Bz, double* aux);
+
 /* Seeded c2p test states (SoA outputs, see grmhd_abi.h for the layouts). */
 int grmhd_setup_c2p_states(int64_t n, uint64_t seed, double gamma, double spin, const double rho_rng[2],
                            const double eps_rng[2], const double W_rng[2], const double sig_rng[2],

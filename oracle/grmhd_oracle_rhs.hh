@@ -19,6 +19,7 @@
 // wider normal range only adds faces nobody reads when FOFC is off.
 #pragma once
 #include "grmhd_oracle.hh"
+#include "grmhd_oracle_amr.hh"
 #include <vector>
 #include <cstdio>
 
@@ -81,6 +82,7 @@ struct Layout {
     Grid grid;
     std::vector<Instance> inst;
     std::vector<BlockDesc> blk;
+    const AmrExchange* amr = nullptr;   // two-level forest: coarse-fine exchange + reflux
     Layout(const Grid& g) : grid(g) {}
     double idx(int64_t q, int d) const { return inst[blk[q].inst].idx[d]; }
 };
@@ -662,7 +664,8 @@ struct Stepper {
                     }
     }
 
-    // advance_substep (evolve.cpp:1671-1717); no reflux (uniform grids)
+    // advance_substep (evolve.cpp:1671-1717); reflux on a two-level forest
+    // (refluxing.cpp), a no-op on uniform grids
     void advance_substep(double dt, double dtfact, double* newst, double* nB[3], const Fields& old, const double* aux) {
         compute_fluxes(old, aux);
         if (opt.emf == EMF_GS) compute_emfs_gs();
@@ -670,6 +673,11 @@ struct Stepper {
         if (opt.fofc) {
             flag_fofc(dt, dtfact, newst, old);
             apply_fofc(old, aux);
+        }
+        if (L.amr) {
+            double* fl[3] = {flux[0].data(), flux[1].data(), flux[2].data()};
+            L.amr->reflux_fluxes(fl, NHRSC);
+            L.amr->reflux_emfs(emf.data());
         }
         add_fluxes_and_sources(dt, dtfact, newst, old, aux);
         update_ct(dt, dtfact, nB);
@@ -690,6 +698,7 @@ struct Stepper {
         return f;
     }
     void fill_ghosts(double* st, double* Bf[3], int nvar_first, int nvar_count) {
+        if (L.amr) { L.amr->fill(st, Bf, nvar_first, nvar_count); return; }
         const Grid& g = G();
         int n[3] = {g.nx, g.ny, g.nz};
         int Ne[3] = {g.Ni, g.Nj, g.Nk};

@@ -100,6 +100,7 @@ struct StepperBase {
     virtual double dt(const double* st, const double* aux, double cfl) = 0;
     virtual int64_t fatal() = 0;
     virtual int64_t fofc_flagged() = 0;
+    virtual const Forest* forest_of() const = 0;
 };
 
 template <typename EOS>
@@ -107,6 +108,8 @@ struct StepperT : StepperBase {
     Layout L;
     EOS eos;
     std::unique_ptr<Stepper<EOS>> S;
+    std::unique_ptr<Forest> forest;
+    std::unique_ptr<AmrExchange> exch;
     StepperT(const grmhd_params* p, const grmhd_grid* g, const grmhd_instance* inst, int n_inst, const EOS& e)
         : L(Grid(g->nx, g->ny, g->nz, g->ngz, g->nq)), eos(e) {
         L.blk.assign(g->nq, BlockDesc{-1, {0, 0, 0}});
@@ -135,8 +138,28 @@ struct StepperT : StepperBase {
     double dt(const double* st, const double* aux, double cfl) override { return S->find_dt(st, aux, cfl); }
     int64_t fatal() override { return S->n_fatal; }
     int64_t fofc_flagged() override { return S->fofc_cells_flagged; }
+    const Forest* forest_of() const override { return forest.get(); }
 };
 
+// Two-level forest: one 1x1x1 instance per block at its level's spacing, the
+// coarse-fine exchange and reflux switched on (grmhd_oracle_amr.hh).
+template <typename EOS>
+StepperBase* make_forest_stepper(const grmhd_params* p, const grmhd_grid* g, const grmhd_forest* f, const EOS& e) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    std::unique_ptr<Forest> F(new Forest(f->nb, f->bc, n, g->ngz, f->refined, f->n_refined));
+    if (F->nq != g->nq) return nullptr;
+    std::vector<grmhd_instance> inst(F->nq);
+    for (int64_t q = 0; q < F->nq; ++q) {
+        for (int d = 0; d < 3; ++d) { inst[q].nb[d] = 1; inst[q].idx[d] = f->idx[d] * (F->level[q] ? 2.0 : 1.0); }
+        for (int d = 0; d < 6; ++d) inst[q].bc[d] = f->bc[d];
+        inst[q].q0 = q;
+    }
+    auto* st = new StepperT<EOS>(p, g, inst.data(), (int)F->nq, e);
+    st->exch.reset(new AmrExchange(*F, NSTATE));
+    st->forest = std::move(F);
+    st->L.amr = st->exch.get();
+    return st;
+}
 }  // namespace
 
 extern "C" {
@@ -271,6 +294,13 @@ void* orc_stepper_create(const grmhd_params* p, const grmhd_grid* g, const grmhd
     if (p->eos_kind == GRMHD_EOS_IDEAL_GAS) return new StepperT<IdealGas>(p, g, inst, n_inst, make_ideal(p));
     return new StepperT<HybridPwpoly>(p, g, inst, n_inst, make_hybrid(p));
 }
+void* orc_stepper_create_forest(const grmhd_params* p, const grmhd_grid* g, const grmhd_forest* f) {
+    if (p->eos_kind == GRMHD_EOS_IDEAL_GAS) return make_forest_stepper(p, g, f, make_ideal(p));
+    return make_forest_stepper(p, g, f, make_hybrid(p));
+}
+// Level and coordinates of block q of a forest stepper (-1 if not a forest).
+int orc_forest_block(void* h, int64_t q, int32_t* level, int32_t coords[3]);
+
 void orc_stepper_destroy(void* h) { delete static_cast<StepperBase*>(h); }
 void orc_rk3_step(void* h, double dt, double* st, double* Bx, double* By, double* Bz, double* aux) {
     double* B[3] = {Bx, By, Bz};
@@ -295,5 +325,12 @@ double orc_find_dt(void* h, const double* st, const double* aux, double cfl) {
 }
 int64_t orc_fatal_count(void* h) { return static_cast<StepperBase*>(h)->fatal(); }
 int64_t orc_fofc_flagged(void* h) { return static_cast<StepperBase*>(h)->fofc_flagged(); }
+int orc_forest_block(void* h, int64_t q, int32_t* level, int32_t coords[3]) {
+    const Forest* F = static_cast<StepperBase*>(h)->forest_of();
+    if (!F || q < 0 || q >= F->nq) return -1;
+    *level = F->level[q];
+    for (int d = 0; d < 3; ++d) coords[d] = F->crd[q][d];
+    return 0;
+}
 
 }  // extern "C"

@@ -38,6 +38,10 @@ def olib(fast=False):
     L.orc_kastaun_f.restype = d
     L.orc_stepper_create.argtypes = [P, P, P, i32]
     L.orc_stepper_create.restype = P
+    L.orc_stepper_create_forest.argtypes = [P, P, P]
+    L.orc_stepper_create_forest.restype = P
+    L.orc_forest_block.argtypes = [P, i64, P, P]
+    L.orc_forest_block.restype = i32
     L.orc_stepper_destroy.argtypes = [P]
     L.orc_rk3_step.argtypes = [P, d, P, P, P, P, P]
     L.orc_advance_substep.argtypes = [P, d, d, P, P, P, P, P, P, P, P, P]
@@ -83,7 +87,12 @@ class Stepper:
     def __init__(self, wl):
         self.L = olib()
         self.wl = wl
-        self.h = self.L.orc_stepper_create(C.byref(wl.params), C.byref(wl.grid), wl.instances, len(wl.inst_specs))
+        if getattr(wl, "forest", None) is not None:
+            self.h = self.L.orc_stepper_create_forest(C.byref(wl.params), C.byref(wl.grid), C.byref(wl.forest))
+        else:
+            self.h = self.L.orc_stepper_create(C.byref(wl.params), C.byref(wl.grid), wl.instances,
+                                               len(wl.inst_specs))
+        assert self.h, "oracle stepper not created"
 
     def __del__(self):
         try:
@@ -109,6 +118,12 @@ class Stepper:
 
     def fatal(self):
         return self.L.orc_fatal_count(self.h)
+
+    def forest_block(self, q):
+        lev = np.zeros(1, dtype=np.int32)
+        crd = np.zeros(3, dtype=np.int32)
+        assert self.L.orc_forest_block(self.h, int(q), lev.ctypes.data, crd.ctypes.data) == 0
+        return int(lev[0]), tuple(int(x) for x in crd)
 
     def fofc_flagged(self):
         return self.L.orc_fofc_flagged(self.h)

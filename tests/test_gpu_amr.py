@@ -1,0 +1,180 @@
+"""GPU parity on two-level forests: the HIP coarse-fine ghost exchange,
+refluxing and RK3 (through the C ABI) against the CPU oracle on identical
+inputs, plus the invariants on the GPU's own results.
+
+Tolerances: ghost fill bit-exact (same operations in the same order);
+conserved variables and face B after a substep / RK3 step L-infinity relative
+<= 1e-12 per variable over interior cells; volume integrals of the conserved
+variables conserved to 5e-14 relative on periodic forests; div B and the
+coarse-fine face consistency at round-off (1e-14).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import grace_amd as ga
+from grace_amd import workloads as W
+import _oracle as O
+from test_oracle_amr import volume_integrals, interface_mismatch, div_b, ghost_mask
+from test_gpu_parity import to_dev, to_host, interior, face_interior, linf_rel
+
+pytestmark = pytest.mark.gpu
+
+FORESTS = {
+    "one": dict(nb=(2, 2, 2), refined=[0]),
+    "three": dict(nb=(2, 2, 2), refined=[0, 3, 5]),
+    "centre": dict(nb=(4, 4, 4), refined=None),
+    "outflow": dict(nb=(3, 2, 2), refined=[0, 4], bc=0),
+    "mixed_bc": dict(nb=(3, 3, 2), refined=[1, 3, 4, 10], bc=(1, 1, 0, 0, 1, 1)),
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def ctx_of(wl):
+    return ga.Context(wl.params, wl.grid, wl.instances, forest=wl.forest)
+
+
+@pytest.mark.parametrize("name", list(FORESTS))
+def test_gpu_forest_ghost_fill_bitexact(dev, name):
+    import torch
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.3, **FORESTS[name])
+    say(f"[forest ghosts] {name} nq={wl.grid.nq}")
+    rng = np.random.default_rng(3)
+    st, B, aux = wl.copy_arrays()
+    st[:, :7] *= 1.0 + 0.01 * rng.normal(size=st[:, :7].shape)
+    for b in B:
+        b *= 1.0 + 0.01 * rng.normal(size=b.shape)
+    sg, Bg = to_dev(st, dev), [to_dev(b, dev) for b in B]
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ctx = ctx_of(wl)
+    ctx.fill_ghosts(sg, Bg)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(sg), st)
+    for d in range(3):
+        assert np.array_equal(to_host(Bg[d]), B[d]), d
+
+
+def test_gpu_forest_block_layout(dev):
+    wl = W.forest_wave(n=8, ngz=4, **FORESTS["three"])
+    ctx = ctx_of(wl)
+    o = O.Stepper(wl)
+    for q in range(wl.grid.nq):
+        assert ctx.forest_block(q) == o.forest_block(q)
+
+
+@pytest.mark.parametrize("emf", [0, 1])
+def test_gpu_forest_substep(dev, emf):
+    import torch
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, emf=emf, **FORESTS["three"])
+    say(f"[forest substep] emf={emf}")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ctx = ctx_of(wl)
+    # both sides run their own c2p: it also sets the cell-centred E the GS EMFs read
+    sg, Bg, ag = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+    o.aux(st, B, aux)
+    ctx.compute_auxiliaries(sg, Bg, ag)
+    dt = 0.25 * wl.extra["dx"] / 2
+    new_o, nB_o = st.copy(), [b.copy() for b in B]
+    o.substep(dt, 1.0, new_o, st, nB_o, B, aux)
+    ng_, nBg = sg.clone(), [b.clone() for b in Bg]
+    dtd = torch.tensor([dt], dtype=torch.float64, device=dev)
+    ctx.advance_substep(dtd, 1.0, ng_, sg, nBg, Bg, ag)
+    torch.cuda.synchronize()
+    gn = to_host(ng_)
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
+        assert e < 1e-12, (v, e)
+    for d in range(3):
+        e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
+        assert e < 1e-12, (d, e)
+
+
+@pytest.mark.parametrize("name,emf", [("three", 0), ("three", 1), ("outflow", 0), ("mixed_bc", 0)])
+def test_gpu_forest_rk3(dev, name, emf):
+    import torch
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, emf=emf, **FORESTS[name])
+    say(f"[forest rk3] {name} emf={emf}")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ctx = ctx_of(wl)
+    sg, Bg, ag = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+    o.aux(st, B, aux)
+    ctx.compute_auxiliaries(sg, Bg, ag)
+    periodic = all(b == 1 for b in wl.forest.bc)
+    tot0 = volume_integrals(o, wl, st)
+    dtd = torch.zeros(1, dtype=torch.float64, device=dev)
+    for _ in range(2):
+        dt = o.dt(st, aux, 0.25)
+        ctx.compute_dt(sg, ag, 0.25, dtd)
+        torch.cuda.synchronize()
+        assert abs(float(dtd[0]) - dt) <= 1e-14 * dt
+        o.rk3(dt, st, B, aux)
+        dtd.fill_(dt)
+        ctx.rk3_step(dtd, sg, Bg, ag)
+    torch.cuda.synchronize()
+    gs, gB, gaux = to_host(sg), [to_host(b) for b in Bg], to_host(ag)
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, gs[:, v]), interior(wl.grid, st[:, v]))
+        assert e < 1e-12, (v, e)
+    for d in range(3):
+        e = linf_rel(face_interior(wl.grid, gB[d], d), face_interior(wl.grid, B[d], d))
+        assert e < 1e-12, (d, e)
+    for v in (ga.RHO, ga.PRESS, ga.ZVECX, ga.BX):
+        assert linf_rel(gaux[:, v], aux[:, v]) < 1e-12, v
+    n, ngz = wl.grid.nx, wl.grid.ngz
+    db = div_b(gB, wl.forest.idx, n, ngz)[:, ~ghost_mask(n, ngz)]
+    assert np.abs(db).max() < 1e-14, np.abs(db).max()
+    if periodic:
+        tot1 = volume_integrals(o, wl, gs)
+        rel = np.abs(tot1 - tot0) / np.maximum(np.abs(tot0), 1e-3)
+        assert rel.max() < 5e-14, rel
+        assert interface_mismatch(o, wl, gB) < 1e-14
+
+
+def test_gpu_forest_graph_matches_eager(dev):
+    import torch
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, **FORESTS["three"])
+    say("[forest graph] 3 replays vs eager")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ctx = ctx_of(wl)
+    sa, Ba, aa = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+    ctx.compute_auxiliaries(sa, Ba, aa)
+    sb, Bb, ab = sa.clone(), [b.clone() for b in Ba], aa.clone()
+    dtd = torch.tensor([0.25 * wl.extra["dx"] / 2], dtype=torch.float64, device=dev)
+    s = torch.cuda.Stream()
+    for it in range(3):
+        ctx.rk3_step(dtd, sa, Ba, aa)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            ctx.rk3_step(dtd, sb, Bb, ab, stream=s.cuda_stream, graph=True)
+        s.synchronize()
+        assert torch.equal(sa, sb), it
+        for d in range(3):
+            assert torch.equal(Ba[d], Bb[d]), it
+        assert torch.equal(aa, ab), it
+
+
+def test_forest_rejects_bad_input(dev):
+    wl = W.forest_wave(n=8, ngz=4, **FORESTS["one"])
+    p = wl.params
+    p.fofc = 1
+    with pytest.raises(RuntimeError):
+        ga.Context(p, wl.grid, wl.instances, forest=wl.forest)
+    p.fofc = 0
+    g = ga.Grid(8, 8, 8, 4, wl.grid.nq + 1)
+    with pytest.raises(RuntimeError):
+        ga.Context(p, g, wl.instances, forest=wl.forest)
+    g = ga.Grid(8, 8, 8, 3, wl.grid.nq)
+    with pytest.raises(RuntimeError):
+        ga.Context(p, g, wl.instances, forest=wl.forest)
