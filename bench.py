@@ -274,22 +274,26 @@ def c3_leg(ga, W, torch, dev, steps=3):
             "ms_per_step_default_shapes": 1e3 * t_default, "tuning": tune}
 
 
-def torus_leg(ga, W, torch, dev, nb=12, steps=5, warmup=2):
+def torus_leg(ga, W, torch, dev, steps=5, warmup=2):
     """Fishbone-Moncrief torus (r_in 6 M, r_max 12 M, Gamma 4/3, beta 100
-    loop) around a spin-0.9375 hole in the Kerr-Schild metric: one uniform
-    level of 16^3 blocks covering [-48, 48]^3 M (192^3 zones, the fine level
-    of the two-level forest everywhere), device CFL dt, graph RK3 steps with
-    the plain c2p and with the learned guess (network trained on the torus's
-    own cells)."""
-    log(f"[bench] torus leg: {nb}^3 blocks of 16^3")
-    wl = W.fm_torus(nb=(nb, nb, nb), n=16, extent=48.0)
+    loop) around a spin-0.9375 hole in the Kerr-Schild metric on a two-level
+    forest of 16^3 blocks: 6^3 coarse blocks on [-48, 48]^3 M (96^3, 1 M) with
+    the central 4 x 4 x 2 refined to 0.5 M (the spacing of a 192^3 grid);
+    coarse-fine prolongation / restriction and refluxing every substep, device
+    CFL dt, graph RK3 steps with the plain c2p and with the learned guess
+    (network trained on the torus's own cells)."""
+    wl = W.fm_torus_forest()
+    log(f"[bench] torus leg: two-level forest, {wl.grid.nq} blocks of 16^3")
     mlp = ga.Mlp()
-    out = {"workload": "Fishbone-Moncrief torus a=0.9375 r_in=6 r_max=12 beta=100, Kerr-Schild (Cowling), "
-                       "192^3 on [-48,48]^3 M, 1728 blocks of 16^3, ngz 3, WENO-Z + HLLE + UCT, rho_atm 1e-10",
+    out = {"workload": "Fishbone-Moncrief torus a=0.9375 r_in=6 r_max=12 beta=100, Kerr-Schild (Cowling), two-level "
+                       "forest on [-48,48]^3 M: 6^3 coarse blocks of 16^3 (1 M), central 4x4x2 refined (0.5 M, "
+                       "192^3-equivalent), ngz 4, WENO-Z + HLLE + UCT, minmod / Toth-Roe prolongation, reflux, "
+                       "rho_atm 1e-10",
+           "blocks": int(wl.grid.nq), "fine_blocks": int(wl.extra["n_fine"]),
            "static_metric_cache": True,
            "zones": wl.zones, "steps": steps}
     for mode in ("plain", "learned"):
-        ctx = ga.Context(wl.params, wl.grid, wl.instances)
+        ctx = ga.Context(wl.params, wl.grid, wl.instances, forest=wl.forest)
         st = torch.from_numpy(wl.state).to(dev)
         B = [torch.from_numpy(b).to(dev) for b in wl.B]
         aux = torch.from_numpy(wl.aux).to(dev)

@@ -89,7 +89,7 @@ _lib = None
 
 ABI_SYMBOLS = [
     "grmhd_default_params", "grmhd_ctx_create", "grmhd_ctx_destroy", "grmhd_last_error",
-    "grmhd_ctx_create_forest", "grmhd_forest_block", "grmhd_setup_forest_wave",
+    "grmhd_ctx_create_forest", "grmhd_forest_block", "grmhd_setup_forest_wave", "grmhd_setup_fm_torus_forest",
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
@@ -172,6 +172,8 @@ def lib():
     L.grmhd_setup_c2p_states.argtypes = [i64, C.c_uint64, d, d, P, P, P, P, P, P, P, P, P]
     L.grmhd_setup_fm_torus.argtypes = [C.POINTER(Grid), C.POINTER(Instance), i32, d, P, d, d, d, d, d, d, d,
                                        C.c_uint64, P, P, P, P, P, P]
+    L.grmhd_setup_fm_torus_forest.argtypes = [C.POINTER(Grid), C.POINTER(Forest), d, P, d, d, d, d, d, d, d,
+                                              C.c_uint64, P, P, P, P, P, P]
     L.grmhd_fm_torus_point.argtypes = [d, d, d, d, d, d, d, P]
     f32 = C.c_float
     L.grmhd_mlp_init_params.argtypes = [C.c_uint64, P]

@@ -248,6 +248,70 @@ FMPoint fm_point(const FMTorus& T, double lnh_in, double kappa, double gamma, do
     return P;
 }
 
+// Where a block lies: pos(d, loc, off) = org[d] + (loc - ngz + off) dx[d];
+// goff + loc - ngz is the block's cell index on its level (Ntot cells wide),
+// which with `a` (instance, or level of a forest) keys the noise.
+struct BlockGeom { double org[3], dx[3], idx[3]; int goff[3], Ntot[3], a; };
+
+bool geoms_of_instances(const grmhd_grid* g, const grmhd_instance* inst, int n_inst, const double lo[3],
+                        std::vector<BlockGeom>& out) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    out.resize((size_t)g->nq);
+    for (int64_t q = 0; q < g->nq; ++q) {
+        int a, b[3];
+        if (!find_block(inst, n_inst, q, a, b)) return false;
+        BlockGeom& G = out[(size_t)q];
+        G.a = a;
+        for (int d = 0; d < 3; ++d) {
+            G.idx[d] = inst[a].idx[d]; G.dx[d] = 1.0 / G.idx[d];
+            G.goff[d] = b[d] * n[d]; G.Ntot[d] = inst[a].nb[d] * n[d];
+            G.org[d] = lo[d] + G.goff[d] * G.dx[d];
+        }
+    }
+    return true;
+}
+
+// Block list of a two-level forest in grmhd_ctx_create_forest order.
+bool geoms_of_forest(const grmhd_grid* g, const grmhd_forest* f, const double lo[3], std::vector<BlockGeom>& out) {
+    const int64_t nc = (int64_t)f->nb[0] * f->nb[1] * f->nb[2];
+    if (g->nq != nc + 7 * (int64_t)f->n_refined) return false;
+    std::vector<char> ref((size_t)nc, 0);
+    for (int r = 0; r < f->n_refined; ++r) {
+        if (f->refined[r] < 0 || f->refined[r] >= nc) return false;
+        ref[f->refined[r]] = 1;
+    }
+    const int n[3] = {g->nx, g->ny, g->nz};
+    out.clear();
+    auto add = [&](int lev, const int crd[3]) {
+        BlockGeom G;
+        G.a = lev;
+        for (int d = 0; d < 3; ++d) {
+            G.idx[d] = f->idx[d] * (lev ? 2.0 : 1.0); G.dx[d] = 1.0 / G.idx[d];
+            G.goff[d] = crd[d] * n[d]; G.Ntot[d] = (f->nb[d] * n[d]) << lev;
+            G.org[d] = lo[d] + G.goff[d] * G.dx[d];
+        }
+        out.push_back(G);
+    };
+    for (int64_t b = 0; b < nc; ++b)
+        if (!ref[b]) {
+            const int crd[3] = {(int)(b % f->nb[0]), (int)((b / f->nb[0]) % f->nb[1]), (int)(b / ((int64_t)f->nb[0] * f->nb[1]))};
+            add(0, crd);
+        }
+    for (int64_t b = 0; b < nc; ++b)
+        if (ref[b]) {
+            const int cb[3] = {(int)(b % f->nb[0]), (int)((b / f->nb[0]) % f->nb[1]), (int)(b / ((int64_t)f->nb[0] * f->nb[1]))};
+            for (int ch = 0; ch < 8; ++ch) {
+                const int crd[3] = {2 * cb[0] + (ch & 1), 2 * cb[1] + ((ch >> 1) & 1), 2 * cb[2] + ((ch >> 2) & 1)};
+                add(1, crd);
+            }
+        }
+    return true;
+}
+
+int setup_fm_torus_impl(const grmhd_grid* g, const std::vector<BlockGeom>& geo, double gamma, double spin, double r_in,
+                        double r_max, double beta, double rho_atm, double p_atm, double noise, uint64_t seed,
+                        double* state, double* Bx, double* By, double* Bz, double* aux, double info[4]);
+
 }  // namespace
 
 extern "C" {
@@ -257,25 +321,8 @@ int grmhd_setup_forest_wave(const grmhd_grid* g, const grmhd_forest* f, double g
                             const double zvel[3], const double Bvec[3], double b_loop, double* state, double* Bx,
                             double* By, double* Bz, double* aux) {
     if (!g || !f || !(gamma > 1.0) || !(rho0 > 0) || !(p0 > 0)) return 1;
-    const int64_t nc = (int64_t)f->nb[0] * f->nb[1] * f->nb[2];
-    if (g->nq != nc + 7 * (int64_t)f->n_refined) return 1;
-    std::vector<char> ref((size_t)nc, 0);
-    for (int r = 0; r < f->n_refined; ++r) {
-        if (f->refined[r] < 0 || f->refined[r] >= nc) return 1;
-        ref[f->refined[r]] = 1;
-    }
-    // block order: coarse leaves, then 8 children per refined block
-    std::vector<int> lev, crd;
-    for (int64_t b = 0; b < nc; ++b)
-        if (!ref[b]) { lev.push_back(0); crd.push_back((int)(b % f->nb[0])); crd.push_back((int)((b / f->nb[0]) % f->nb[1])); crd.push_back((int)(b / ((int64_t)f->nb[0] * f->nb[1]))); }
-    for (int64_t b = 0; b < nc; ++b)
-        if (ref[b]) {
-            const int cb[3] = {(int)(b % f->nb[0]), (int)((b / f->nb[0]) % f->nb[1]), (int)(b / ((int64_t)f->nb[0] * f->nb[1]))};
-            for (int ch = 0; ch < 8; ++ch) {
-                lev.push_back(1);
-                for (int d = 0; d < 3; ++d) crd.push_back(2 * cb[d] + ((ch >> d) & 1));
-            }
-        }
+    std::vector<BlockGeom> geo;
+    if (!geoms_of_forest(g, f, lo, geo)) return 1;
     Idx X(g);
     double* B[3] = {Bx, By, Bz};
     const int n[3] = {g->nx, g->ny, g->nz};
@@ -286,14 +333,9 @@ int grmhd_setup_forest_wave(const grmhd_grid* g, const grmhd_forest* f, double g
     parallel_blocks(g->nq, [&](int64_t q0, int64_t q1, int) {
         for (int64_t q = q0; q < q1; ++q) {
             std::memset(aux + q * NAUX * X.ncell, 0, sizeof(double) * NAUX * X.ncell);
-            const double s = lev[q] ? 2.0 : 1.0;
-            double dx[3], org[3], idq[3];
-            for (int d = 0; d < 3; ++d) {
-                idq[d] = f->idx[d] * s;
-                dx[d] = 1.0 / idq[d];
-                org[d] = lo[d] + (double)crd[3 * q + d] * n[d] * dx[d];
-            }
-            auto pos = [&](int d, int l, double off) { return org[d] + (l - g->ngz + off) * dx[d]; };
+            const BlockGeom& G = geo[(size_t)q];
+            const double* idq = G.idx;
+            auto pos = [&](int d, int l, double off) { return G.org[d] + (l - g->ngz + off) * G.dx[d]; };
             for (int k = 0; k < X.N[2]; ++k)
                 for (int j = 0; j < X.N[1]; ++j)
                     for (int i = 0; i < X.N[0]; ++i) {
@@ -501,6 +543,28 @@ int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_
                          const double lo[3], double spin, double r_in, double r_max, double beta, double rho_atm,
                          double p_atm, double noise, uint64_t seed, double* state, double* Bx, double* By,
                          double* Bz, double* aux, double info[4]) {
+    std::vector<BlockGeom> geo;
+    if (!geoms_of_instances(g, inst, n_inst, lo, geo)) return 1;
+    return setup_fm_torus_impl(g, geo, gamma, spin, r_in, r_max, beta, rho_atm, p_atm, noise, seed, state, Bx, By, Bz,
+                               aux, info);
+}
+
+int grmhd_setup_fm_torus_forest(const grmhd_grid* g, const grmhd_forest* f, double gamma, const double lo[3],
+                                double spin, double r_in, double r_max, double beta, double rho_atm, double p_atm,
+                                double noise, uint64_t seed, double* state, double* Bx, double* By, double* Bz,
+                                double* aux, double info[4]) {
+    std::vector<BlockGeom> geo;
+    if (!g || !f || !geoms_of_forest(g, f, lo, geo)) return 1;
+    return setup_fm_torus_impl(g, geo, gamma, spin, r_in, r_max, beta, rho_atm, p_atm, noise, seed, state, Bx, By, Bz,
+                               aux, info);
+}
+
+}  // extern "C"
+
+namespace {
+int setup_fm_torus_impl(const grmhd_grid* g, const std::vector<BlockGeom>& geo, double gamma, double spin, double r_in,
+                        double r_max, double beta, double rho_atm, double p_atm, double noise, uint64_t seed,
+                        double* state, double* Bx, double* By, double* Bz, double* aux, double info[4]) {
     const double l = fm_ell(spin, r_max);
     if (!(l > 0.0)) return 2;
     FMTorus T{spin, l};
@@ -519,13 +583,11 @@ int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_
     parallel_blocks(g->nq, [&](int64_t qa, int64_t qb, int t) {
     double& pmax = pmax_t[t];
     for (int64_t q = qa; q < qb; ++q) {
-        int a, b[3];
-        if (!find_block(inst, n_inst, q, a, b)) { bad[t] = 1; return; }
-        const grmhd_instance& I = inst[a];
-        double dx[3];
-        int Ntot[3];
-        for (int d = 0; d < 3; ++d) { dx[d] = 1.0 / I.idx[d]; Ntot[d] = I.nb[d] * n[d]; }
-        auto pos = [&](int d, int loc, double off) { return lo[d] + (b[d] * n[d] + loc - g->ngz + off) * dx[d]; };
+        const BlockGeom& G = geo[(size_t)q];
+        const double* dx = G.dx;
+        const int* Ntot = G.Ntot;
+        const int a = G.a;
+        auto pos = [&](int d, int loc, double off) { return G.org[d] + (loc - g->ngz + off) * dx[d]; };
         for (int k = 0; k < X.N[2]; ++k)
             for (int j = 0; j < X.N[1]; ++j)
                 for (int i = 0; i < X.N[0]; ++i) {
@@ -538,7 +600,7 @@ int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_
                     S(ALP_) = al;
                     for (int d = 0; d < 3; ++d) S(BETAX_ + d) = be[d];
                     FMPoint P = fm_point(T, lnh_in, kappa, gamma, r_in, rho_atm, p_atm, x, y, z, be);
-                    const int gl[3] = {b[0] * n[0] + i - g->ngz, b[1] * n[1] + j - g->ngz, b[2] * n[2] + k - g->ngz};
+                    const int gl[3] = {G.goff[0] + i - g->ngz, G.goff[1] + j - g->ngz, G.goff[2] + k - g->ngz};
                     const int64_t key = (int64_t)(gl[0] + g->ngz) + (int64_t)(Ntot[0] + 2 * g->ngz) *
                                         ((gl[1] + g->ngz) + (int64_t)(Ntot[1] + 2 * g->ngz) * (gl[2] + g->ngz));
                     double p = P.p * (1.0 + noise * (2.0 * u01(seed, a, (uint64_t)key, 1) - 1.0));
@@ -615,16 +677,17 @@ int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_
     const double scale = (pmag > 0.0 && beta > 0.0) ? std::sqrt(pmax / (beta * pmag)) : 0.0;
     parallel_blocks(g->nq, [&](int64_t qa, int64_t qb, int) {
         for (int64_t q = qa; q < qb; ++q) {
-            int a, b[3];
-            find_block(inst, n_inst, q, a, b);
             for (int d = 0; d < 3; ++d)
                 for (int64_t f = 0; f < X.nface[d]; ++f) B[d][q * X.nface[d] + f] *= scale;
-            finish_cells(g, X, state, B, aux, inst[a].idx, q);
+            finish_cells(g, X, state, B, aux, geo[(size_t)q].idx, q);
         }
     });
     if (info) { info[0] = l; info[1] = kappa; info[2] = scale; info[3] = pmag * scale * scale; }
     return 0;
 }
+}  // namespace
+
+extern "C" {
 
 void grmhd_fm_torus_point(double spin, double gamma, double r_in, double r_max, double x, double y, double z,
                           double out[6]) {

@@ -158,6 +158,39 @@ def fm_torus(nb=(4, 4, 4), n=16, ngz=3, extent=48.0, spin=0.9375, r_in=6.0, r_ma
                                pmag_max=info[3]))
 
 
+def fm_torus_forest(nb=(6, 6, 6), n=16, ngz=4, extent=48.0, refined=None, spin=0.9375, r_in=6.0, r_max=12.0,
+                    beta=100.0, rho_atm=1e-10, p_atm=1e-14, gamma=4.0 / 3.0, noise=0.01, seed=SEED, emf=EMF_UCT,
+                    excision_lapse=0.55):
+    """The Fishbone-Moncrief torus of fm_torus on a two-level forest: nb coarse
+    blocks of n^3 cells on [-extent, extent]^3 (outflow faces), the coarse
+    blocks in `refined` refined once.  Default: 6^3 blocks of 16^3 (96^3 coarse)
+    with the central 4 x 4 x 2 coarse blocks -- the torus body -- refined, so
+    the finest level has the spacing of a 192^3 grid."""
+    if refined is None:
+        lo_b = [(nb[0] - 4) // 2, (nb[1] - 4) // 2, (nb[2] - 2) // 2]
+        refined = [(lo_b[0] + a) + nb[0] * ((lo_b[1] + b) + nb[1] * (lo_b[2] + e))
+                   for e in range(2) for b in range(4) for a in range(4)]
+    dx = 2.0 * extent / (nb[0] * n)
+    forest = make_forest(nb, (BC_OUTFLOW,) * 6, (1.0 / dx,) * 3, refined)
+    nq = nb[0] * nb[1] * nb[2] + 7 * forest.n_refined
+    grid = Grid(n, n, n, ngz, nq)
+    state, B, aux = alloc(grid)
+    lo = np.array([-0.5 * nb[d] * n * dx for d in range(3)], dtype=np.float64)
+    info = np.zeros(4)
+    rc = lib().grmhd_setup_fm_torus_forest(grid, forest, gamma, _p(lo), spin, r_in, r_max, beta, rho_atm, p_atm,
+                                           noise, seed, _p(state), _p(B[0]), _p(B[1]), _p(B[2]), _p(aux), _p(info))
+    assert rc == 0, rc
+    params = default_params(gamma=gamma, emf=emf, flat_metric=0)
+    params.rho_fl = rho_atm
+    params.temp_fl = p_atm / rho_atm
+    params.excision_lapse = excision_lapse
+    wl = Workload("fm_torus_forest", params, grid, [], state, B, aux, zones=nq * n ** 3,
+                  extra=dict(dx=dx, ell=info[0], kappa=info[1], bscale=info[2], pmag_max=info[3],
+                             n_fine=8 * forest.n_refined, equiv=nb[0] * n * 2))
+    wl.forest = forest
+    return wl
+
+
 def fm_torus_point(x, y, z, spin=0.9375, gamma=4.0 / 3.0, r_in=6.0, r_max=12.0):
     """Torus primitives at one Kerr-Schild point: rho, p, z^i, u^t."""
     out = np.zeros(6)

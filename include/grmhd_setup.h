@@ -51,6 +51,13 @@ int grmhd_setup_fm_torus(const grmhd_grid* g, const grmhd_instance* inst, int32_
                          const double lo[3], double spin, double r_in, double r_max, double beta, double rho_atm,
                          double p_atm, double noise, uint64_t seed, double* state, double* Bx, double* By,
                          double* Bz, double* aux, double info[4]);
+/* The same torus on a two-level forest (block order of
+ * grmhd_ctx_create_forest), the box starting at lo; noise keyed by level and
+ * cell index on the level. */
+int grmhd_setup_fm_torus_forest(const grmhd_grid* g, const grmhd_forest* f, double gamma, const double lo[3],
+                                double spin, double r_in, double r_max, double beta, double rho_atm, double p_atm,
+                                double noise, uint64_t seed, double* state, double* Bx, double* By, double* Bz,
+                                double* aux, double info[4]);
 /* The torus primitives at one Cartesian Kerr-Schild point (no atmosphere):
  * out = {rho, p, z^x, z^y, z^z, u^t}. */
 void grmhd_fm_torus_point(double spin, double gamma, double r_in, double r_max, double x, double y, double z,

@@ -178,3 +178,78 @@ def test_forest_rejects_bad_input(dev):
     g = ga.Grid(8, 8, 8, 3, wl.grid.nq)
     with pytest.raises(RuntimeError):
         ga.Context(p, g, wl.instances, forest=wl.forest)
+
+
+def _torus_forest_small():
+    # 2^3 coarse blocks of 8^3 on [-16, 16]^3 (hole, excised ring, torus
+    # inner edge and atmosphere), two of them refined
+    return W.fm_torus_forest(nb=(2, 2, 2), n=8, ngz=4, extent=16.0, refined=[0, 7])
+
+
+def test_gpu_torus_forest_rk3(dev):
+    """Curved Cowling metric on a forest: one RK3 step of the FM torus against
+    the oracle (metric of every ghost cell from the initial data), and the
+    same step with the static-metric derivative cache bit-identical."""
+    import torch
+    wl = _torus_forest_small()
+    say(f"[torus forest] nq={wl.grid.nq}")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    dt = 0.25 * wl.extra["dx"] / 2
+    o.rk3(dt, st, B, aux)
+    res = []
+    for cache in (False, True):
+        ctx = ctx_of(wl)
+        sg, Bg, ag = to_dev(wl.state, dev), [to_dev(b, dev) for b in wl.B], to_dev(wl.aux, dev)
+        if cache:
+            ctx.set_static_metric(sg)
+        dtd = torch.tensor([dt], dtype=torch.float64, device=dev)
+        ctx.rk3_step(dtd, sg, Bg, ag)
+        torch.cuda.synchronize()
+        res.append((to_host(sg), [to_host(b) for b in Bg], to_host(ag)))
+    gs, gB, gaux = res[0]
+    for arr in [gs, gaux] + gB:
+        assert np.isfinite(arr).all()
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, gs[:, v]), interior(wl.grid, st[:, v]))
+        assert e < 1e-12, (v, e)
+    for d in range(3):
+        e = linf_rel(face_interior(wl.grid, gB[d], d), face_interior(wl.grid, B[d], d))
+        assert e < 1e-12, (d, e)
+    for v in (ga.RHO, ga.PRESS, ga.ZVECX):
+        assert linf_rel(gaux[:, v], aux[:, v]) < 1e-11, v
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][2], res[1][2])
+    for d in range(3):
+        assert np.array_equal(res[0][1][d], res[1][1][d])
+
+
+def test_gpu_torus_forest_learned_c2p(dev):
+    """Learned c2p guess in the forest's RK3 cell passes (network trained on
+    the forest's cells): agrees with the plain step to 1e-9, guess used in
+    most cells."""
+    import torch
+    wl = W.fm_torus_forest(nb=(2, 2, 2), n=16, ngz=4, extent=24.0, refined=[0, 7])
+    say(f"[torus forest learned] nq={wl.grid.nq}")
+    dt = torch.tensor([0.25 * wl.extra["dx"] / 2], dtype=torch.float64, device=dev)
+    out = {}
+    mlp = ga.Mlp()
+    for mode in ("plain", "learned"):
+        ctx = ctx_of(wl)
+        st, B, aux = to_dev(wl.state, dev), [to_dev(b, dev) for b in wl.B], to_dev(wl.aux, dev)
+        if mode == "learned":
+            rep = mlp.fit_cells(ctx, st, B, max_states=1 << 15, opts=ga.Mlp.default_fit_opts(epochs=40))
+            say(f"  fit report {rep}")
+            ctx.set_learned_c2p(mlp)
+        ctx.rk3_step(dt, st, B, aux)
+        torch.cuda.synchronize()
+        res = (to_host(st), [to_host(b) for b in B], to_host(aux))
+        ctx.compute_auxiliaries(st, B, aux)
+        torch.cuda.synchronize()
+        out[mode] = res + (ctx.c2p_stats(),)
+    (sp, Bp, ap, _), (sl, Bl, al, stl) = out["plain"], out["learned"]
+    for v in range(7):
+        assert linf_rel(interior(wl.grid, sl[:, v]), interior(wl.grid, sp[:, v])) < 1e-9, v
+    for d in range(3):
+        assert linf_rel(face_interior(wl.grid, Bl[d], d), face_interior(wl.grid, Bp[d], d)) < 1e-9, d
+    assert np.array_equal(al[:, ga.C2P_ERR], ap[:, ga.C2P_ERR])
+    assert stl["n_guess_used"] > 0.5 * stl["n_cells"]

@@ -12,6 +12,7 @@ vector potential are divergence free to round-off; the field is scaled to
 max p / max(b^2/2) = beta.  A dropped term of ln h, a wrong sign of the
 frame-dragging velocity or a wrong l breaks one of them."""
 import numpy as np
+import pytest
 
 import grace_amd as ga
 from grace_amd import workloads as W
@@ -102,3 +103,41 @@ def test_grid_data_divergence_free_and_beta():
     # atmosphere outside, torus inside
     rho = wl.aux[:, ga.RHO][sl]
     assert rho.min() == 1e-10 and 0.5 < rho.max() <= 1.0
+
+
+def test_torus_forest_fully_refined_equals_uniform():
+    """A forest with every coarse block refined holds, block for block, the
+    uniform torus of the fine spacing (noise off: its key includes the level)."""
+    wf = W.fm_torus_forest(nb=(2, 2, 2), n=8, ngz=4, extent=16.0, refined=list(range(8)), noise=0.0)
+    wu = W.fm_torus(nb=(4, 4, 4), n=8, ngz=4, extent=16.0, noise=0.0)
+    # forest block q = 8 c + child: fine coords 2 cb + child bits; uniform q = bi + 4 (bj + 4 bk)
+    for q in range(wf.grid.nq):
+        c, ch = divmod(q, 8)
+        cb = (c % 2, (c // 2) % 2, c // 4)
+        fc = [2 * cb[d] + ((ch >> d) & 1) for d in range(3)]
+        qu = fc[0] + 4 * (fc[1] + 4 * fc[2])
+        np.testing.assert_allclose(wf.state[q], wu.state[qu], rtol=1e-14, atol=1e-300)
+        np.testing.assert_allclose(wf.aux[q], wu.aux[qu], rtol=1e-14, atol=1e-300)
+        for d in range(3):
+            np.testing.assert_allclose(wf.B[d][q], wu.B[d][qu], rtol=1e-13, atol=1e-18)
+    assert wf.extra["bscale"] == pytest.approx(wu.extra["bscale"], rel=1e-14)
+
+
+def test_torus_forest_levels_agree_on_the_metric():
+    """Coarse and fine blocks sample the same Kerr-Schild metric: a coarse
+    cell's value equals the fine value at the same centre only up to the
+    half-cell offset, but the analytic metric at a shared point is identical."""
+    wf = W.fm_torus_forest(nb=(2, 2, 2), n=8, ngz=4, extent=16.0, refined=[0], noise=0.0)
+    assert wf.grid.nq == 15
+    assert np.all(np.isfinite(wf.state[:, :7])) and np.all(np.isfinite(wf.aux[:, 0]))
+    # divergence of the face field in every interior cell of every block
+    n, g = 8, 4
+    for q in range(wf.grid.nq):
+        lev = 0 if q < 7 else 1
+        idx = (1.0 / wf.extra["dx"]) * (2 if lev else 1)
+        Bx, By, Bz = (b[q, 0] for b in wf.B)
+        div = (Bx[g:g + n, g:g + n, g + 1:g + n + 1] - Bx[g:g + n, g:g + n, g:g + n]) * idx + \
+              (By[g:g + n, g + 1:g + n + 1, g:g + n] - By[g:g + n, g:g + n, g:g + n]) * idx + \
+              (Bz[g + 1:g + n + 1, g:g + n, g:g + n] - Bz[g:g + n, g:g + n, g:g + n]) * idx
+        scale = max(np.abs(b[q]).max() for b in wf.B) * idx + 1e-300
+        assert np.abs(div).max() <= 1e-13 * scale, (q, np.abs(div).max(), scale)
