@@ -401,6 +401,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--only-torus", action="store_true",
+                    help="only run the two-level-forest torus leg and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -414,6 +416,9 @@ def main():
     from grace_amd import workloads as W
 
     dev = torch.device("cuda:0")
+    if args.only_torus:
+        print(json.dumps(torus_leg(ga, W, torch, dev, steps=args.steps, warmup=args.warmup), default=float), flush=True)
+        return
     if args.copy_probe:
         a = torch.ones(1 << 27, dtype=torch.float64, device=dev)   # 1 GiB
         b = torch.empty_like(a)

@@ -192,16 +192,9 @@ GHD double ff_fine_even(const FForest& F, const double* const B[3], int d, int f
 // direction d, fine index 2 C[d] + 1, offsets o0, o1 in {0,1} along the two
 // other axes (lower axis first).  The 24 fine faces on the cell's six coarse
 // faces are gathered once.
-GHD double ff_fine_mid(const FForest& F, const double* const B[3], int d, const int C[3], int o0, int o1) {
-    const int f0[3] = {2 * C[0], 2 * C[1], 2 * C[2]};
-    double u[2][2][2], v[2][2][2], w[2][2][2];   // [normal side][first tangential][second tangential]
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int c = 0; c < 2; ++c) {
-                { const int t[3] = {0, f0[1] + b, f0[2] + c}; u[a][b][c] = ff_fine_even(F, B, 0, f0[0] + 2 * a, t); }
-                { const int t[3] = {f0[0] + b, 0, f0[2] + c}; v[a][b][c] = ff_fine_even(F, B, 1, f0[1] + 2 * a, t); }
-                { const int t[3] = {f0[0] + b, f0[1] + c, 0}; w[a][b][c] = ff_fine_even(F, B, 2, f0[2] + 2 * a, t); }
-            }
+// Toth-Roe interior face from the 24 fine faces on the coarse cell's faces:
+// u/v/w[normal side][first tangential][second tangential].
+GHD double ff_toth_roe(const double u[2][2][2], const double v[2][2][2], const double w[2][2][2], int d, int o0, int o1) {
     // u(i_f+2ii, j_f+jj, k_f+kk) = u[ii][jj][kk]; v(i_f+ii, j_f+2jj, k_f+kk) = v[jj][ii][kk];
     // w(i_f+ii, j_f+jj, k_f+2kk) = w[kk][ii][jj]
     double Uxx = 0, Vyy = 0, Wzz = 0, Uxyz = 0, Vxyz = 0, Wxyz = 0;
@@ -222,6 +215,18 @@ GHD double ff_fine_mid(const FForest& F, const double* const B[3], int d, const 
     if (d == 0) return 0.5 * (u[0][o0][o1] + u[1][o0][o1]) + Uxx + s1 * Vxyz + s0 * Wxyz;
     if (d == 1) return 0.5 * (v[0][o0][o1] + v[1][o0][o1]) + Vyy + s0 * Wxyz + s1 * Uxyz;
     return 0.5 * (w[0][o0][o1] + w[1][o0][o1]) + Wzz + s1 * Uxyz + s0 * Vxyz;
+}
+GHD double ff_fine_mid(const FForest& F, const double* const B[3], int d, const int C[3], int o0, int o1) {
+    const int f0[3] = {2 * C[0], 2 * C[1], 2 * C[2]};
+    double u[2][2][2], v[2][2][2], w[2][2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                { const int t[3] = {0, f0[1] + b, f0[2] + c}; u[a][b][c] = ff_fine_even(F, B, 0, f0[0] + 2 * a, t); }
+                { const int t[3] = {f0[0] + b, 0, f0[2] + c}; v[a][b][c] = ff_fine_even(F, B, 1, f0[1] + 2 * a, t); }
+                { const int t[3] = {f0[0] + b, f0[1] + c, 0}; w[a][b][c] = ff_fine_even(F, B, 2, f0[2] + 2 * a, t); }
+            }
+    return ff_toth_roe(u, v, w, d, o0, o1);
 }
 
 // ---- the coarse-fine ghost elements (kernel bodies, also run on the host) ----
@@ -280,6 +285,56 @@ GHD void ff_amr_face(const FForest& F, int q, int fdoff, double* B0, double* B1,
     Bd[(int64_t)q * F.nface[fd] + off] = val;
 }
 
+// Interior (odd) fine ghost face after the first pass has written every face
+// lying on a coarse face: the 24 faces around its coarse cell are then in the
+// block's own arrays (existing fine data, same-level copies or first-pass
+// values, each equal to ff_fine_even at that position), so they are read
+// instead of re-derived.  A face whose position was clamped across an outflow
+// face takes the full derivation (the block's arrays there hold the clamped
+// copies, not the faces around the clamped coarse cell).
+GHD void ff_amr_face_mid(const FForest& F, int q, int fdoff, double* B0, double* B1, double* B2) {
+    const int fd = fdoff >> 24, off = fdoff & 0xFFFFFF;
+    const int E0 = F.N[0] + (fd == 0), E1 = F.N[1] + (fd == 1);
+    const int loc[3] = {off % E0, (off / E0) % E1, off / (E0 * E1)};
+    int lev, g[3];
+    ff_global(F, q, fd, loc, &lev, g);
+    const double* const B[3] = {B0, B1, B2};
+    const int4 b = F.blk[q];
+    const int cr[3] = {b.y, b.z, b.w};
+    bool clamped = false;
+    for (int d = 0; d < 3; ++d) {
+        const int gg = cr[d] * F.n[d] + loc[d] - F.ng;
+        const int N = ff_ntot(F, lev, d);
+        clamped |= (gg < 0 || gg > N - (d == fd ? 0 : 1)) && F.bc[2 * d + (gg < 0 ? 0 : 1)] != 1;
+    }
+    int C[3], o[2], m = 0;
+    for (int d = 0; d < 3; ++d) {
+        C[d] = d == fd ? (g[d] - 1) >> 1 : g[d] >> 1;
+        if (d != fd) o[m++] = g[d] & 1;
+    }
+    double val;
+    if (clamped) {
+        val = ff_fine_mid(F, B, fd, C, o[0], o[1]);
+    } else {
+        int lc[3];   // local index of the coarse cell's lower fine corner
+        for (int d = 0; d < 3; ++d) lc[d] = loc[d] - (d == fd ? 1 : (g[d] & 1));
+        double u[2][2][2], v[2][2][2], w[2][2][2];
+        const double* P0 = B0 + (int64_t)q * F.nface[0];
+        const double* P1 = B1 + (int64_t)q * F.nface[1];
+        const double* P2 = B2 + (int64_t)q * F.nface[2];
+        for (int a = 0; a < 2; ++a)
+            for (int bb = 0; bb < 2; ++bb)
+                for (int c = 0; c < 2; ++c) {
+                    u[a][bb][c] = P0[ff_fidx(F, 0, lc[0] + 2 * a, lc[1] + bb, lc[2] + c)];
+                    v[a][bb][c] = P1[ff_fidx(F, 1, lc[0] + bb, lc[1] + 2 * a, lc[2] + c)];
+                    w[a][bb][c] = P2[ff_fidx(F, 2, lc[0] + bb, lc[1] + c, lc[2] + 2 * a)];
+                }
+        val = ff_toth_roe(u, v, w, fd, o[0], o[1]);
+    }
+    double* Bd = fd == 0 ? B0 : (fd == 1 ? B1 : B2);
+    Bd[(int64_t)q * F.nface[fd] + off] = val;
+}
+
 // ---- kernels ---------------------------------------------------------------------
 // Entries {block, local cell offset}.
 __global__ __launch_bounds__(256) void k_amr_cells(const int2* __restrict__ ent, int n, double* st, int nv, int v0,
@@ -289,13 +344,21 @@ __global__ __launch_bounds__(256) void k_amr_cells(const int2* __restrict__ ent,
     const int2 t = ent[e];
     ff_amr_cell(F, t.x, t.y, st, nv, v0, vc);
 }
-// Entries {block, fd << 24 | local face offset}.
+// Entries {block, fd << 24 | local face offset}: first pass (faces on coarse
+// faces and restricted coarse ghosts), then the interior fine faces.
 __global__ __launch_bounds__(256) void k_amr_faces(const int2* __restrict__ ent, int n, double* B0, double* B1,
                                                    double* B2, FForest F) {
     const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= n) return;
     const int2 t = ent[e];
     ff_amr_face(F, t.x, t.y, B0, B1, B2);
+}
+__global__ __launch_bounds__(256) void k_amr_faces_mid(const int2* __restrict__ ent, int n, double* B0, double* B1,
+                                                       double* B2, FForest F) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n) return;
+    const int2 t = ent[e];
+    ff_amr_face_mid(F, t.x, t.y, B0, B1, B2);
 }
 
 // Refluxing (refluxing.cpp:547-603): coarse face flux <- area average of the

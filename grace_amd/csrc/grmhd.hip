@@ -1323,7 +1323,8 @@ struct grmhd_ctx {
     int32_t* cmap_d = nullptr;
     int4* fblk_d = nullptr;
     int2* amr_cells = nullptr; int64_t n_amr_cells = 0;
-    int2* amr_faces = nullptr; int64_t n_amr_faces = 0;
+    int2* amr_faces = nullptr; int64_t n_amr_faces = 0;           // first pass: faces on coarse faces
+    int2* amr_faces_mid = nullptr; int64_t n_amr_faces_mid = 0;   // second pass: interior fine faces
     RefluxFace* rflux = nullptr; int64_t n_rflux = 0;
     RefluxEdge* remf = nullptr; int64_t n_remf = 0;
     int64_t n_own_gather = 0;                 // own-c2p ghosts that copy a clamped source
@@ -1399,7 +1400,7 @@ static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
-    F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->rflux); F(c->remf);
+    F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->amr_faces_mid); F(c->rflux); F(c->remf);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1570,7 +1571,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                 src[(size_t)(q * per + x)] = make_int2(qs, so);
             }
         }
-        std::vector<int2> own, amr_c, amr_f;
+        std::vector<int2> own, amr_c, amr_f, amr_fm;
         std::vector<int4> owng;
         for (int64_t q = 0; q < G.nq; ++q)
             for (int64_t x = 0; x < per; ++x) {
@@ -1578,7 +1579,11 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                 if (x >= c->ncg) {
                     if (sr.x & GSRC_AMR) {
                         const int fd = x < c->ncg + c->nfg[0] ? 0 : (x < c->ncg + c->nfg[0] + c->nfg[1] ? 1 : 2);
-                        amr_f.push_back(make_int2((int)q, (fd << 24) | lst[x]));
+                        const int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
+                        const int loc[3] = {lst[x] % E0, (lst[x] / E0) % E1, lst[x] / (E0 * E1)};
+                        int lev, gpos[3];
+                        ff_global(ffh, (int)q, fd, loc, &lev, gpos);
+                        (lev == 1 && (gpos[fd] & 1) ? amr_fm : amr_f).push_back(make_int2((int)q, (fd << 24) | lst[x]));
                     }
                     continue;
                 }
@@ -1590,6 +1595,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         c->n_own_gather = (int64_t)owng.size();
         c->n_amr_cells = (int64_t)amr_c.size();
         c->n_amr_faces = (int64_t)amr_f.size();
+        c->n_amr_faces_mid = (int64_t)amr_fm.size();
         auto upload = [&](auto** dst, const auto& v, const char* what) -> bool {
             if (v.empty()) return true;
             const size_t bytes = sizeof(v[0]) * v.size();
@@ -1602,6 +1608,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         if (!upload(&c->own_gather, owng, "own gather")) return fail("own gather", e);
         if (!upload(&c->amr_cells, amr_c, "amr cells")) return fail("coarse-fine cells", e);
         if (!upload(&c->amr_faces, amr_f, "amr faces")) return fail("coarse-fine faces", e);
+        if (!upload(&c->amr_faces_mid, amr_fm, "amr faces mid")) return fail("coarse-fine interior faces", e);
         if (fin) {
             std::vector<RefluxFace> rf;
             std::vector<RefluxEdge> re;
@@ -1815,8 +1822,12 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
         k_gather_faces<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_faces, 0, s>>>(
             c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0], (int)c->nfg[1], (int)c->nfg[2], wpb,
             a.B[0], a.B[1], a.B[2], G.nface[0], G.nface[1], G.nface[2]);
+        // the second pass reads what the copies and the first pass wrote
         if (c->n_amr_faces > 0)
             k_amr_faces<<<nblk(c->n_amr_faces), 256, 0, s>>>(c->amr_faces, (int)c->n_amr_faces, a.B[0], a.B[1], a.B[2], c->ff);
+        if (c->n_amr_faces_mid > 0)
+            k_amr_faces_mid<<<nblk(c->n_amr_faces_mid), 256, 0, s>>>(c->amr_faces_mid, (int)c->n_amr_faces_mid, a.B[0],
+                                                                     a.B[1], a.B[2], c->ff);
     }
     if (a.cell_mode == 0) return;
     Scope sc(c, "ghost", s);
