@@ -59,6 +59,10 @@ public:
         : grid_(g) {
         check(grmhd_ctx_create(&p, &g, inst.data(), (int32_t)inst.size(), &ctx_), "grmhd_ctx_create");
     }
+    // A two-level forest (coarse-fine ghosts and refluxing in every call).
+    grmhd_evolution(const grmhd_params& p, const grmhd_grid& g, const grmhd_forest& f) : grid_(g) {
+        check(grmhd_ctx_create_forest(&p, &g, &f, &ctx_), "grmhd_ctx_create_forest");
+    }
     ~grmhd_evolution() { grmhd_ctx_destroy(ctx_); }
     grmhd_evolution(const grmhd_evolution&) = delete;
     grmhd_evolution& operator=(const grmhd_evolution&) = delete;
@@ -120,6 +124,11 @@ public:
                            hipStream_t stream = nullptr) {
         check(grmhd_prims_to_conservs_batch(ctx_, n, prims_soa, metric_soa, cons_soa, (grmhd_stream_t)stream),
               "prims_to_conservs");
+    }
+
+    // Level (0 coarse, 1 fine) and coordinates at that level of block q of a forest.
+    void forest_block(int64_t q, int32_t& level, int32_t coords[3]) const {
+        check(grmhd_forest_block(ctx_, q, &level, coords), "forest_block");
     }
 
     grmhd_c2p_stats_t c2p_stats(hipStream_t stream = nullptr) {

@@ -210,10 +210,55 @@ static void test_rejected_options_throw() {
     report("rejected_options_throw", threw && msg.find("UCT") != std::string::npos, msg.c_str());
 }
 
+// Two-level forest through the C++ layer: the block order (coarse leaves, then
+// the 8 Morton children of each refined block) and the reference's
+// constraints (cubic cells) enforced with an exception.
+static void test_forest_layout_and_checks() {
+    grmhd_params p;
+    grmhd_default_params(&p);
+    grmhd_grid g{8, 8, 8, 4, 8 + 7 * 2};
+    const int32_t refined[2] = {7, 0};
+    grmhd_forest f;
+    std::memset(&f, 0, sizeof(f));
+    f.nb[0] = f.nb[1] = f.nb[2] = 2;
+    for (int d = 0; d < 6; ++d) f.bc[d] = GRMHD_BC_PERIODIC;
+    f.idx[0] = f.idx[1] = f.idx[2] = 16.0;
+    f.n_refined = 2;
+    f.refined = refined;
+    bool ok = true;
+    std::string why;
+    try {
+        grmhd_evolution ev(p, g, f);
+        int32_t lev, c[3];
+        ev.forest_block(0, lev, c);
+        ok &= lev == 0 && c[0] == 1 && c[1] == 0 && c[2] == 0;
+        ev.forest_block(6, lev, c);
+        ok &= lev == 1 && c[0] == 0 && c[1] == 0 && c[2] == 0;     // child 0 of coarse block 0
+        ev.forest_block(13, lev, c);
+        ok &= lev == 1 && c[0] == 1 && c[1] == 1 && c[2] == 1;     // child 7 of coarse block 0
+        ev.forest_block(21, lev, c);
+        ok &= lev == 1 && c[0] == 3 && c[1] == 3 && c[2] == 3;     // child 7 of coarse block 7
+        if (!ok) why = "block order";
+    } catch (const error& e) {
+        ok = false;
+        why = e.what();
+    }
+    f.idx[2] = 8.0;   // anisotropic cells
+    bool threw = false;
+    try {
+        grmhd_evolution ev(p, g, f);
+    } catch (const error& e) {
+        threw = std::string(e.what()).find("cubic") != std::string::npos;
+    }
+    if (!threw && why.empty()) why = "anisotropic forest accepted";
+    report("forest_layout_and_checks", ok && threw, why.c_str());
+}
+
 int main() {
     test_c2p_minkowski_hydro_sweep();
     test_blast_substep_against_oracle();
     test_rejected_options_throw();
+    test_forest_layout_and_checks();
     std::printf("%d failure(s)\n", g_fail);
     return g_fail;
 }

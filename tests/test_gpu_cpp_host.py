@@ -24,4 +24,4 @@ def test_gpu_cpp_host_api():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=100)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("[PASS]") == 3
+    assert r.stdout.count("[PASS]") == 4
