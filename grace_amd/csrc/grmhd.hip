@@ -406,16 +406,18 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
                                                 const double* __restrict__ ef2, const double* __restrict__ ec,
                                                 const double* __restrict__ f0, const double* __restrict__ f1,
                                                 const double* __restrict__ f2, double* __restrict__ emf, int wpb,
-                                                const uint8_t* __restrict__ only) {
-    // wpb workgroups per block over its (n+1)^3 sites (see wg_pos); only !=
-    // null: FOFC correction of the tagged edges (evolve.cpp:780-808)
-    const int R0 = G.n[0] + 1, R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+                                                const uint8_t* __restrict__ only, int wide) {
+    // wpb workgroups per block over its (n+1+2 wide)^3 sites from ng - wide
+    // (see wg_pos); only != null: FOFC correction of the tagged edges
+    // (evolve.cpp:780-808); wide = 1: one ghost layer of edges more on every
+    // side (grmhd_params.fofc_ghost_emf)
+    const int R0 = G.n[0] + 1 + 2 * wide, R1 = G.n[1] + 1 + 2 * wide, R2 = G.n[2] + 1 + 2 * wide;
     const WgPos wp = wg_pos(wpb);
     const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
     const int64_t q = wp.q;
     const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
-    const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
+    const int i = G.ng - wide + rxy % R0, j = G.ng - wide + rxy / R0, k = G.ng - wide + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2], nc = G.ncell;
     auto EF = [&](int d, int c, int ii, int jj, int kk) {
         const double* base = d == 0 ? ef0 : (d == 1 ? ef1 : ef2);
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
     (void)nfx; (void)nfy; (void)nfz;
     double* E = emf + q * 3 * G.nedge;
     const int64_t eo = eidx(G, i, j, k);
-    if (i < G.n[0] + G.ng && (!only || only[(q * 3 + 0) * G.nedge + eo])) {
+    if (i < G.n[0] + G.ng + wide && (!only || only[(q * 3 + 0) * G.nedge + eo])) {
         double Exzf = EF(2, 0, i, j, k), Exzfm = EF(2, 0, i, j - 1, k), Exyf = EF(1, 0, i, j, k), Exyfm = EF(1, 0, i, j, k - 1);
         double NE = EC(0, i, j, k), NW = EC(0, i, j - 1, k), SE = EC(0, i, j, k - 1), SW = EC(0, i, j - 1, k - 1);
         double Eavg = 0.25 * (Exzf + Exzfm + Exyf + Exyfm);
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
         double dW = (1. - Szm) * (Exyf - NW) + (1. + Szm) * (Exyfm - SW);
         E[eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
     }
-    if (j < G.n[1] + G.ng && (!only || only[(q * 3 + 1) * G.nedge + eo])) {
+    if (j < G.n[1] + G.ng + wide && (!only || only[(q * 3 + 1) * G.nedge + eo])) {
         double Eyzf = EF(2, 1, i, j, k), Eyzfm = EF(2, 1, i - 1, j, k), Eyxf = EF(0, 0, i, j, k), Eyxfm = EF(0, 0, i, j, k - 1);
         double NE = EC(1, i, j, k), NW = EC(1, i - 1, j, k), SE = EC(1, i, j, k - 1), SW = EC(1, i - 1, j, k - 1);
         double Eavg = 0.25 * (Eyzf + Eyzfm + Eyxf + Eyxfm);
@@ -455,7 +457,7 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
         double dW = (1. - Szm) * (Eyxf - NW) + (1. + Szm) * (Eyxfm - SW);
         E[G.nedge + eo] = Eavg + (1. / 8.) * (dS - dN) + (1. / 8.) * (dW - dE);
     }
-    if (k < G.n[2] + G.ng && (!only || only[(q * 3 + 2) * G.nedge + eo])) {
+    if (k < G.n[2] + G.ng + wide && (!only || only[(q * 3 + 2) * G.nedge + eo])) {
         double Ezyf = EF(1, 1, i, j, k), Ezyfm = EF(1, 1, i - 1, j, k), Ezxf = EF(0, 1, i, j, k), Ezxfm = EF(0, 1, i, j - 1, k);
         double NE = EC(2, i, j, k), NW = EC(2, i - 1, j, k), SE = EC(2, i, j - 1, k), SW = EC(2, i - 1, j - 1, k);
         double Eavg = 0.25 * (Ezyf + Ezyfm + Ezxf + Ezxfm);
@@ -1364,7 +1366,7 @@ extern "C" void grmhd_default_params(grmhd_params* p) {
     p->recon = GRMHD_RECON_WENOZ; p->riemann = GRMHD_RIEMANN_HLLE; p->emf = GRMHD_EMF_UCT;
     p->thermo_recon = GRMHD_THERMO_TEMP; p->fd_order = 6; p->rusanov_c_limit = 0.0;
     p->flat_metric = 0;
-    p->fofc = 0; p->fofc_dmp_enable = 1; p->fofc_dmp_M = 1.2;
+    p->fofc = 0; p->fofc_dmp_enable = 1; p->fofc_dmp_M = 1.2; p->fofc_ghost_emf = 0;
 }
 
 static void build_eos(grmhd_ctx* c) {
@@ -1436,6 +1438,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     if (p->fofc && p->emf != GRMHD_EMF_GS) { set_err("FOFC is not supported with UCT"); return GRMHD_ERR_UNSUPPORTED; }
     if (p->fofc && g->ngz < 4) { set_err("FOFC requires ngz >= 4"); return GRMHD_ERR_INVALID; }
     if (p->fofc && !(p->fofc_dmp_M > 0)) { set_err("fofc_dmp_M must be > 0"); return GRMHD_ERR_INVALID; }
+    if (p->fofc_ghost_emf != 0 && p->fofc_ghost_emf != 1) { set_err("fofc_ghost_emf must be 0 or 1"); return GRMHD_ERR_INVALID; }
     // instance / block tables
     std::vector<DevBlk> blk(g->nq, DevBlk{-1, {0, 0, 0}});
     for (int a = 0; a < n_inst; ++a) {
@@ -1666,7 +1669,6 @@ extern "C" grmhd_status grmhd_ctx_create_forest(const grmhd_params* p, const grm
     if ((int64_t)(g->nx + 2 * g->ngz + 1) * (g->ny + 2 * g->ngz + 1) * (g->nz + 2 * g->ngz + 1) >= (1 << 24)) {
         set_err("forest: blocks too large"); return GRMHD_ERR_INVALID;
     }
-    if (p->fofc) { set_err("FOFC on a forest is not supported"); return GRMHD_ERR_UNSUPPORTED; }
     ForestIn fin;
     for (int d = 0; d < 3; ++d) fin.nb[d] = f->nb[d];
     for (int d = 0; d < 6; ++d) fin.bc[d] = f->bc[d];
@@ -1784,10 +1786,12 @@ static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
     Scope sc(c, "emf", s);
     const DevGrid& G = c->G;
     if (c->P.emf == GRMHD_EMF_GS) {
-        site_launch(G, c->tune.emf, [&](dim3 g, int wpb) {
-            k_emf_gs<<<g, c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
-                                       c->flux[2], c->emf, wpb, nullptr);
-        });
+        const int wide = (c->P.fofc && c->P.fofc_ghost_emf) ? 1 : 0;
+        const int sites = (G.n[0] + 1 + 2 * wide) * (G.n[1] + 1 + 2 * wide) * (G.n[2] + 1 + 2 * wide);
+        const int wpb = (int)nblk_bs(sites, c->tune.emf);
+        k_emf_gs<<<dim3((unsigned)(wpb * G.nq)), c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen,
+                                                                   c->flux[0], c->flux[1], c->flux[2], c->emf, wpb,
+                                                                   nullptr, wide);
         return;
     }
     site_launch(G, c->tune.emf, [&](dim3 g, int wpb) {
@@ -1936,7 +1940,7 @@ static void launch_fofc(grmhd_ctx* c, const double* ast, int anv, const double* 
     launch_fluxes(c, aux, ms, NSTATE, oB, s, true);
     site_launch(G, c->tune.emf, [&](dim3 g, int w) {
         k_emf_gs<<<g, c->tune.emf, 0, s>>>(G, c->vbar[0], c->vbar[1], c->vbar[2], c->ecen, c->flux[0], c->flux[1],
-                                           c->flux[2], c->emf, w, c->etag);
+                                           c->flux[2], c->emf, w, c->etag, 0);
     });
 }
 

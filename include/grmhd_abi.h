@@ -88,6 +88,15 @@ typedef struct {
     int32_t fofc;
     int32_t fofc_dmp_enable;
     double fofc_dmp_M;
+    /* 0 (default, as the reference): the GS edge EMFs are computed on the
+     * interior edge range only, while the FOFC flag pass also advances face B
+     * for the first ghost layer of cells (evolve.cpp:492-525 against
+     * :1090-1104) and so reads edges no pass has written; the ghost-layer
+     * cells' tentative states then flag spuriously and the two blocks beside
+     * a shared face can tag it differently, which breaks conservation.
+     * 1: compute the GS edge EMFs one ghost layer wider when FOFC is on, so
+     * the ghost-layer flags equal the neighbour's interior flags. */
+    int32_t fofc_ghost_emf;
 } grmhd_params;
 
 /* Block grid: every block has the same interior extents. */
@@ -130,9 +139,9 @@ grmhd_status grmhd_ctx_create(const grmhd_params* p, const grmhd_grid* grid,
  * restrict_kernels.hh:125-164), and every substep refluxes: the coarse flux
  * of each coarse-fine face becomes the average of the four fine fluxes and
  * each coarse edge EMF touched by a fine cell the average of the two fine
- * EMFs along it (refluxing.cpp:547-806, 959-1173).  The Cowling metric of the
- * ghost cells is the caller's (initial data), as on uniform grids.  FOFC is
- * not supported on a forest (GRMHD_ERR_UNSUPPORTED). */
+ * EMFs along it (refluxing.cpp:547-806, 959-1173), after the FOFC
+ * correction when FOFC is on (evolve.cpp:1685-1697).  The Cowling metric of
+ * the ghost cells is the caller's (initial data), as on uniform grids. */
 typedef struct {
     int32_t nb[3];
     int32_t bc[6];

@@ -37,6 +37,9 @@ struct Options {
     // first-order flux correction (evolve.cpp:456-811, grmhd_helpers.hh:63-75,226-231)
     int fofc = 0, dmp_enable = 1;
     double dmp_M = 1.2;
+    // 1: GS edge EMFs one ghost layer wider with FOFC, so the flag pass's
+    // ghost-layer cells read computed edges (grmhd_params.fofc_ghost_emf)
+    int fofc_ghost_emf = 0;
 };
 
 // Block grid.  All blocks share (nx,ny,nz,ngz).
@@ -464,17 +467,20 @@ struct Stepper {
         return Eavg + dEdy + dEdx;
     }
 
+    // edge range [ngz, n+ngz] (evolve.cpp:1090-1104); one layer wider with
+    // FOFC and fofc_ghost_emf
     void compute_emfs_gs() {
         const Grid& g = G();
         int ng = g.ngz;
+        const int w = (opt.fofc && opt.fofc_ghost_emf) ? 1 : 0;
         for (int64_t q = 0; q < g.nq; ++q) {
 #pragma omp parallel for collapse(2) schedule(static)
-            for (int k = ng; k < g.nz + ng + 1; ++k)
-                for (int j = ng; j < g.ny + ng + 1; ++j)
-                    for (int i = ng; i < g.nx + ng + 1; ++i) {
-                        if (i < g.nx + ng) emf[g.ed(i, j, k, 0, q)] = gs_edge(0, i, j, k, q);
-                        if (j < g.ny + ng) emf[g.ed(i, j, k, 1, q)] = gs_edge(1, i, j, k, q);
-                        if (k < g.nz + ng) emf[g.ed(i, j, k, 2, q)] = gs_edge(2, i, j, k, q);
+            for (int k = ng - w; k < g.nz + ng + 1 + w; ++k)
+                for (int j = ng - w; j < g.ny + ng + 1 + w; ++j)
+                    for (int i = ng - w; i < g.nx + ng + 1 + w; ++i) {
+                        if (i < g.nx + ng + w) emf[g.ed(i, j, k, 0, q)] = gs_edge(0, i, j, k, q);
+                        if (j < g.ny + ng + w) emf[g.ed(i, j, k, 1, q)] = gs_edge(1, i, j, k, q);
+                        if (k < g.nz + ng + w) emf[g.ed(i, j, k, 2, q)] = gs_edge(2, i, j, k, q);
                     }
         }
     }

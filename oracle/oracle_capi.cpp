@@ -49,6 +49,7 @@ ExcisionParams make_exc(const grmhd_params* p) {
 Options make_opt(const grmhd_params* p) {
     Options o; o.recon = p->recon; o.riemann = p->riemann; o.emf = p->emf; o.thermo = p->thermo_recon;
     o.fofc = p->fofc; o.dmp_enable = p->fofc_dmp_enable; o.dmp_M = p->fofc_dmp_M;
+    o.fofc_ghost_emf = p->fofc_ghost_emf;
     o.fd_order = p->fd_order; o.rusanov_c_limit = p->rusanov_c_limit; return o;
 }
 

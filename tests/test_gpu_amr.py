@@ -168,10 +168,9 @@ def test_gpu_forest_graph_matches_eager(dev):
 def test_forest_rejects_bad_input(dev):
     wl = W.forest_wave(n=8, ngz=4, **FORESTS["one"])
     p = wl.params
-    p.fofc = 1
+    f = ga.make_forest((2, 2, 2), (1,) * 6, (16.0, 16.0, 8.0), [0])   # non-cubic cells
     with pytest.raises(RuntimeError):
-        ga.Context(p, wl.grid, wl.instances, forest=wl.forest)
-    p.fofc = 0
+        ga.Context(p, wl.grid, wl.instances, forest=f)
     g = ga.Grid(8, 8, 8, 4, wl.grid.nq + 1)
     with pytest.raises(RuntimeError):
         ga.Context(p, g, wl.instances, forest=wl.forest)
@@ -253,3 +252,83 @@ def test_gpu_torus_forest_learned_c2p(dev):
         assert linf_rel(face_interior(wl.grid, Bl[d], d), face_interior(wl.grid, Bp[d], d)) < 1e-9, d
     assert np.array_equal(al[:, ga.C2P_ERR], ap[:, ga.C2P_ERR])
     assert stl["n_guess_used"] > 0.5 * stl["n_cells"]
+
+
+@pytest.mark.parametrize("ghost_emf,case", [(0, "smooth"), (0, "harsh"), (1, "smooth"), (1, "harsh")])
+def test_gpu_forest_fofc_substep(dev, ghost_emf, case):
+    """FOFC (GS) on a forest, one substep, both edge ranges, against the oracle
+    at 1e-12.  (With the reference's edge range the flag pass decides on
+    tentative ghost-layer states built from unwritten edges; over RK3 stages
+    the last-bit differences of the stage combination -- contracted on the
+    GPU, not in the oracle -- flip some of those decisions, so that mode is
+    compared per substep.)"""
+    import torch
+    from test_oracle_amr import FOFC_HARSH, FOFC_SMOOTH
+    kw = FOFC_HARSH if case == "harsh" else FOFC_SMOOTH
+    wl = W.forest_wave(nb=(2, 2, 2), n=8, ngz=4, refined=[0, 3], emf=1, **kw)
+    wl.params.fofc = 1
+    wl.params.fofc_ghost_emf = ghost_emf
+    say(f"[forest fofc substep] ghost_emf={ghost_emf} {case}")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ctx = ctx_of(wl)
+    sg, Bg, ag = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+    o.aux(st, B, aux)
+    ctx.compute_auxiliaries(sg, Bg, ag)
+    dt = o.dt(st, aux, 0.25)
+    new_o, nB_o = st.copy(), [b.copy() for b in B]
+    o.substep(dt, 1.0, new_o, st, nB_o, B, aux)
+    assert o.fofc_flagged() > 0
+    ng_, nBg = sg.clone(), [b.clone() for b in Bg]
+    ctx.advance_substep(torch.tensor([dt], dtype=torch.float64, device=dev), 1.0, ng_, sg, nBg, Bg, ag)
+    torch.cuda.synchronize()
+    gn = to_host(ng_)
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
+        assert e < 1e-12, (v, e)
+    for d in range(3):
+        e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
+        assert e < 1e-12, (d, e)
+
+
+@pytest.mark.parametrize("case", ["smooth", "harsh"])
+def test_gpu_forest_fofc_rk3(dev, case):
+    """FOFC with the ghost-layer edge EMFs (fofc_ghost_emf = 1) on a forest:
+    two RK3 steps against the oracle (1e-12; the entropy tracer 1e-11 on the
+    harsh blast, pressure ratio 1e9, where FOFC is active in tens of
+    thousands of cells), and the GPU's own result conserves the volume
+    integrals and keeps the coarse-fine faces consistent."""
+    import torch
+    from test_oracle_amr import FOFC_HARSH, FOFC_SMOOTH
+    kw = FOFC_HARSH if case == "harsh" else FOFC_SMOOTH
+    wl = W.forest_wave(nb=(2, 2, 2), n=8, ngz=4, refined=[0, 3], emf=1, **kw)
+    wl.params.fofc = 1
+    wl.params.fofc_ghost_emf = 1
+    say(f"[forest fofc rk3] {case}")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ctx = ctx_of(wl)
+    sg, Bg, ag = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+    o.aux(st, B, aux)
+    ctx.compute_auxiliaries(sg, Bg, ag)
+    tot0 = volume_integrals(o, wl, st)
+    dtd = torch.zeros(1, dtype=torch.float64, device=dev)
+    for _ in range(2):
+        dt = o.dt(st, aux, 0.25)
+        o.rk3(dt, st, B, aux)
+        dtd.fill_(dt)
+        ctx.rk3_step(dtd, sg, Bg, ag)
+    torch.cuda.synchronize()
+    assert o.fofc_flagged() > 0
+    gs, gB = to_host(sg), [to_host(b) for b in Bg]
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, gs[:, v]), interior(wl.grid, st[:, v]))
+        assert e < (1e-11 if v == 6 else 1e-12), (v, e)
+    for d in range(3):
+        e = linf_rel(face_interior(wl.grid, gB[d], d), face_interior(wl.grid, B[d], d))
+        assert e < 1e-12, (d, e)
+    tot1 = volume_integrals(o, wl, gs)
+    assert (np.abs(tot1 - tot0) / np.maximum(np.abs(tot0), 1e-3)).max() < 5e-14
+    assert interface_mismatch(o, wl, gB) < 1e-14

@@ -294,3 +294,49 @@ def test_rk3_conserves_and_keeps_interfaces_consistent(emf):
     m1 = interface_mismatch(o, wl, B)
     assert m1 < 1e-14, m1
     assert o.fatal() == 0
+
+
+FOFC_SMOOTH = dict(rho0=1.0, p0=1e-3, p_in=10.0, r_in=0.25, amp=0.9, zvel=(0.9, 0.5, 0.2), b_loop=0.2)
+FOFC_HARSH = dict(rho0=1.0, p0=1e-6, p_in=1e3, r_in=0.2, amp=0.99, zvel=(3.0, 2.0, 1.0), b_loop=1.0)
+
+
+def _fofc_run(refined, kw, ghost_emf, steps=3):
+    wl = W.forest_wave(nb=(2, 2, 2), n=8, refined=refined, emf=1, **kw)
+    wl.params.fofc = 1
+    wl.params.fofc_ghost_emf = ghost_emf
+    o = Stepper(wl)
+    st, B, aux = wl.copy_arrays()
+    o.ghosts(st, B)
+    o.aux(st, B, aux)
+    tot0 = volume_integrals(o, wl, st)
+    for _ in range(steps):
+        o.rk3(o.dt(st, aux, 0.25), st, B, aux)
+    tot1 = volume_integrals(o, wl, st)
+    rel = float((np.abs(tot1 - tot0) / np.maximum(np.abs(tot0), 1e-3)).max())
+    return o, wl, B, rel
+
+
+def test_fofc_reference_edge_range_flags_spuriously():
+    """The reference computes GS edge EMFs on [ngz, n+ngz] (evolve.cpp:1090-1104)
+    but its FOFC flag pass also advances face B for the first ghost layer
+    (evolve.cpp:492-525), reading edges nothing wrote: on smooth periodic data
+    cells are flagged and the two sides of a block face disagree, so the run
+    stops conserving.  With the EMFs one ghost layer wider
+    (fofc_ghost_emf = 1) nothing is flagged and conservation holds."""
+    o_ref, _, _, rel_ref = _fofc_run([], FOFC_SMOOTH, 0)
+    o_fix, _, _, rel_fix = _fofc_run([], FOFC_SMOOTH, 1)
+    assert o_ref.fofc_flagged() > 100 and rel_ref > 1e-8, (o_ref.fofc_flagged(), rel_ref)
+    assert o_fix.fofc_flagged() == 0 and rel_fix < 5e-14, (o_fix.fofc_flagged(), rel_fix)
+
+
+@pytest.mark.parametrize("refined", [[], [0, 3]])
+def test_fofc_with_ghost_emfs_conserves_on_forests(refined):
+    """Strong blast with FOFC active in tens of thousands of cells: with the
+    ghost-layer EMFs the first-order fallback keeps the volume integrals
+    (and, on the forest, the coarse-fine face consistency) at round-off."""
+    o, wl, B, rel = _fofc_run(refined, FOFC_HARSH, 1)
+    assert o.fofc_flagged() > 10000, o.fofc_flagged()
+    assert rel < 5e-14, rel
+    assert o.fatal() == 0
+    if refined:
+        assert interface_mismatch(o, wl, B) < 1e-14
