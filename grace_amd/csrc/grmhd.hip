@@ -111,8 +111,14 @@ GHD void load_metric(Met& m, const double* __restrict__ ms, int64_t ncell, int64
 // ---------------------------------------------------------------------------
 // Face flux kernel
 // ---------------------------------------------------------------------------
+// Minimum waves per SIMD the compiler must fit (VGPR budget); unset: its own choice.
+#ifdef GRMHD_FLUX_MINW
+#define GRMHD_FLUX_BOUNDS __launch_bounds__(256, GRMHD_FLUX_MINW)
+#else
+#define GRMHD_FLUX_BOUNDS __launch_bounds__(256)
+#endif
 template <typename EOS, int REC, bool FLAT, int D>
-__global__ __launch_bounds__(256) void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
+__global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
                                               const double* __restrict__ mstate, int mnv,
                                               const double* __restrict__ Bd, double* __restrict__ flux,
                                               double* __restrict__ vbar, int wpb,
