@@ -332,3 +332,35 @@ def test_gpu_forest_fofc_rk3(dev, case):
     tot1 = volume_integrals(o, wl, gs)
     assert (np.abs(tot1 - tot0) / np.maximum(np.abs(tot0), 1e-3)).max() < 5e-14
     assert interface_mismatch(o, wl, gB) < 1e-14
+
+
+def test_gpu_forest_workgroup_shapes_bit_identical(dev):
+    """Workgroup sizes (grmhd_autotune / grmhd_set_tuning) change only the
+    launch shapes: an RK3 step on a forest is bit-identical under two
+    different tunings, and grmhd_autotune leaves the caller's arrays alone."""
+    import torch
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, **FORESTS["three"])
+    dtd = torch.tensor([0.25 * wl.extra["dx"] / 2], dtype=torch.float64, device=dev)
+    out = []
+    for mode in ("default", "other", "autotuned"):
+        ctx = ctx_of(wl)
+        sg, Bg, ag = to_dev(wl.state, dev), [to_dev(b, dev) for b in wl.B], to_dev(wl.aux, dev)
+        ctx.compute_auxiliaries(sg, Bg, ag)
+        if mode == "other":
+            ctx.set_tuning(flux=(64, 128, 64), emf=128, update=128, gather_faces=64, gather_cells=128, c2p=128)
+        elif mode == "autotuned":
+            before = (sg.clone(), [b.clone() for b in Bg], ag.clone())
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                ctx.autotune(sg, Bg, ag, reps=1, stream=s.cuda_stream)
+            s.synchronize()
+            assert torch.equal(sg, before[0]) and torch.equal(ag, before[2])
+            for d in range(3):
+                assert torch.equal(Bg[d], before[1][d])
+        ctx.rk3_step(dtd, sg, Bg, ag)
+        torch.cuda.synchronize()
+        out.append((sg, Bg, ag))
+    for o in out[1:]:
+        assert torch.equal(o[0], out[0][0]) and torch.equal(o[2], out[0][2])
+        for d in range(3):
+            assert torch.equal(o[1][d], out[0][1][d])
