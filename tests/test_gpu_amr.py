@@ -364,3 +364,39 @@ def test_gpu_forest_workgroup_shapes_bit_identical(dev):
         assert torch.equal(o[0], out[0][0]) and torch.equal(o[2], out[0][2])
         for d in range(3):
             assert torch.equal(o[1][d], out[0][1][d])
+
+
+def test_gpu_fully_refined_forest_equals_uniform_grid(dev):
+    """Every coarse block refined: the library's forest path (no coarse-fine
+    interface left) reproduces its uniform-grid path on the fine spacing to
+    round-off after two RK3 steps (block order permuted; the periodic faces'
+    two initial copies differ in the last bit, see the oracle test)."""
+    import copy
+    import torch
+    from test_oracle_amr import _uniform_twin
+    wf = W.forest_wave(nb=(2, 2, 2), n=8, ngz=4, refined=list(range(8)), b_loop=0.2, p_in=3.0, r_in=0.3)
+    spec, qmap = _uniform_twin(wf)
+    inv = np.argsort(qmap)
+    dtd = torch.tensor([0.25 * wf.extra["dx"] / 2], dtype=torch.float64, device=dev)
+    res = []
+    for forest in (True, False):
+        if forest:
+            ctx = ctx_of(wf)
+            st, B, aux = wf.state, wf.B, wf.aux
+        else:
+            ctx = ga.Context(wf.params, wf.grid, ga.make_instances(spec))
+            st, B, aux = wf.state[inv], [b[inv] for b in wf.B], wf.aux[inv]
+        sg, Bg, ag = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+        ctx.fill_ghosts(sg, Bg)
+        ctx.compute_auxiliaries(sg, Bg, ag)
+        for _ in range(2):
+            ctx.rk3_step(dtd, sg, Bg, ag)
+        torch.cuda.synchronize()
+        res.append((to_host(sg), [to_host(b) for b in Bg], to_host(ag)))
+    (sf, Bf, af), (su, Bu, au) = res
+    for v in range(7):
+        assert linf_rel(sf[:, v], su[qmap][:, v]) < 1e-13, v
+    for d in range(3):
+        assert linf_rel(Bf[d], Bu[d][qmap]) < 1e-13, d
+    for v in (ga.RHO, ga.PRESS, ga.ZVECX, ga.BX):
+        assert linf_rel(af[:, v], au[qmap][:, v]) < 1e-13, v
