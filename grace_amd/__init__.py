@@ -91,6 +91,7 @@ _lib = None
 ABI_SYMBOLS = [
     "grmhd_default_params", "grmhd_ctx_create", "grmhd_ctx_destroy", "grmhd_last_error",
     "grmhd_ctx_create_forest", "grmhd_forest_block", "grmhd_setup_forest_wave", "grmhd_setup_fm_torus_forest",
+    "grmhd_fill_metric_ghosts",
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
@@ -151,6 +152,7 @@ def lib():
     L.grmhd_last_error.restype = C.c_char_p
     L.grmhd_advance_substep.argtypes = [P, P, d, P, P, P * 3, P * 3, P, P]
     L.grmhd_fill_ghosts.argtypes = [P, P, P * 3, P]
+    L.grmhd_fill_metric_ghosts.argtypes = [P, P, P]
     L.grmhd_compute_auxiliaries.argtypes = [P, P, P * 3, P, P]
     L.grmhd_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P, P]
     L.grmhd_prims_to_conservs_batch.argtypes = [P, i64, P, P, P, P]
@@ -298,6 +300,9 @@ class Context:
 
     def fill_ghosts(self, state, B, stream=None):
         check(self.L.grmhd_fill_ghosts(self.h, _ptr(state), ptr3(*B), _ptr(stream)), "fill_ghosts")
+
+    def fill_metric_ghosts(self, state, stream=None):
+        check(self.L.grmhd_fill_metric_ghosts(self.h, _ptr(state), _ptr(stream)), "fill_metric_ghosts")
 
     def compute_auxiliaries(self, state, B, aux, stream=None):
         check(self.L.grmhd_compute_auxiliaries(self.h, _ptr(state), ptr3(*B), _ptr(aux), _ptr(stream)),

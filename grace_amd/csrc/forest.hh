@@ -335,6 +335,146 @@ GHD void ff_amr_face_mid(const FForest& F, int q, int fdoff, double* B0, double*
     Bd[(int64_t)q * F.nface[fd] + off] = val;
 }
 
+// ---- the Cowling metric (4th-order variables of the reference) --------------------
+// 5-point Lagrange weights (lagrange_prolong_op<4> / lagrange_restrict_op<4>,
+// pr_helpers.hh:105-330) from the Lagrange basis; numerator and denominator
+// products are exact, so each weight is its exact dyadic value.  pro[r]:
+// fine centre at -1/4 (r = 0) or +1/4 (r = 1) of a coarse cell from coarse
+// nodes -2..2; res[s]: coarse centre from 5 fine cells starting s - 3.5 fine
+// cells away (the L2, L1, CENTER, R2 stencils).
+struct LagW { double pro[2][5], res[4][5]; };
+inline LagW lagrange_weights() {
+    auto w = [](const double* x, int d, double t) {
+        double num = 1.0, den = 1.0;
+        for (int m = 0; m < 5; ++m)
+            if (m != d) { num *= t - x[m]; den *= x[d] - x[m]; }
+        return num / den;
+    };
+    LagW W;
+    const double xc[5] = {-2, -1, 0, 1, 2};
+    for (int r = 0; r < 2; ++r)
+        for (int d = 0; d < 5; ++d) W.pro[r][d] = w(xc, d, r ? 0.25 : -0.25);
+    for (int st = 0; st < 4; ++st) {
+        double xf[5];
+        for (int d = 0; d < 5; ++d) xf[d] = st - 3.5 + d;
+        for (int d = 0; d < 5; ++d) W.res[st][d] = w(xf, d, 0.0);
+    }
+    return W;
+}
+// contract1d_pairsym<5> (pr_helpers.hh:52-58): mirror pairs summed first, so
+// mirrored stencils give bit-identical results.
+GHD double ff_contract5(const double* c, const double u[5]) {
+    const double so = c[0] * u[0] + c[4] * u[4];
+    const double si = c[1] * u[1] + c[3] * u[3];
+    const double sc = c[2] * u[2];
+    return (so + si) + sc;
+}
+// lagrange_restrict_op<4>::compute_offset (pr_helpers.hh:309-330): stencil
+// kept inside the fine block, mirrored between the two halves of the parent.
+GHD int ff_ho_offset(int pos, int n, int ng, int half) {
+    const int lb = pos - ng, ub = n + ng - pos - 1;
+    if (ub == 1) return 0;
+    if (lb == 0) return 3;
+    if (ub == 2) return 1;
+    return half ? 1 : 2;
+}
+
+// Virtual coarse metric field on every coarse cell of the domain (vars
+// [v0, v0 + vc) of the state, vc-major): a coarse leaf's cell, or the
+// 5-point Lagrange restriction of the fine block covering it.
+__global__ __launch_bounds__(256) void k_vc_metric(FForest F, LagW W, const double* __restrict__ st, int nv, int v0,
+                                                   int vc, double* __restrict__ vcm, int64_t ncc) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ncc) return;
+    const int Nc0 = ff_ntot(F, 0, 0), Nc1 = ff_ntot(F, 0, 1);
+    const int g[3] = {(int)(x % Nc0), (int)((x / Nc0) % Nc1), (int)(x / ((int64_t)Nc0 * Nc1))};
+    const FOwner o = ff_owner(F, 0, g);
+    if (o.cover == 0) {
+        const double* p = st + ((int64_t)o.q * nv + v0) * F.ncell + ff_cidx(F, o.loc[0], o.loc[1], o.loc[2]);
+        for (int v = 0; v < vc; ++v) vcm[v * ncc + x] = p[v * F.ncell];
+        return;
+    }
+    const int4 b = F.blk[o.q];
+    const int half[3] = {b.y & 1, b.z & 1, b.w & 1};
+    int base[3];
+    const double* c[3];
+    for (int d = 0; d < 3; ++d) {
+        const int s = ff_ho_offset(o.loc[d], F.n[d], F.ng, half[d]);
+        c[d] = W.res[s];
+        base[d] = o.loc[d] + s - 3;
+    }
+    const int64_t sj = F.N[0], sk = (int64_t)F.N[0] * F.N[1];
+    for (int v = 0; v < vc; ++v) {
+        const double* p = st + ((int64_t)o.q * nv + v0 + v) * F.ncell + ff_cidx(F, base[0], base[1], base[2]);
+        double Sx[5][5];
+        for (int dk = 0; dk < 5; ++dk)
+            for (int dj = 0; dj < 5; ++dj) {
+                double line[5];
+                for (int di = 0; di < 5; ++di) line[di] = p[di + dj * sj + dk * sk];
+                Sx[dj][dk] = ff_contract5(c[0], line);
+            }
+        double Sy[5];
+        for (int dk = 0; dk < 5; ++dk) {
+            double line[5];
+            for (int dj = 0; dj < 5; ++dj) line[dj] = Sx[dj][dk];
+            Sy[dk] = ff_contract5(c[1], line);
+        }
+        vcm[v * ncc + x] = ff_contract5(c[2], Sy);
+    }
+}
+// Metric of the coarse-fine ghost cells (the amr cell entries) from the
+// virtual coarse field: coarse ghosts take it, fine ghosts its 5-point
+// Lagrange prolongation.  Ghosts beyond an outflow face keep their values
+// (the metric's bc "none").
+__global__ __launch_bounds__(256) void k_amr_metric(const int2* __restrict__ ent, int n, FForest F, LagW W,
+                                                    double* st, int nv, int v0, int vc,
+                                                    const double* __restrict__ vcm, int64_t ncc) {
+    const int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n) return;
+    const int2 t = ent[e];
+    const int q = t.x, off = t.y;
+    const int loc[3] = {off % F.N[0], (off / F.N[0]) % F.N[1], off / (F.N[0] * F.N[1])};
+    const int4 b = F.blk[q];
+    const int lev = b.x, cr[3] = {b.y, b.z, b.w};
+    int g[3];
+    for (int d = 0; d < 3; ++d) {
+        const int gg = cr[d] * F.n[d] + loc[d] - F.ng;
+        const int Nt = ff_ntot(F, lev, d);
+        if ((gg < 0 && F.bc[2 * d] != 1) || (gg >= Nt && F.bc[2 * d + 1] != 1)) return;
+        g[d] = ff_map_cc(F, lev, d, gg);
+    }
+    const int Nc0 = ff_ntot(F, 0, 0), Nc1 = ff_ntot(F, 0, 1);
+    auto VC = [&](int v, int X0, int X1, int X2) {
+        const int a0 = ff_map_cc(F, 0, 0, X0), a1 = ff_map_cc(F, 0, 1, X1), a2 = ff_map_cc(F, 0, 2, X2);
+        return vcm[v * ncc + a0 + (int64_t)Nc0 * (a1 + (int64_t)Nc1 * a2)];
+    };
+    double* dst = st + (int64_t)q * nv * F.ncell + off;
+    if (lev == 0) {
+        for (int v = 0; v < vc; ++v) dst[(v0 + v) * F.ncell] = VC(v, g[0], g[1], g[2]);
+        return;
+    }
+    const int X[3] = {g[0] >> 1, g[1] >> 1, g[2] >> 1};
+    const double* c0 = W.pro[g[0] & 1];
+    const double* c1 = W.pro[g[1] & 1];
+    const double* c2 = W.pro[g[2] & 1];
+    for (int v = 0; v < vc; ++v) {
+        double Sx[5][5];
+        for (int dk = 0; dk < 5; ++dk)
+            for (int dj = 0; dj < 5; ++dj) {
+                double line[5];
+                for (int di = 0; di < 5; ++di) line[di] = VC(v, X[0] + di - 2, X[1] + dj - 2, X[2] + dk - 2);
+                Sx[dj][dk] = ff_contract5(c0, line);
+            }
+        double Sy[5];
+        for (int dk = 0; dk < 5; ++dk) {
+            double line[5];
+            for (int dj = 0; dj < 5; ++dj) line[dj] = Sx[dj][dk];
+            Sy[dk] = ff_contract5(c1, line);
+        }
+        dst[(v0 + v) * F.ncell] = ff_contract5(c2, Sy);
+    }
+}
+
 // ---- kernels ---------------------------------------------------------------------
 // Entries {block, local cell offset}.
 __global__ __launch_bounds__(256) void k_amr_cells(const int2* __restrict__ ent, int n, double* st, int nv, int v0,

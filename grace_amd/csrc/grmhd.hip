@@ -1336,6 +1336,7 @@ struct grmhd_ctx {
     RefluxFace* rflux = nullptr; int64_t n_rflux = 0;
     RefluxEdge* remf = nullptr; int64_t n_remf = 0;
     int64_t n_own_gather = 0;                 // own-c2p ghosts that copy a clamped source
+    double* vcm = nullptr;                    // virtual coarse metric field (grmhd_fill_metric_ghosts)
 };
 
 static hipEvent_t pool_event(grmhd_ctx* c) {
@@ -1409,6 +1410,7 @@ static void free_ctx(grmhd_ctx* c) {
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
     F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->amr_faces_mid); F(c->rflux); F(c->remf);
+    F(c->vcm);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1978,6 +1980,30 @@ extern "C" grmhd_status grmhd_fill_ghosts(grmhd_ctx* c, double* state, double* c
     for (int d = 0; d < 3; ++d) a.B[d] = B[d];
     launch_gather(c, a, (hipStream_t)stream);
     return launch_check("fill_ghosts");
+}
+
+extern "C" grmhd_status grmhd_fill_metric_ghosts(grmhd_ctx* c, double* state, grmhd_stream_t stream) {
+    if (!c || !state) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    hipStream_t s = (hipStream_t)stream;
+    {   // same-level copies and periodic images; nothing beyond outflow faces
+        GatherArgs a{};
+        a.st = state; a.nv = NSTATE; a.v0 = GXX_; a.vc = 16; a.cell_mode = 3;
+        launch_gather(c, a, s);
+    }
+    if (c->forest && c->n_amr_cells > 0) {
+        const int64_t ncc = (int64_t)ff_ntot(c->ff, 0, 0) * ff_ntot(c->ff, 0, 1) * ff_ntot(c->ff, 0, 2);
+        if (16 * ncc > (int64_t)1 << 29) { set_err("fill_metric_ghosts: coarse domain too large"); return GRMHD_ERR_UNSUPPORTED; }
+        if (!c->vcm) {
+            hipError_t e = hipMalloc(&c->vcm, sizeof(double) * 16 * ncc);
+            if (e != hipSuccess) { c->vcm = nullptr; set_err("vcm: %s", hipGetErrorString(e)); return GRMHD_ERR_ALLOC; }
+        }
+        const LagW W = lagrange_weights();
+        Scope sc(c, "ghost_metric", s);
+        k_vc_metric<<<nblk(ncc), 256, 0, s>>>(c->ff, W, state, NSTATE, GXX_, 16, c->vcm, ncc);
+        k_amr_metric<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, c->ff, W, state, NSTATE,
+                                                         GXX_, 16, c->vcm, ncc);
+    }
+    return launch_check("fill_metric_ghosts");
 }
 
 extern "C" grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* c, double* state, const double* const B[3], double* aux,

@@ -90,6 +90,12 @@ public:
               "compute_auxiliary_quantities");
     }
 
+    // The Cowling metric's ghost exchange (copies; 5-point Lagrange across
+    // coarse-fine faces of a forest), once after the initial data.
+    void exchange_metric_ghosts(var_array_t state, hipStream_t stream = nullptr) {
+        check(grmhd_fill_metric_ghosts(ctx_, state.data, (grmhd_stream_t)stream), "exchange_metric_ghosts");
+    }
+
     // dt = cfl * dx / max characteristic speed, left on the device.
     void find_stable_timestep(var_array_t state, var_array_t aux, double cfl, double* dt_dev,
                               hipStream_t stream = nullptr) {

@@ -173,6 +173,18 @@ grmhd_status grmhd_advance_substep(grmhd_ctx* ctx, const double* dt_dev, double 
 grmhd_status grmhd_fill_ghosts(grmhd_ctx* ctx, double* state, double* const B[3],
                                grmhd_stream_t stream);
 
+/* The ghost exchange of the Cowling metric (state vars GXX_..KZZ_), which the
+ * reference runs with the evolved variables (they are registered
+ * "fourth_order" with bc "none", variable_indices.cpp:442-446): copies from
+ * same-level neighbours and periodic images; on a forest, 5-point Lagrange
+ * restriction into coarse ghosts and prolongation into fine ghosts
+ * (lagrange_restrict_op<4> / lagrange_prolong_op<4>, pr_helpers.hh:105-330,
+ * with the reference's stencil choice and pair-symmetric sums); ghosts beyond
+ * an outflow face keep their values.  The metric is static, so one call after
+ * the initial data (before grmhd_set_static_metric) gives the ghost values the
+ * reference's exchange settles on. */
+grmhd_status grmhd_fill_metric_ghosts(grmhd_ctx* ctx, double* state, grmhd_stream_t stream);
+
 /* compute_auxiliary_quantities<eos_t> (include/grace/evolution/auxiliaries.hh:49-54,
  * src/evolution/auxiliaries.cpp:81-180): c2p on every cell incl. ghosts,
  * conserved rewrite on reset bits, sticky-OR of C2P_ERR, BDIV, and (GS)

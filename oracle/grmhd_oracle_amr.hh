@@ -135,6 +135,58 @@ inline double amr_minmod(double sL, double sR) {
     return 0.5 * (a + b) * std::fmin(std::fabs(sL), std::fabs(sR));
 }
 
+// 4th-order (5-point) Lagrange weights, computed from the Lagrange basis
+// (numerator and denominator products are exact, so each weight is the exact
+// dyadic value): prolongation to a fine centre at -1/4 (row 0) or +1/4 (row 1)
+// of a coarse cell from coarse nodes -2..2 (lagrange_prolong_op<4>,
+// pr_helpers.hh:105-170); restriction to a coarse centre from 5 fine cells
+// starting s - 3.5 fine cells away, s = 0..3 (the L2, L1, CENTER, R2 stencils
+// of lagrange_restrict_op<4>, pr_helpers.hh:222-330).
+struct LagrangeW {
+    double pro[2][5], res[4][5];
+    static double w(const double* x, int d, double t) {
+        double num = 1.0, den = 1.0;
+        for (int m = 0; m < 5; ++m)
+            if (m != d) { num *= t - x[m]; den *= x[d] - x[m]; }
+        return num / den;
+    }
+    LagrangeW() {
+        const double xc[5] = {-2, -1, 0, 1, 2};
+        for (int r = 0; r < 2; ++r)
+            for (int d = 0; d < 5; ++d) pro[r][d] = w(xc, d, r ? 0.25 : -0.25);
+        for (int st = 0; st < 4; ++st) {
+            double xf[5];
+            for (int d = 0; d < 5; ++d) xf[d] = st - 3.5 + d;
+            for (int d = 0; d < 5; ++d) res[st][d] = w(xf, d, 0.0);
+        }
+    }
+};
+// contract1d_pairsym<5> (pr_helpers.hh:52-58): mirror pairs summed first.
+inline double contract5(const double c[5], const double u[5]) {
+    const double so = c[0] * u[0] + c[4] * u[4];
+    const double si = c[1] * u[1] + c[3] * u[3];
+    const double sc = c[2] * u[2];
+    return (so + si) + sc;
+}
+// Tensor-product contraction over a 5x5x5 stencil, x then y then z.
+template <typename U>
+inline double contract555(const double cx[5], const double cy[5], const double cz[5], U u) {
+    double Sx[5][5];
+    for (int dk = 0; dk < 5; ++dk)
+        for (int dj = 0; dj < 5; ++dj) {
+            double line[5];
+            for (int di = 0; di < 5; ++di) line[di] = u(di, dj, dk);
+            Sx[dj][dk] = contract5(cx, line);
+        }
+    double Sy[5];
+    for (int dk = 0; dk < 5; ++dk) {
+        double line[5];
+        for (int dj = 0; dj < 5; ++dj) line[dj] = Sx[dj][dk];
+        Sy[dk] = contract5(cy, line);
+    }
+    return contract5(cz, Sy);
+}
+
 struct AmrExchange {
     const Forest& F;
     AmrArrays A;
@@ -206,7 +258,7 @@ struct AmrExchange {
         int t[3];
         for (int e = 0; e < 3; ++e) t[e] = e == d ? 0 : F.map_cc(0, e, T[e]);
         const int f = F.map_fc(0, d, F0);
-        int64_t q; int loc[3];
+        int64_t q = 0; int loc[3] = {0, 0, 0};
         if (face_src(0, d, f, t, q, loc)) return B[d][A.fc(d, loc[0], loc[1], loc[2], q)];
         // both sides refined: div_free_restrict_op (restrict_kernels.hh:145-164)
         int tf[3];
@@ -221,7 +273,7 @@ struct AmrExchange {
     // Fine face on a coarse face (f even along d): existing fine data, or
     // fill_inside_face (prolongation_kernels.hh:130-160).
     double fine_even(double* const B[3], int d, int f, const int t[3]) const {
-        int64_t q; int loc[3];
+        int64_t q = 0; int loc[3] = {0, 0, 0};
         int tm[3];
         for (int e = 0; e < 3; ++e) tm[e] = e == d ? 0 : F.map_cc(1, e, t[e]);
         const int fm = F.map_fc(1, d, f);
@@ -311,7 +363,7 @@ struct AmrExchange {
                                 else t[d] = F.map_cc(lev, d, gg);
                             }
                             double val;
-                            int64_t qs; int sl[3];
+                            int64_t qs = 0; int sl[3] = {0, 0, 0};
                             if (face_src(lev, fd, f, t, qs, sl)) val = B[fd][A.fc(fd, sl[0], sl[1], sl[2], qs)];
                             else if (lev == 0) {
                                 int T[3] = {t[0], t[1], t[2]};
@@ -330,6 +382,94 @@ struct AmrExchange {
                         }
             }
         }
+    }
+
+    // ---- Cowling metric (variables registered "fourth_order",
+    // variable_indices.cpp:442-446): exchanged with the evolved variables, so
+    // copies between same-level blocks, 5-point Lagrange across coarse-fine
+    // boundaries, and no physical BC (bc "none": ghosts beyond an outflow face
+    // keep their values).  The metric is static, so one exchange after the
+    // initial data gives the reference's ghost values.
+    // lagrange_restrict_op<4>::compute_offset (pr_helpers.hh:309-330).
+    int ho_offset(int pos, int n, int half) const {
+        const int lb = pos - F.ng, ub = n + F.ng - pos - 1;
+        if (ub == 1) return 0;
+        if (lb == 0) return 3;
+        if (ub == 2) return 1;
+        return half ? 1 : 2;
+    }
+    // Restriction of fine block q at lower fine cell loc (child bits of q
+    // select the half-dependent stencil).
+    double restrict_ho(const LagrangeW& W, const double* st, int v, int64_t q, const int loc[3]) const {
+        int off[3];
+        const double* c[3];
+        for (int d = 0; d < 3; ++d) {
+            const int o = ho_offset(loc[d], F.n[d], F.crd[q][d] & 1);
+            off[d] = o - 3;
+            c[d] = W.res[o];
+        }
+        return contract555(c[0], c[1], c[2], [&](int a, int b, int cc) {
+            return st[A.cc(loc[0] + off[0] + a, loc[1] + off[1] + b, loc[2] + off[2] + cc, v, q)];
+        });
+    }
+    void fill_metric(double* st, int v0, int vcnt) const {
+        LagrangeW W;
+        const int ng = F.ng;
+        const int N[3] = {A.Ni, A.Nj, A.Nk};
+        // the virtual coarse metric field on every coarse cell of the domain
+        const int Nc[3] = {F.ntot(0, 0), F.ntot(0, 1), F.ntot(0, 2)};
+        const int64_t ncc = (int64_t)Nc[0] * Nc[1] * Nc[2];
+        std::vector<double> vcm((size_t)(ncc * vcnt));
+        for (int64_t x = 0; x < ncc; ++x) {
+            const int g[3] = {(int)(x % Nc[0]), (int)((x / Nc[0]) % Nc[1]), (int)(x / ((int64_t)Nc[0] * Nc[1]))};
+            const Forest::Owner o = F.owner(0, g);
+            for (int v = 0; v < vcnt; ++v)
+                vcm[(size_t)(v * ncc + x)] = o.cover == 0 ? st[A.cc(o.loc[0], o.loc[1], o.loc[2], v0 + v, o.q)]
+                                                          : restrict_ho(W, st, v0 + v, o.q, o.loc);
+        }
+        auto VC = [&](int v, const int X[3]) {
+            int g[3];
+            for (int d = 0; d < 3; ++d) g[d] = F.map_cc(0, d, X[d]);
+            return vcm[(size_t)(v * ncc + g[0] + (int64_t)Nc[0] * (g[1] + (int64_t)Nc[1] * g[2]))];
+        };
+        std::vector<double> out;
+        std::vector<int64_t> where;
+        for (int64_t q = 0; q < F.nq; ++q) {
+            const int lev = F.level[q];
+            const auto& cr = F.crd[q];
+            for (int k = 0; k < N[2]; ++k)
+                for (int j = 0; j < N[1]; ++j)
+                    for (int i = 0; i < N[0]; ++i) {
+                        const int loc[3] = {i, j, k};
+                        bool ghost = false, outside = false;
+                        int g[3];
+                        for (int d = 0; d < 3; ++d) {
+                            ghost |= loc[d] < ng || loc[d] >= F.n[d] + ng;
+                            const int gg = cr[d] * F.n[d] + loc[d] - ng;
+                            const int Nt = F.ntot(lev, d);
+                            if ((gg < 0 && F.bc[2 * d] != 1) || (gg >= Nt && F.bc[2 * d + 1] != 1)) outside = true;
+                            g[d] = F.map_cc(lev, d, gg);
+                        }
+                        if (!ghost || outside) continue;
+                        const Forest::Owner o = F.owner(lev, g);
+                        for (int v = 0; v < vcnt; ++v) {
+                            double val;
+                            if (o.cover == lev) val = st[A.cc(o.loc[0], o.loc[1], o.loc[2], v0 + v, o.q)];
+                            else if (lev == 0) val = VC(v, g);
+                            else {
+                                const int X[3] = {g[0] / 2, g[1] / 2, g[2] / 2};
+                                const double* c[3] = {W.pro[g[0] & 1], W.pro[g[1] & 1], W.pro[g[2] & 1]};
+                                val = contract555(c[0], c[1], c[2], [&](int a, int b, int cc) {
+                                    const int Y[3] = {X[0] + a - 2, X[1] + b - 2, X[2] + cc - 2};
+                                    return VC(v, Y);
+                                });
+                            }
+                            out.push_back(val);
+                            where.push_back(A.cc(i, j, k, v0 + v, q));
+                        }
+                    }
+        }
+        for (size_t x = 0; x < out.size(); ++x) st[where[x]] = out[x];
     }
 
     // ---- refluxing -------------------------------------------------------------

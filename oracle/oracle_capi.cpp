@@ -102,6 +102,7 @@ struct StepperBase {
     virtual int64_t fatal() = 0;
     virtual int64_t fofc_flagged() = 0;
     virtual const Forest* forest_of() const = 0;
+    virtual void metric_ghosts(double* st) = 0;
 };
 
 template <typename EOS>
@@ -140,6 +141,42 @@ struct StepperT : StepperBase {
     int64_t fatal() override { return S->n_fatal; }
     int64_t fofc_flagged() override { return S->fofc_cells_flagged; }
     const Forest* forest_of() const override { return forest.get(); }
+    // The Cowling metric's ghost exchange: on a forest copies + 5-point
+    // Lagrange (AmrExchange::fill_metric); on uniform instances copies from
+    // the same-level neighbour or periodic image, none beyond outflow faces.
+    void metric_ghosts(double* st) override {
+        if (exch) { exch->fill_metric(st, GXX_, 16); return; }
+        const Grid& g = L.grid;
+        const int n[3] = {g.nx, g.ny, g.nz}, Ne[3] = {g.Ni, g.Nj, g.Nk}, ng = g.ngz;
+        std::vector<double> out;
+        std::vector<int64_t> where;
+        for (int64_t q = 0; q < g.nq; ++q) {
+            const BlockDesc& bd = L.blk[q];
+            const Instance& I = L.inst[bd.inst];
+            for (int k = 0; k < Ne[2]; ++k)
+                for (int j = 0; j < Ne[1]; ++j)
+                    for (int i = 0; i < Ne[0]; ++i) {
+                        const int loc[3] = {i, j, k};
+                        bool ghost = false, outside = false;
+                        int sb[3], sl[3];
+                        for (int d = 0; d < 3; ++d) {
+                            ghost |= loc[d] < ng || loc[d] >= n[d] + ng;
+                            const int Nt = I.nb[d] * n[d];
+                            int gg = bd.b[d] * n[d] + loc[d] - ng;
+                            if ((gg < 0 && I.bc[2 * d] != BC_PERIODIC) || (gg >= Nt && I.bc[2 * d + 1] != BC_PERIODIC)) outside = true;
+                            gg = ((gg % Nt) + Nt) % Nt;
+                            sb[d] = gg / n[d]; sl[d] = gg % n[d] + ng;
+                        }
+                        if (!ghost || outside) continue;
+                        const int64_t qs = I.q0 + sb[0] + (int64_t)I.nb[0] * (sb[1] + (int64_t)I.nb[1] * sb[2]);
+                        for (int v = GXX_; v < GXX_ + 16; ++v) {
+                            out.push_back(st[g.cc(sl[0], sl[1], sl[2], v, qs, NSTATE)]);
+                            where.push_back(g.cc(i, j, k, v, q, NSTATE));
+                        }
+                    }
+        }
+        for (size_t x = 0; x < out.size(); ++x) st[where[x]] = out[x];
+    }
 };
 
 // Two-level forest: one 1x1x1 instance per block at its level's spacing, the
@@ -326,6 +363,7 @@ double orc_find_dt(void* h, const double* st, const double* aux, double cfl) {
 }
 int64_t orc_fatal_count(void* h) { return static_cast<StepperBase*>(h)->fatal(); }
 int64_t orc_fofc_flagged(void* h) { return static_cast<StepperBase*>(h)->fofc_flagged(); }
+void orc_fill_metric_ghosts(void* h, double* st) { static_cast<StepperBase*>(h)->metric_ghosts(st); }
 int orc_forest_block(void* h, int64_t q, int32_t* level, int32_t coords[3]) {
     const Forest* F = static_cast<StepperBase*>(h)->forest_of();
     if (!F || q < 0 || q >= F->nq) return -1;

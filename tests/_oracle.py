@@ -40,6 +40,7 @@ def olib(fast=False):
     L.orc_stepper_create.restype = P
     L.orc_stepper_create_forest.argtypes = [P, P, P]
     L.orc_stepper_create_forest.restype = P
+    L.orc_fill_metric_ghosts.argtypes = [P, P]
     L.orc_forest_block.argtypes = [P, i64, P, P]
     L.orc_forest_block.restype = i32
     L.orc_stepper_destroy.argtypes = [P]
@@ -118,6 +119,9 @@ class Stepper:
 
     def fatal(self):
         return self.L.orc_fatal_count(self.h)
+
+    def metric_ghosts(self, state):
+        self.L.orc_fill_metric_ghosts(self.h, p(state))
 
     def forest_block(self, q):
         lev = np.zeros(1, dtype=np.int32)

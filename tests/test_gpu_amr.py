@@ -194,12 +194,19 @@ def test_gpu_torus_forest_rk3(dev):
     say(f"[torus forest] nq={wl.grid.nq}")
     st, B, aux = wl.copy_arrays()
     o = O.Stepper(wl)
+    st0 = st.copy()
+    o.metric_ghosts(st)          # the reference's ghost metric (copies + 5-point Lagrange)
+    assert not np.array_equal(st[:, 7:23], st0[:, 7:23])
+    init = st.copy()
     dt = 0.25 * wl.extra["dx"] / 2
     o.rk3(dt, st, B, aux)
     res = []
     for cache in (False, True):
         ctx = ctx_of(wl)
-        sg, Bg, ag = to_dev(wl.state, dev), [to_dev(b, dev) for b in wl.B], to_dev(wl.aux, dev)
+        sg, Bg, ag = to_dev(st0, dev), [to_dev(b, dev) for b in wl.B], to_dev(wl.aux, dev)
+        ctx.fill_metric_ghosts(sg)
+        torch.cuda.synchronize()
+        assert linf_rel(to_host(sg)[:, 7:23], init[:, 7:23]) < 1e-14
         if cache:
             ctx.set_static_metric(sg)
         dtd = torch.tensor([dt], dtype=torch.float64, device=dev)
@@ -400,3 +407,30 @@ def test_gpu_fully_refined_forest_equals_uniform_grid(dev):
         assert linf_rel(Bf[d], Bu[d][qmap]) < 1e-13, d
     for v in (ga.RHO, ga.PRESS, ga.ZVECX, ga.BX):
         assert linf_rel(af[:, v], au[qmap][:, v]) < 1e-13, v
+
+
+@pytest.mark.parametrize("name", ["centre", "outflow"])
+def test_gpu_forest_metric_exchange(dev, name):
+    """grmhd_fill_metric_ghosts against the oracle: copies bit-exact, 5-point
+    Lagrange restriction / prolongation to 1e-14 (the GPU contracts the pair
+    sums into FMAs), ghosts beyond outflow faces untouched."""
+    import torch
+    from test_oracle_amr import _metric_field
+    wl = W.forest_wave(n=8, ngz=4, **FORESTS[name])
+    o = O.Stepper(wl)
+    f = lambda x, y, z: np.cos(0.37 * x) * (1.0 + 0.1 * y) + np.exp(-0.01 * (x * x + z * z)) * np.sin(0.2 * z + 0.3)
+    st, B, aux, blocks, index = _metric_field(wl, o, f)
+    st[:, 8] = 0.5 * st[:, 7] + 0.25     # a second metric variable
+    n, ngz = wl.grid.nx, wl.grid.ngz
+    gm = ghost_mask(n, ngz)
+    st[:, 7:9][:, :, gm] = -3.0           # ghosts to be filled (or kept, beyond outflow faces)
+    sg = to_dev(st, dev)
+    o.metric_ghosts(st)
+    ctx = ctx_of(wl)
+    ctx.fill_metric_ghosts(sg)
+    torch.cuda.synchronize()
+    gs = to_host(sg)
+    for v in (7, 8):
+        assert linf_rel(gs[:, v], st[:, v]) < 1e-14, v
+    assert np.array_equal(gs[:, 9:], st[:, 9:])
+    assert np.array_equal(gs[:, :7], st[:, :7])

@@ -396,3 +396,59 @@ def test_fully_refined_forest_equals_uniform_grid():
         assert rel(af[:, v], au[qmap][:, v]) < 1e-13, v
     for d in range(3):
         assert rel(Bf[d], Bu[d][qmap]) < 1e-13, d
+
+
+def _metric_field(wl, o, fun):
+    """Fill state var GXX with fun(u, v, w) of cell-centre coordinates
+    (centred on the domain, in coarse cells) in every cell of every block."""
+    blocks, index = geometry(o, wl)
+    st, B, aux = wl.copy_arrays()
+    n, ng = wl.grid.nx, wl.grid.ngz
+    for q, (lev, crd) in enumerate(blocks):
+        h = 1.0 / (1 << lev)
+        c = [((crd[d] * n + np.arange(-ng, n + ng) + 0.5) - 0.5 * (wl.forest.nb[d] * n << lev)) * h for d in range(3)]
+        st[q, 7] = fun(c[0][None, None, :], c[1][None, :, None], c[2][:, None, None])
+    return st, B, aux, blocks, index
+
+
+def test_metric_exchange_reproduces_quartics():
+    """Cowling-metric ghost exchange (copies + 5-point Lagrange prolongation /
+    restriction): a quartic polynomial of position is reproduced in every
+    in-domain ghost cell of every block."""
+    wl = W.forest_wave(nb=(4, 4, 4), n=8)
+    o = Stepper(wl)
+    f = lambda x, y, z: 1.0 + 0.3 * x - 0.2 * y + 0.1 * z + 0.01 * x * x * y - 0.007 * y * z * z + 4e-4 * x ** 4 \
+        - 3e-4 * (x * y) ** 2 + 2.5e-4 * z ** 3 * y
+    st, B, aux, blocks, index = _metric_field(wl, o, f)
+    exact = st.copy()
+    n, ng = wl.grid.nx, wl.grid.ngz
+    gm = ghost_mask(n, ng)
+    st[:, 7][:, gm] = 0.0
+    o.metric_ghosts(st)
+    worst = 0.0
+    for q, (lev, crd) in enumerate(blocks):
+        N = 4 * n << lev
+        ins = [((crd[d] * n + np.arange(-ng, n + ng)) >= 0) & ((crd[d] * n + np.arange(-ng, n + ng)) < N)
+               for d in range(3)]
+        m = gm & ins[0][None, None, :] & ins[1][None, :, None] & ins[2][:, None, None]
+        worst = max(worst, float(np.abs(st[q, 7][m] - exact[q, 7][m]).max()))
+    assert worst < 1e-13 * np.abs(exact[:, 7]).max(), worst
+
+
+def test_metric_exchange_is_mirror_symmetric():
+    """Data even in x on a forest symmetric under x -> -x: the exchanged ghosts
+    are bit-for-bit mirror images (the half-dependent restriction stencils and
+    the pair-symmetric contractions of pr_helpers.hh:52-58, 222-330; a fixed
+    stencil choice leaves O(h^5) asymmetries)."""
+    wl = W.forest_wave(nb=(4, 4, 4), n=8)
+    o = Stepper(wl)
+    f = lambda x, y, z: np.cos(0.37 * x) * (1.0 + 0.1 * y) + np.exp(-0.01 * (x * x + z * z)) * np.sin(0.2 * z + 0.3)
+    st, B, aux, blocks, index = _metric_field(wl, o, f)
+    n, ng = wl.grid.nx, wl.grid.ngz
+    gm = ghost_mask(n, ng)
+    st[:, 7][:, gm] = 0.0
+    o.metric_ghosts(st)
+    for q, (lev, crd) in enumerate(blocks):
+        mc = ((wl.forest.nb[0] << lev) - 1 - crd[0], crd[1], crd[2])
+        qm = index[(lev, mc)]
+        assert np.array_equal(st[q, 7], st[qm, 7][:, :, ::-1]), (q, qm)
