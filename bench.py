@@ -415,8 +415,8 @@ def main():
                     help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU halo exchange is a design, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported: the kernels run one "
+              "GPU's share of the blocks, the multi-GPU halo exchange is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
