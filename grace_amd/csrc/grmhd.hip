@@ -217,6 +217,10 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     double pL[NPRIM], pR[NPRIM];
 #pragma unroll
     for (int v = 0; v < 10; ++v) {
+        // Work whose result is never read: the normal B component is replaced
+        // by the face value below, and a ring face (UCT velocities and wave
+        // speeds only) reads neither Ye nor the entropy.
+        if (v == 7 + D || (ring && (v == 4 || v == 6))) { pL[rloc[v]] = 0.0; pR[rloc[v]] = 0.0; continue; }
         const double* P = A + ridx[v] * nc + c0;
         double uL, uR;
         if (excise_override) { uL = P[-st]; uR = P[0]; }
