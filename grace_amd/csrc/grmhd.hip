@@ -1321,6 +1321,10 @@ struct grmhd_ctx {
     bool timing = false;
     std::vector<TimerRec> timers;
     std::vector<hipEvent_t> ev_pool;
+    // side streams and fork/join events of the concurrent RK3 stages
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fev[6] = {};
+    hipStream_t flux_s[3] = {nullptr, nullptr, nullptr};   // per-direction flux streams while set
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1420,6 +1424,8 @@ static void free_ctx(grmhd_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& t : c->timers) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto e : c->fev) if (e) (void)hipEventDestroy(e);
+    for (auto t : c->side) if (t) (void)hipStreamDestroy(t);
 }
 
 // Host description of a two-level forest (grmhd_ctx_create_forest).
@@ -1736,7 +1742,7 @@ static grmhd_status launch_check(const char* what) {
 // (donor cell + LLF); otherwise every face the later passes read.
 template <typename EOS, int REC, bool FLAT>
 static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const double* ms, int mnv,
-                         const double* const B[3], hipStream_t s, const Opts& O, uint8_t* const* only) {
+                         const double* const B[3], hipStream_t s0, const Opts& O, uint8_t* const* only) {
     const DevGrid& G = c->G;
     const bool uct = c->P.emf != GRMHD_EMF_GS;
     int nf[3];
@@ -1747,6 +1753,7 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
     }
     static const char* fam[3] = {"flux_x", "flux_y", "flux_z"};
     for (int d = 0; d < 3; ++d) {
+        const hipStream_t s = c->flux_s[d] ? c->flux_s[d] : s0;
         Scope sc(c, only ? "fofc" : fam[d], s);
         const int bs = c->tune.flux[d];
         const int wpb = (int)nblk_bs(nf[d], bs);
@@ -1886,7 +1893,8 @@ static void launch_reflux(grmhd_ctx* c, hipStream_t s) {
 // c2p over the cells of every block: the interior cells, plus either every
 // ghost cell (all) or only the ghost cells that need their own c2p.
 static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms, int mnv, const double* const B[3],
-                             double* aux, bool with_stats, bool selective, hipStream_t s) {
+                             double* aux, bool with_stats, bool selective, hipStream_t s,
+                             hipStream_t s_list = nullptr) {
     Scope sc(c, "c2p", s);
     const DevGrid& G = c->G;
     double* ec = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
@@ -1895,10 +1903,14 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     const int bs = c->tune.c2p;
     int64_t npart = 0;
     const MlpDev md = c->mlp ? c->mlp->dev() : MlpDev{};
+    // the explicit-list launch may go to its own stream (no statistics then:
+    // k_stats_final below reads every partial record on s)
+    const hipStream_t sl = (s_list && !with_stats) ? s_list : s;
     auto launch = [&](int cset, int ne, dim3 grid, int wpb) {
         DevStats* part = with_stats ? c->stats_part + npart : nullptr;
         npart += (int64_t)grid.x * grid.y;
-#define C2PL(EOSV, FL, LR) k_c2p_cells<decltype(EOSV), FL, LR><<<grid, bs, 0, s>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, wpb, md)
+        const hipStream_t ls = cset == CS_LIST ? sl : s;
+#define C2PL(EOSV, FL, LR) k_c2p_cells<decltype(EOSV), FL, LR><<<grid, bs, 0, ls>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, wpb, md)
         if (c->mlp) {
             if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, true); else C2PL(c->ideal, false, true); }
             else { if (flat) C2PL(c->hybrid, true, true); else C2PL(c->hybrid, false, true); }
@@ -2240,6 +2252,10 @@ extern "C" grmhd_status grmhd_compute_dt(grmhd_ctx* c, const double* state, cons
 static grmhd_status ensure_stage_buffers(grmhd_ctx* c) {
     if (c->U1) return GRMHD_OK;
     const DevGrid& G = c->G;
+    for (auto& t : c->side)
+        if (!t) HIP_TRY(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    for (auto& e : c->fev)
+        if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIP_TRY(hipMalloc(&c->U1, sizeof(double) * NHRSC * G.ncell * G.nq));
     HIP_TRY(hipMalloc(&c->U2, sizeof(double) * NHRSC * G.ncell * G.nq));
     for (int d = 0; d < 3; ++d) {
@@ -2271,9 +2287,29 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         {c->U2, NHRSC, state, NSTATE, c->U1, NHRSC, 0.75, 0.25, 0.25, nB2, Bc, B1c, c->U1, NHRSC, B1c},
         {state, NSTATE, state, NSTATE, c->U2, NHRSC, 1. / 3, 2. / 3., 2. / 3., B, Bc, B2c, c->U2, NHRSC, B2c},
     };
+    // Uniform grids without FOFC run independent launches of a stage side by
+    // side: the three flux directions; then the ghost-face copy and the c2p
+    // of the outflow ghosts next to the c2p of the interior cells.  Ordering
+    // by events, so the same sequence is captured into the step's graph.
+    // (Per-family timing runs everything on s.)
+    const bool conc = !c->timing && !c->forest && !c->P.fofc && c->side[0] && c->side[1];
+    auto fork = [&](int k, hipStream_t from, hipStream_t to) {
+        (void)hipEventRecord(c->fev[k], from);
+        (void)hipStreamWaitEvent(to, c->fev[k], 0);
+    };
     for (int sidx = 0; sidx < 3; ++sidx) {
         Stage& S = st[sidx];
-        launch_fluxes(c, aux, state, NSTATE, S.oB, s);
+        if (conc) {
+            fork(0, s, c->side[0]);
+            fork(0, s, c->side[1]);
+            c->flux_s[1] = c->side[0]; c->flux_s[2] = c->side[1];
+            launch_fluxes(c, aux, state, NSTATE, S.oB, s);
+            c->flux_s[1] = c->flux_s[2] = nullptr;
+            fork(1, c->side[0], s);
+            fork(2, c->side[1], s);
+        } else {
+            launch_fluxes(c, aux, state, NSTATE, S.oB, s);
+        }
         launch_emf(c, S.oB, s);
         if (c->P.fofc)
             launch_fofc(c, S.ast, S.anv, S.bst, S.bnv, S.ca, S.cb, state, S.oB, aux, dt_dev, S.dtf, s);
@@ -2284,15 +2320,33 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         for (int d = 0; d < 3; ++d) { a.nB[d] = S.nB[d]; a.aB[d] = S.aB[d]; a.bB[d] = S.bB ? S.bB[d] : nullptr; }
         a.dt = dt_dev; a.dtfact = S.dtf;
         launch_update(c, a, s);
-        {   // ghost faces everywhere; conserved only where a ghost runs its own c2p
-            // (everywhere with FOFC: its next-stage tests read the ghost layer)
-            GatherArgs g{};
-            g.st = S.nst; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC; g.cell_mode = c->P.fofc ? 1 : 2;
-            for (int d = 0; d < 3; ++d) g.B[d] = S.nB[d];
-            launch_gather(c, g, s);
-        }
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
-        launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s);
+        if (conc) {
+            // side[0]: ghost faces, then (after the conserved copy on s) the
+            // c2p of the outflow ghosts; s: conserved copy, interior c2p.  The
+            // interior c2p may rewrite interior conserved, so the copy that
+            // reads them precedes it on s.
+            fork(3, s, c->side[0]);
+            GatherArgs gf{};
+            for (int d = 0; d < 3; ++d) gf.B[d] = S.nB[d];
+            gf.cell_mode = 0;
+            launch_gather(c, gf, c->side[0]);
+            GatherArgs gc{};
+            gc.st = S.nst; gc.nv = S.nnv; gc.v0 = 0; gc.vc = NHRSC; gc.cell_mode = 2;
+            launch_gather(c, gc, s);
+            fork(4, s, c->side[0]);
+            launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s, c->side[0]);
+            fork(5, c->side[0], s);
+        } else {
+            {   // ghost faces everywhere; conserved only where a ghost runs its own c2p
+                // (everywhere with FOFC: its next-stage tests read the ghost layer)
+                GatherArgs g{};
+                g.st = S.nst; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC; g.cell_mode = c->P.fofc ? 1 : 2;
+                for (int d = 0; d < 3; ++d) g.B[d] = S.nB[d];
+                launch_gather(c, g, s);
+            }
+            launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s);
+        }
         {   // c2p outputs into the other ghost cells: the reconstructed
             // primitives (and the GS cell E, which the next stage's edge EMFs
             // read one cell into the ghosts) between stages, everything after
