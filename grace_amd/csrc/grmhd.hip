@@ -2287,12 +2287,16 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         {c->U2, NHRSC, state, NSTATE, c->U1, NHRSC, 0.75, 0.25, 0.25, nB2, Bc, B1c, c->U1, NHRSC, B1c},
         {state, NSTATE, state, NSTATE, c->U2, NHRSC, 1. / 3, 2. / 3., 2. / 3., B, Bc, B2c, c->U2, NHRSC, B2c},
     };
-    // Uniform grids without FOFC run independent launches of a stage side by
-    // side: the three flux directions; then the ghost-face copy and the c2p
-    // of the outflow ghosts next to the c2p of the interior cells.  Ordering
-    // by events, so the same sequence is captured into the step's graph.
-    // (Per-family timing runs everything on s.)
-    const bool conc = !c->timing && !c->forest && !c->P.fofc && c->side[0] && c->side[1];
+    // Without FOFC, independent launches of a stage run side by side: the
+    // three flux directions; then the ghost-face passes (copies and, on a
+    // forest, the coarse-fine face passes) and the c2p of the ghosts that run
+    // their own (outflow and coarse-fine ghosts) next to the c2p of the
+    // interior cells.  The conserved fills those ghost c2p read (copies and
+    // coarse-fine prolongation/restriction from interior cells) stay on s
+    // ahead of the interior c2p, which may rewrite interior conserved.
+    // Ordering by events, so the same sequence is captured into the step's
+    // graph.  (Per-family timing runs everything on s.)
+    const bool conc = !c->timing && !c->P.fofc && c->side[0] && c->side[1];
     auto fork = [&](int k, hipStream_t from, hipStream_t to) {
         (void)hipEventRecord(c->fev[k], from);
         (void)hipStreamWaitEvent(to, c->fev[k], 0);
@@ -2322,10 +2326,9 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         launch_update(c, a, s);
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
         if (conc) {
-            // side[0]: ghost faces, then (after the conserved copy on s) the
-            // c2p of the outflow ghosts; s: conserved copy, interior c2p.  The
-            // interior c2p may rewrite interior conserved, so the copy that
-            // reads them precedes it on s.
+            // side[0]: ghost faces, then (after the conserved fills on s) the
+            // c2p of the ghosts that run their own; s: conserved fills,
+            // interior c2p.
             fork(3, s, c->side[0]);
             GatherArgs gf{};
             for (int d = 0; d < 3; ++d) gf.B[d] = S.nB[d];

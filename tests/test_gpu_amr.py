@@ -165,6 +165,37 @@ def test_gpu_forest_graph_matches_eager(dev):
         assert torch.equal(aa, ab), it
 
 
+@pytest.mark.parametrize("name", ["three", "outflow"])
+def test_gpu_forest_concurrent_stages_match_serial(dev, name):
+    """Forest RK3 with the stage launches side by side (the default) against
+    the one-stream order (per-family timing on): same bits, eager and graph."""
+    import torch
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, **FORESTS[name])
+    say(f"[forest concurrent] {name}")
+    st, B, aux = wl.copy_arrays()
+    o = O.Stepper(wl)
+    o.ghosts(st, B)
+    ser, con = ctx_of(wl), ctx_of(wl)
+    ser.set_timing(1)
+    sa, Ba, aa = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
+    ser.compute_auxiliaries(sa, Ba, aa)
+    sb, Bb, ab = sa.clone(), [b.clone() for b in Ba], aa.clone()
+    sc, Bc, ac = sa.clone(), [b.clone() for b in Ba], aa.clone()
+    dtd = torch.tensor([0.25 * wl.extra["dx"] / 2], dtype=torch.float64, device=dev)
+    s = torch.cuda.Stream()
+    for it in range(3):
+        ser.rk3_step(dtd, sa, Ba, aa)
+        con.rk3_step(dtd, sb, Bb, ab)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            con.rk3_step(dtd, sc, Bc, ac, stream=s.cuda_stream, graph=True)
+        s.synchronize()
+        for x, y in ((sa, sb), (sa, sc), (aa, ab), (aa, ac)):
+            assert torch.equal(x, y), it
+        for d in range(3):
+            assert torch.equal(Ba[d], Bb[d]) and torch.equal(Ba[d], Bc[d]), it
+
+
 def test_forest_rejects_bad_input(dev):
     wl = W.forest_wave(n=8, ngz=4, **FORESTS["one"])
     p = wl.params
