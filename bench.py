@@ -414,6 +414,9 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="every launch on one stream (grmhd_params.stage_streams = 0), for kernel traces "
+                         "whose launch durations are not overlapped")
     ap.add_argument("--only-torus", action="store_true",
                     help="only run the two-level-forest torus leg and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -443,6 +446,8 @@ def main():
         return
     log("[bench] building 128^3 blast wave (512 blocks of 16^3)")
     wl = W.blast(nb=(8, 8, 8), n=16)
+    if args.one_stream:
+        wl.params.stage_streams = 0
     ctx = ga.Context(wl.params, wl.grid, wl.instances)
     st = torch.from_numpy(wl.state).to(dev)
     B = [torch.from_numpy(b).to(dev) for b in wl.B]
@@ -550,7 +555,8 @@ def main():
         "data": "synthetic (seeded blast-wave initial data, seed 20261202)",
         "config": {"workload": "blast3d_128cube_512x16cube_blocks_ngz3_wenoz_hlle_uct_rk3_plain_c2p",
                    "zones": wl.zones, "blocks": int(wl.grid.nq), "c2p": "plain Kastaun (FP64)",
-                   "graph": use_graph, "gpu_event_ms_per_step": gpu_ms / args.steps, "tuning": tuning},
+                   "graph": use_graph, "stage_streams": int(wl.params.stage_streams),
+                   "gpu_event_ms_per_step": gpu_ms / args.steps, "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
         "c2p_learned_vs_plain": c2p_leg,

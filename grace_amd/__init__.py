@@ -49,7 +49,7 @@ class Params(C.Structure):
         ("recon", C.c_int32), ("riemann", C.c_int32), ("emf", C.c_int32), ("thermo_recon", C.c_int32),
         ("fd_order", C.c_int32), ("rusanov_c_limit", C.c_double), ("flat_metric", C.c_int32),
         ("fofc", C.c_int32), ("fofc_dmp_enable", C.c_int32), ("fofc_dmp_M", C.c_double),
-        ("fofc_ghost_emf", C.c_int32),
+        ("fofc_ghost_emf", C.c_int32), ("stage_streams", C.c_int32),
     ]
 
 

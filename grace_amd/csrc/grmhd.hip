@@ -1382,6 +1382,7 @@ extern "C" void grmhd_default_params(grmhd_params* p) {
     p->thermo_recon = GRMHD_THERMO_TEMP; p->fd_order = 6; p->rusanov_c_limit = 0.0;
     p->flat_metric = 0;
     p->fofc = 0; p->fofc_dmp_enable = 1; p->fofc_dmp_M = 1.2; p->fofc_ghost_emf = 0;
+    p->stage_streams = 1;
 }
 
 static void build_eos(grmhd_ctx* c) {
@@ -1457,6 +1458,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     if (p->fofc && g->ngz < 4) { set_err("FOFC requires ngz >= 4"); return GRMHD_ERR_INVALID; }
     if (p->fofc && !(p->fofc_dmp_M > 0)) { set_err("fofc_dmp_M must be > 0"); return GRMHD_ERR_INVALID; }
     if (p->fofc_ghost_emf != 0 && p->fofc_ghost_emf != 1) { set_err("fofc_ghost_emf must be 0 or 1"); return GRMHD_ERR_INVALID; }
+    if (p->stage_streams != 0 && p->stage_streams != 1) { set_err("stage_streams must be 0 or 1"); return GRMHD_ERR_INVALID; }
     // instance / block tables
     std::vector<DevBlk> blk(g->nq, DevBlk{-1, {0, 0, 0}});
     for (int a = 0; a < n_inst; ++a) {
@@ -2295,8 +2297,9 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
     // coarse-fine prolongation/restriction from interior cells) stay on s
     // ahead of the interior c2p, which may rewrite interior conserved.
     // Ordering by events, so the same sequence is captured into the step's
-    // graph.  (Per-family timing runs everything on s.)
-    const bool conc = !c->timing && !c->P.fofc && c->side[0] && c->side[1];
+    // graph.  (grmhd_params.stage_streams = 0 or per-family timing: every
+    // launch on s.)
+    const bool conc = c->P.stage_streams && !c->timing && !c->P.fofc && c->side[0] && c->side[1];
     auto fork = [&](int k, hipStream_t from, hipStream_t to) {
         (void)hipEventRecord(c->fev[k], from);
         (void)hipStreamWaitEvent(to, c->fev[k], 0);

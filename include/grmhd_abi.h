@@ -97,6 +97,14 @@ typedef struct {
      * 1: compute the GS edge EMFs one ghost layer wider when FOFC is on, so
      * the ghost-layer flags equal the neighbour's interior flags. */
     int32_t fofc_ghost_emf;
+    /* 1 (default): independent launches of an RK3 stage run side by side on
+     * two library-owned streams (the three flux directions; ghost faces and
+     * the c2p of ghosts that run their own next to the interior c2p), joined
+     * back into the caller's stream by events (also inside the captured
+     * graph); results are bit-identical to 0, which issues every launch on
+     * the caller's stream (for kernel traces with one launch at a time).
+     * FOFC runs use one stream either way. */
+    int32_t stage_streams;
 } grmhd_params;
 
 /* Block grid: every block has the same interior extents. */

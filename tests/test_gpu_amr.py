@@ -168,15 +168,17 @@ def test_gpu_forest_graph_matches_eager(dev):
 @pytest.mark.parametrize("name", ["three", "outflow"])
 def test_gpu_forest_concurrent_stages_match_serial(dev, name):
     """Forest RK3 with the stage launches side by side (the default) against
-    the one-stream order (per-family timing on): same bits, eager and graph."""
+    the one-stream order (stage_streams = 0): same bits, eager and graph."""
     import torch
     wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, **FORESTS[name])
     say(f"[forest concurrent] {name}")
     st, B, aux = wl.copy_arrays()
     o = O.Stepper(wl)
     o.ghosts(st, B)
-    ser, con = ctx_of(wl), ctx_of(wl)
-    ser.set_timing(1)
+    p1 = type(wl.params).from_buffer_copy(wl.params)
+    p1.stage_streams = 0
+    ser = ga.Context(p1, wl.grid, wl.instances, forest=wl.forest)
+    con = ctx_of(wl)
     sa, Ba, aa = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
     ser.compute_auxiliaries(sa, Ba, aa)
     sb, Bb, ab = sa.clone(), [b.clone() for b in Ba], aa.clone()

@@ -328,14 +328,16 @@ def test_gpu_rk3_graph_matches_eager(dev):
 def test_gpu_rk3_concurrent_stages_match_serial(dev):
     """The RK3 step runs independent launches of a stage on side streams
     (flux directions; ghost faces and the outflow-ghost c2p beside the interior
-    c2p).  With per-family timing on, every launch goes to one stream: both
-    orders must give the same bits, eager and replayed from the graph."""
+    c2p).  With stage_streams = 0 every launch goes to the caller's stream:
+    both orders must give the same bits, eager and replayed from the graph."""
     import torch
     wl = small_blast()
     say("[rk3] concurrent stages vs serial")
     dt = 0.25 * wl.extra["dx"]
-    ser = ga.Context(wl.params, wl.grid, wl.instances)
-    ser.set_timing(1)
+    assert wl.params.stage_streams == 1
+    p1 = type(wl.params).from_buffer_copy(wl.params)
+    p1.stage_streams = 0
+    ser = ga.Context(p1, wl.grid, wl.instances)
     con = ga.Context(wl.params, wl.grid, wl.instances)
     sa, Ba, aa = dev_arrays(wl, dev)
     sb, Bb, ab = dev_arrays(wl, dev)
