@@ -499,6 +499,7 @@ struct UpdArgs {
     const double* dt; double dtfact;
     const DevInst* inst; const DevBlk* blk;
     const double* mder;              // cached metric derivatives (30 per interior cell) or null
+    int part;                        // 0: cells and faces; 1: cells only (fluxes, sources); 2: faces only (CT)
 };
 
 GHD double fdder(const double* P, int64_t s, double ih, int order) {
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
     const double dtf = (*a.dt) * a.dtfact;
     const bool ci = i < G.n[0] + G.ng, cj = j < G.n[1] + G.ng, ck = k < G.n[2] + G.ng;
     const int64_t nc = G.ncell;
-    if (ci && cj && ck) {
+    if (a.part != 2 && ci && cj && ck) {
         const int64_t c = cidx(G, i, j, k);
         double* Nn = a.nst + q * (int64_t)a.nnv * nc + c;
         const double* An = a.ast + q * (int64_t)a.anv * nc + c;
@@ -656,6 +657,7 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
 #pragma unroll
         for (int v = 0; v < NHRSC; ++v) Nn[v * nc] = u[v];
     }
+    if (a.part == 1) return;
     const double* E = a.emf + q * 3 * G.nedge;
     auto Ex = [&](int ii, int jj, int kk) { return E[eidx(G, ii, jj, kk)]; };
     auto Ey = [&](int ii, int jj, int kk) { return E[G.nedge + eidx(G, ii, jj, kk)]; };
@@ -1323,7 +1325,7 @@ struct grmhd_ctx {
     std::vector<hipEvent_t> ev_pool;
     // side streams and fork/join events of the concurrent RK3 stages
     hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t fev[6] = {};
+    hipEvent_t fev[8] = {};
     hipStream_t flux_s[3] = {nullptr, nullptr, nullptr};   // per-direction flux streams while set
     // graph
     hipGraph_t graph = nullptr;
@@ -2317,6 +2319,11 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         } else {
             launch_fluxes(c, aux, state, NSTATE, S.oB, s);
         }
+        // the cell part of the update needs only the fluxes: on a uniform
+        // grid (no reflux between the EMFs and the update) it runs on
+        // side[0] beside the edge EMFs, the face (CT) part after them on s
+        const bool split_update = conc && !c->forest;
+        if (split_update) fork(6, s, c->side[0]);
         launch_emf(c, S.oB, s);
         if (c->P.fofc)
             launch_fofc(c, S.ast, S.anv, S.bst, S.bnv, S.ca, S.cb, state, S.oB, aux, dt_dev, S.dtf, s);
@@ -2326,7 +2333,16 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         a.ca = S.ca; a.cb = S.cb; a.mstate = state; a.mnv = NSTATE; a.aux = aux;
         for (int d = 0; d < 3; ++d) { a.nB[d] = S.nB[d]; a.aB[d] = S.aB[d]; a.bB[d] = S.bB ? S.bB[d] : nullptr; }
         a.dt = dt_dev; a.dtfact = S.dtf;
-        launch_update(c, a, s);
+        if (split_update) {
+            UpdArgs ac = a;
+            ac.part = 1;
+            launch_update(c, ac, c->side[0]);
+            a.part = 2;
+            launch_update(c, a, s);
+            fork(7, c->side[0], s);
+        } else {
+            launch_update(c, a, s);
+        }
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
         if (conc) {
             // side[0]: ghost faces, then (after the conserved fills on s) the
