@@ -1884,14 +1884,15 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
 
 // reflux_correct_fluxes / reflux_correct_emfs (refluxing.cpp:547-806): after
 // the edge EMFs, before the update that consumes both.
-static void launch_reflux(grmhd_ctx* c, hipStream_t s) {
+// part: 0 both, 1 the fluxes only, 2 the edge EMFs only.
+static void launch_reflux(grmhd_ctx* c, hipStream_t s, int part = 0) {
     if (!c->forest) return;
     Scope sc(c, "reflux", s);
     const DevGrid& G = c->G;
-    if (c->n_rflux > 0)
+    if (c->n_rflux > 0 && part != 2)
         k_reflux_flux<<<dim3(nblk(c->n_rflux), NHRSC), 256, 0, s>>>(c->rflux, (int)c->n_rflux, c->flux[0], c->flux[1],
                                                                     c->flux[2], G.nface[0], G.nface[1], G.nface[2]);
-    if (c->n_remf > 0) k_reflux_emf<<<nblk(c->n_remf), 256, 0, s>>>(c->remf, (int)c->n_remf, c->emf);
+    if (c->n_remf > 0 && part != 1) k_reflux_emf<<<nblk(c->n_remf), 256, 0, s>>>(c->remf, (int)c->n_remf, c->emf);
 }
 
 // c2p over the cells of every block: the interior cells, plus either every
@@ -2319,15 +2320,19 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         } else {
             launch_fluxes(c, aux, state, NSTATE, S.oB, s);
         }
-        // the cell part of the update needs only the fluxes: on a uniform
-        // grid (no reflux between the EMFs and the update) it runs on
-        // side[0] beside the edge EMFs, the face (CT) part after them on s
-        const bool split_update = conc && !c->forest;
-        if (split_update) fork(6, s, c->side[0]);
+        // the cell part of the update needs only the fluxes (on a forest
+        // after their reflux): it runs on side[0] beside the edge EMFs, the
+        // face (CT) part after them (and their reflux) on s.  Not with GS
+        // EMFs on a forest: those read the mass fluxes the reflux rewrites.
+        const bool split_update = conc && !(c->forest && c->P.emf == GRMHD_EMF_GS);
+        if (split_update) {
+            fork(6, s, c->side[0]);
+            launch_reflux(c, c->side[0], 1);
+        }
         launch_emf(c, S.oB, s);
         if (c->P.fofc)
             launch_fofc(c, S.ast, S.anv, S.bst, S.bnv, S.ca, S.cb, state, S.oB, aux, dt_dev, S.dtf, s);
-        launch_reflux(c, s);
+        launch_reflux(c, s, split_update ? 2 : 0);
         UpdArgs a{};
         a.nst = S.nst; a.nnv = S.nnv; a.ast = S.ast; a.anv = S.anv; a.bst = S.bst; a.bnv = S.bnv;
         a.ca = S.ca; a.cb = S.cb; a.mstate = state; a.mnv = NSTATE; a.aux = aux;

@@ -165,24 +165,28 @@ def test_gpu_forest_graph_matches_eager(dev):
         assert torch.equal(aa, ab), it
 
 
-@pytest.mark.parametrize("name", ["three", "outflow"])
-def test_gpu_forest_concurrent_stages_match_serial(dev, name):
+@pytest.mark.parametrize("name,emf", [("three", W.EMF_UCT), ("outflow", W.EMF_UCT), ("three", W.EMF_GS)])
+def test_gpu_forest_concurrent_stages_match_serial(dev, name, emf):
     """Forest RK3 with the stage launches side by side (the default) against
     the one-stream order (stage_streams = 0): same bits, eager and graph."""
     import torch
-    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, **FORESTS[name])
-    say(f"[forest concurrent] {name}")
+    wl = W.forest_wave(n=8, ngz=4, b_loop=0.2, p_in=3.0, r_in=0.3, emf=emf, **FORESTS[name])
+    say(f"[forest concurrent] {name} emf {emf}")
     st, B, aux = wl.copy_arrays()
     o = O.Stepper(wl)
     o.ghosts(st, B)
     p1 = type(wl.params).from_buffer_copy(wl.params)
     p1.stage_streams = 0
+    # one context per array set: a context keeps the cell-centred E of its
+    # last c2p, which the next GS step's edge EMFs read
     ser = ga.Context(p1, wl.grid, wl.instances, forest=wl.forest)
-    con = ctx_of(wl)
+    con, cgr = ctx_of(wl), ctx_of(wl)
     sa, Ba, aa = to_dev(st, dev), [to_dev(b, dev) for b in B], to_dev(aux, dev)
-    ser.compute_auxiliaries(sa, Ba, aa)
     sb, Bb, ab = sa.clone(), [b.clone() for b in Ba], aa.clone()
     sc, Bc, ac = sa.clone(), [b.clone() for b in Ba], aa.clone()
+    ser.compute_auxiliaries(sa, Ba, aa)
+    con.compute_auxiliaries(sb, Bb, ab)
+    cgr.compute_auxiliaries(sc, Bc, ac)
     dtd = torch.tensor([0.25 * wl.extra["dx"] / 2], dtype=torch.float64, device=dev)
     s = torch.cuda.Stream()
     for it in range(3):
@@ -190,7 +194,7 @@ def test_gpu_forest_concurrent_stages_match_serial(dev, name):
         con.rk3_step(dtd, sb, Bb, ab)
         torch.cuda.synchronize()
         with torch.cuda.stream(s):
-            con.rk3_step(dtd, sc, Bc, ac, stream=s.cuda_stream, graph=True)
+            cgr.rk3_step(dtd, sc, Bc, ac, stream=s.cuda_stream, graph=True)
         s.synchronize()
         for x, y in ((sa, sb), (sa, sc), (aa, ab), (aa, ac)):
             assert torch.equal(x, y), it
