@@ -2275,7 +2275,6 @@ static grmhd_status ensure_stage_buffers(grmhd_ctx* c) {
 static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, double* const B[3], double* aux,
                          hipStream_t s) {
     const DevGrid& G = c->G;
-    k_reset_diag<<<nblk(G.ncell * G.nq), 256, 0, s>>>(G.ncell, G.nq, aux);
     const double* const Bc[3] = {B[0], B[1], B[2]};
     const double* const B1c[3] = {c->B1[0], c->B1[1], c->B1[2]};
     const double* const B2c[3] = {c->B2[0], c->B2[1], c->B2[2]};
@@ -2307,11 +2306,15 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
         (void)hipEventRecord(c->fev[k], from);
         (void)hipStreamWaitEvent(to, c->fev[k], 0);
     };
+    // the c2p error fields are cleared once per step; only the c2p reads
+    // them, so with side streams the clear runs beside the first fluxes
+    if (!conc) k_reset_diag<<<nblk(G.ncell * G.nq), 256, 0, s>>>(G.ncell, G.nq, aux);
     for (int sidx = 0; sidx < 3; ++sidx) {
         Stage& S = st[sidx];
         if (conc) {
             fork(0, s, c->side[0]);
             fork(0, s, c->side[1]);
+            if (sidx == 0) k_reset_diag<<<nblk(G.ncell * G.nq), 256, 0, c->side[1]>>>(G.ncell, G.nq, aux);
             c->flux_s[1] = c->side[0]; c->flux_s[2] = c->side[1];
             launch_fluxes(c, aux, state, NSTATE, S.oB, s);
             c->flux_s[1] = c->flux_s[2] = nullptr;
