@@ -156,6 +156,23 @@ inline double hll(double fL, double fR, double uL, double uR, double cmin, doubl
     return (cmax * fL + cmin * fR - cmax * cmin * (uR - uL)) / (cmax + cmin);   // riemann_solvers.hh:69
 }
 
+// face_interp_4 (grmhd_helpers.hh:360-380): value at the face between u1 and
+// u2 from the four cells u0..u3 around it, the inner and outer pairs summed
+// first (pair-symmetric, so mirrored data give bit-mirrored results).
+inline double face_interp_4(double u0, double u1, double u2, double u3) {
+    const double si = u1 + u2, so = u0 + u3;
+    return 0.5625 * si + (-0.0625) * so;
+}
+
+// Centred first derivative of order 2, 4 or 6 (fd_subexpressions.hh:21-57)
+// from u[0..6] = u(-3..3) at spacing 1/ih.
+inline double fd_der_central(int order, const double u[7], double ih) {
+    auto D = [&](int s) { return u[3 + s] - u[3 - s]; };
+    if (order == 2) return (1.0 / 2.0) * ih * (D(1));
+    if (order == 4) return (1.0 / 12.0) * ih * (+8.0 * D(1) - D(2));
+    return (1.0 / 60.0) * ih * (+45.0 * D(1) - 9.0 * D(2) + D(3));
+}
+
 template <typename EOS>
 struct Stepper {
     const Layout& L;
@@ -193,14 +210,13 @@ struct Stepper {
         return Metric(gg, b, st[g.cc(i, j, k, ALP_, q, NSTATE)]);
     }
 
-    // face_interp_4 (grmhd_helpers.hh:360-380): pair-symmetric 4-point interpolation
+    // face_interp_4 (grmhd_helpers.hh:360-380) along d at face i-1/2
     double face4(const double* st, int i, int j, int k, int v, int64_t q, int d) const {
         const Grid& g = G();
         int di = d == 0, dj = d == 1, dk = d == 2;
-        double si = st[g.cc(i - di, j - dj, k - dk, v, q, NSTATE)] + st[g.cc(i, j, k, v, q, NSTATE)];
-        double so = st[g.cc(i - 2 * di, j - 2 * dj, k - 2 * dk, v, q, NSTATE)]
-                    + st[g.cc(i + di, j + dj, k + dk, v, q, NSTATE)];
-        return 0.5625 * si + (-0.0625) * so;
+        return face_interp_4(st[g.cc(i - 2 * di, j - 2 * dj, k - 2 * dk, v, q, NSTATE)],
+                             st[g.cc(i - di, j - dj, k - dk, v, q, NSTATE)], st[g.cc(i, j, k, v, q, NSTATE)],
+                             st[g.cc(i + di, j + dj, k + dk, v, q, NSTATE)]);
     }
     Metric face_metric(const double* st, int i, int j, int k, int64_t q, int d) const {
         double gg[6], b[3];
@@ -489,10 +505,12 @@ struct Stepper {
     double fd_der(const double* st, int i, int j, int k, int v, int64_t q, int d, double ih) const {
         const Grid& g = G();
         int di = d == 0, dj = d == 1, dk = d == 2;
-        auto U = [&](int s) { return st[g.cc(i + s * di, j + s * dj, k + s * dk, v, q, NSTATE)]; };
-        if (opt.fd_order == 2) return (1.0 / 2.0) * ih * ((U(1) - U(-1)));
-        if (opt.fd_order == 4) return (1.0 / 12.0) * ih * (+8.0 * (U(1) - U(-1)) - (U(2) - U(-2)));
-        return (1.0 / 60.0) * ih * (+45.0 * (U(1) - U(-1)) - 9.0 * (U(2) - U(-2)) + (U(3) - U(-3)));
+        double u[7];
+        for (int s = -3; s <= 3; ++s) {
+            const int w = (opt.fd_order / 2 >= (s < 0 ? -s : s)) ? s : 0;   // read only the stencil's points
+            u[s + 3] = st[g.cc(i + w * di, j + w * dj, k + w * dk, v, q, NSTATE)];
+        }
+        return fd_der_central(opt.fd_order, u, ih);
     }
 
     // compute_source_terms (grmhd.hh:190-297) + divergence (evolve.cpp:1395-1401)

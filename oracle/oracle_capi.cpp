@@ -7,6 +7,7 @@
 #include "grmhd_oracle_rhs.hh"
 #include "../include/grmhd_abi.h"
 #include <memory>
+#include <algorithm>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -103,6 +104,9 @@ struct StepperBase {
     virtual int64_t fofc_flagged() = 0;
     virtual const Forest* forest_of() const = 0;
     virtual void metric_ghosts(double* st) = 0;
+    // the substep's scratch arrays: which 0 face fluxes, 1 face values
+    // (UCT vbar / GS face EMFs), 2 edge EMFs, 3 cell E (GS); d = direction
+    virtual const std::vector<double>* scratch(int which, int d) const = 0;
 };
 
 template <typename EOS>
@@ -141,6 +145,16 @@ struct StepperT : StepperBase {
     int64_t fatal() override { return S->n_fatal; }
     int64_t fofc_flagged() override { return S->fofc_cells_flagged; }
     const Forest* forest_of() const override { return forest.get(); }
+    const std::vector<double>* scratch(int which, int d) const override {
+        if (d < 0 || d > 2) return nullptr;
+        switch (which) {
+            case 0: return &S->flux[d];
+            case 1: return &S->vbar[d];
+            case 2: return &S->emf;
+            case 3: return &S->ecen;
+            default: return nullptr;
+        }
+    }
     // The Cowling metric's ghost exchange: on a forest copies + 5-point
     // Lagrange (AmrExchange::fill_metric); on uniform instances copies from
     // the same-level neighbour or periodic image, none beyond outflow faces.
@@ -269,6 +283,10 @@ void orc_geom_sources(double alp, const double beta[3], const double g[6], const
     out[1] = ss[0]; out[2] = ss[1]; out[3] = ss[2];
 }
 void orc_recon(int kind, const double u[6], double out[2]) { recon_any(kind, u, out[0], out[1]); }
+// face_interp_4 at the face between u[1] and u[2]; the centred FD first
+// derivative of order 2/4/6 from u(-3..3)
+double orc_face4(const double u[4]) { return face_interp_4(u[0], u[1], u[2], u[3]); }
+double orc_fd_der(int order, const double u[7], double ih) { return fd_der_central(order, u, ih); }
 void orc_metric(const double g[6], double gi[6], double* sqrtg) {
     double b[3] = {0, 0, 0};
     Metric m(g, b, 1.0);
@@ -360,6 +378,15 @@ void orc_compute_aux(void* h, double* st, double* Bx, double* By, double* Bz, do
 }
 double orc_find_dt(void* h, const double* st, const double* aux, double cfl) {
     return static_cast<StepperBase*>(h)->dt(st, aux, cfl);
+}
+// Copy of a scratch array of the last substep (layouts of the product's
+// context scratch: fluxes [q][7][face], face values [q][4][face], edge EMFs
+// [q][3][edge], cell E [q][3][cell]); returns its length, copies when out != 0.
+int64_t orc_stepper_scratch(void* h, int which, int d, double* out) {
+    const std::vector<double>* v = static_cast<StepperBase*>(h)->scratch(which, d);
+    if (!v) return -1;
+    if (out) std::copy(v->begin(), v->end(), out);
+    return (int64_t)v->size();
 }
 int64_t orc_fatal_count(void* h) { return static_cast<StepperBase*>(h)->fatal(); }
 int64_t orc_fofc_flagged(void* h) { return static_cast<StepperBase*>(h)->fofc_flagged(); }

@@ -54,6 +54,12 @@ def olib(fast=False):
     L.orc_fatal_count.restype = i64
     L.orc_fofc_flagged.argtypes = [P]
     L.orc_fofc_flagged.restype = i64
+    L.orc_face4.argtypes = [P]
+    L.orc_face4.restype = d
+    L.orc_fd_der.argtypes = [i32, P, d]
+    L.orc_fd_der.restype = d
+    L.orc_stepper_scratch.argtypes = [P, i32, i32, P]
+    L.orc_stepper_scratch.restype = i64
     _libs[name] = L
     return L
 
@@ -131,3 +137,12 @@ class Stepper:
 
     def fofc_flagged(self):
         return self.L.orc_fofc_flagged(self.h)
+
+    def scratch(self, which, d=0):
+        """Copy of the last substep's scratch: which 0 fluxes [q][7][face d],
+        1 face values [q][4][face d], 2 edge EMFs [q][3][edge], 3 cell E."""
+        n = self.L.orc_stepper_scratch(self.h, which, d, None)
+        assert n >= 0
+        out = np.zeros(n)
+        self.L.orc_stepper_scratch(self.h, which, d, out.ctypes.data)
+        return out
