@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("c2p_tol", C.c_double), ("max_lorentz", C.c_double), ("max_sigma", C.c_double),
         ("beta_fallback", C.c_double), ("use_entropy_backup", C.c_int32), ("bh_alp_thresh", C.c_double),
         ("excise_by_radius", C.c_int32), ("excision_radius", C.c_double), ("excision_lapse", C.c_double),
+        ("excision_cks", C.c_int32), ("excision_spin", C.c_double),
         ("recon", C.c_int32), ("riemann", C.c_int32), ("emf", C.c_int32), ("thermo_recon", C.c_int32),
         ("fd_order", C.c_int32), ("rusanov_c_limit", C.c_double), ("flat_metric", C.c_int32),
         ("fofc", C.c_int32), ("fofc_dmp_enable", C.c_int32), ("fofc_dmp_M", C.c_double),
@@ -58,21 +59,23 @@ class Grid(C.Structure):
 
 
 class Instance(C.Structure):
-    _fields_ = [("nb", C.c_int32 * 3), ("q0", C.c_int64), ("bc", C.c_int32 * 6), ("idx", C.c_double * 3)]
+    _fields_ = [("nb", C.c_int32 * 3), ("q0", C.c_int64), ("bc", C.c_int32 * 6), ("idx", C.c_double * 3),
+                ("x0", C.c_double * 3)]
 
 
 class Forest(C.Structure):
     """grmhd_forest: two-level block forest (see include/grmhd_abi.h)."""
     _fields_ = [("nb", C.c_int32 * 3), ("bc", C.c_int32 * 6), ("idx", C.c_double * 3), ("n_refined", C.c_int32),
-                ("refined", C.POINTER(C.c_int32))]
+                ("refined", C.POINTER(C.c_int32)), ("x0", C.c_double * 3)]
 
 
-def make_forest(nb, bc, idx, refined):
-    """A Forest struct plus the int32 array it points to (keep both alive)."""
+def make_forest(nb, bc, idx, refined, x0=(0.0, 0.0, 0.0)):
+    """A Forest struct plus the int32 array it points to (keep both alive);
+    x0: lower corner of the domain (cell radii for radius excision)."""
     ref = np.ascontiguousarray(sorted(set(int(r) for r in refined)), dtype=np.int32)
     f = Forest()
     for d in range(3):
-        f.nb[d] = int(nb[d]); f.idx[d] = float(idx[d])
+        f.nb[d] = int(nb[d]); f.idx[d] = float(idx[d]); f.x0[d] = float(x0[d])
     for d in range(6):
         f.bc[d] = int(bc[d])
     f.n_refined = len(ref)
@@ -245,12 +248,14 @@ def ptr3(a, b, c):
 
 
 def make_instances(specs):
-    """specs: list of dict(nb=(bx,by,bz), q0=int, bc=(6 ints), idx=(3 floats))."""
+    """specs: list of dict(nb=(bx,by,bz), q0=int, bc=(6 ints), idx=(3 floats)
+    [, x0=(3 floats): lower corner of the interior, default the origin])."""
     arr = (Instance * max(1, len(specs)))()
     for i, s in enumerate(specs):
         for d in range(3):
             arr[i].nb[d] = int(s["nb"][d])
             arr[i].idx[d] = float(s["idx"][d])
+            arr[i].x0[d] = float(s.get("x0", (0.0, 0.0, 0.0))[d])
         arr[i].q0 = int(s["q0"])
         for d in range(6):
             arr[i].bc[d] = int(s["bc"][d])

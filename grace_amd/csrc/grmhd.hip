@@ -62,7 +62,18 @@ struct DevGrid {
     int64_t nedge;     // edges per block per component
     int64_t nq;
 };
-struct DevInst { int nb[3]; int64_t q0; int bc[6]; double idx[3]; };
+struct DevInst { int nb[3]; int64_t q0; int bc[6]; double idx[3]; double x0[3]; };
+
+// Radius of cell (i, j, k) (ghosts included) of block b for radius-based
+// excision: centre x0 + (global index + 1/2) / idx, as the reference's
+// get_physical_coordinates (cartesian_coordinate_systems.hh:56-66).
+GHD double cell_radius(const ExcPar& ex, const DevInst& I, const int bb[3], const int n[3], int ng, int i, int j,
+                       int k) {
+    const int loc[3] = {i, j, k};
+    double x[3];
+    for (int d = 0; d < 3; ++d) x[d] = I.x0[d] + ((double)(bb[d] * n[d] + loc[d] - ng) + 0.5) / I.idx[d];
+    return excision_radius_of(ex, x[0], x[1], x[2]);
+}
 struct DevBlk { int inst; int b[3]; };
 
 struct Opts {
@@ -798,7 +809,8 @@ __global__ __launch_bounds__(256) void k_fofc_flag(DevGrid G, EOS eos, Opts o, F
     C2PCount cnt;
     C2PGuess gs{0, 0, 0, 0, 0, 0};
     C2PRoots rt{-1, -1, -1, -1};
-    const C2POut out = c2p(cons, pr, m, eos, o.atmo, o.ex, cp, 1e300, err, true, cnt, gs, rt);
+    const double rad = o.ex.by_radius ? cell_radius(o.ex, I, a.blk[q].b, G.n, G.ng, i, j, k) : 1e300;
+    const C2POut out = c2p(cons, pr, m, eos, o.atmo, o.ex, cp, rad, err, true, cnt, gs, rt);
     if (!(out.floored || dmp)) return;
     for (int t = 0; t < 2; ++t) {
         a.ftag[0][q * nf0 + fidx<0>(G, i + t, j, k)] = 1;
@@ -1103,7 +1115,8 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
     if (active) {
         A[BX_ * nc] = Bc0 / m.sqrtg; A[BY_ * nc] = Bc1 / m.sqrtg; A[BZ_ * nc] = Bc2 / m.sqrtg;
         C2PRoots rt{-1, -1, -1, -1};
-        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, 1e300, err, false, cnt, gs, rt);
+        const double rad = o.ex.by_radius ? cell_radius(o.ex, inst[blk[q].inst], blk[q].b, G.n, G.ng, i, j, k) : 1e300;
+        out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, rad, err, false, cnt, gs, rt);
         A[RHO_ * nc] = pr[RHOL]; A[EPS_ * nc] = pr[EPSL]; A[PRESS_ * nc] = pr[PRESSL];
         A[TEMP_ * nc] = pr[TEMPL]; A[ENTROPY_ * nc] = pr[ENTL]; A[YE_ * nc] = pr[YEL];
         A[ZVECX_ * nc] = pr[ZXL]; A[ZVECY_ * nc] = pr[ZYL]; A[ZVECZ_ * nc] = pr[ZZL];
@@ -1461,6 +1474,8 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     if (p->fofc && !(p->fofc_dmp_M > 0)) { set_err("fofc_dmp_M must be > 0"); return GRMHD_ERR_INVALID; }
     if (p->fofc_ghost_emf != 0 && p->fofc_ghost_emf != 1) { set_err("fofc_ghost_emf must be 0 or 1"); return GRMHD_ERR_INVALID; }
     if (p->stage_streams != 0 && p->stage_streams != 1) { set_err("stage_streams must be 0 or 1"); return GRMHD_ERR_INVALID; }
+    if (p->excise_by_radius != 0 && p->excise_by_radius != 1) { set_err("excise_by_radius must be 0 or 1"); return GRMHD_ERR_INVALID; }
+    if (p->excision_cks != 0 && p->excision_cks != 1) { set_err("excision_cks must be 0 or 1"); return GRMHD_ERR_INVALID; }
     // instance / block tables
     std::vector<DevBlk> blk(g->nq, DevBlk{-1, {0, 0, 0}});
     for (int a = 0; a < n_inst; ++a) {
@@ -1471,6 +1486,21 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         }
         for (int d = 0; d < 6; ++d) if (I.bc[d] != 0 && I.bc[d] != 1) { set_err("bc kind"); return GRMHD_ERR_INVALID; }
         for (int d = 0; d < 3; ++d) if (!(I.idx[d] > 0)) { set_err("idx must be > 0"); return GRMHD_ERR_INVALID; }
+        // the source terms' metric derivatives take idx[0] in every direction
+        // (grmhd.hh:231-235, written for the reference's cubic cells)
+        if (!p->flat_metric && (I.idx[1] != I.idx[0] || I.idx[2] != I.idx[0])) {
+            set_err("instance %d: a curved metric needs equal spacing in x, y and z (idx %g %g %g)", a, I.idx[0],
+                    I.idx[1], I.idx[2]);
+            return GRMHD_ERR_UNSUPPORTED;
+        }
+        // radius excision at a ghost cell uses its unwrapped position in the
+        // reference; this library copies a periodic image's c2p outcome
+        if (p->excise_by_radius)
+            for (int d = 0; d < 6; ++d)
+                if (I.bc[d] == GRMHD_BC_PERIODIC) {
+                    set_err("excise_by_radius is not supported with periodic faces (instance %d)", a);
+                    return GRMHD_ERR_UNSUPPORTED;
+                }
         for (int bk = 0; bk < I.nb[2]; ++bk)
             for (int bj = 0; bj < I.nb[1]; ++bj)
                 for (int bi = 0; bi < I.nb[0]; ++bi) {
@@ -1491,7 +1521,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     Opts& O = c->O;
     O.atmo = Atmo{p->rho_fl, p->temp_fl, p->ye_fl, p->atmo_tol};
     O.cp = C2PPar{p->c2p_tol, p->max_lorentz, p->max_sigma, p->beta_fallback, p->bh_alp_thresh, p->use_entropy_backup};
-    O.ex = ExcPar{p->excision_radius, p->excision_lapse, p->excise_by_radius};
+    O.ex = ExcPar{p->excision_radius, p->excision_lapse, p->excise_by_radius, p->excision_cks, p->excision_spin};
     O.riemann = p->riemann; O.emf = p->emf; O.thermo = p->thermo_recon; O.fd_order = p->fd_order;
     O.c_limit = p->rusanov_c_limit;
     O.fofc = p->fofc; O.dmp = p->fofc_dmp_enable; O.dmp_M = p->fofc_dmp_M;
@@ -1499,7 +1529,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     c->blk_h = blk;
     for (int a = 0; a < n_inst; ++a) {
         DevInst d;
-        for (int x = 0; x < 3; ++x) { d.nb[x] = inst[a].nb[x]; d.idx[x] = inst[a].idx[x]; }
+        for (int x = 0; x < 3; ++x) { d.nb[x] = inst[a].nb[x]; d.idx[x] = inst[a].idx[x]; d.x0[x] = inst[a].x0[x]; }
         for (int x = 0; x < 6; ++x) d.bc[x] = inst[a].bc[x];
         d.q0 = inst[a].q0;
         c->inst_h.push_back(d);
@@ -1712,7 +1742,11 @@ extern "C" grmhd_status grmhd_ctx_create_forest(const grmhd_params* p, const grm
     for (int32_t q = 0; q < nq; ++q) {
         grmhd_instance& I = inst[q];
         const double s = fin.fblk[q].x ? 2.0 : 1.0;
-        for (int d = 0; d < 3; ++d) { I.nb[d] = 1; I.idx[d] = f->idx[d] * s; }
+        const int crd[3] = {fin.fblk[q].y, fin.fblk[q].z, fin.fblk[q].w};
+        for (int d = 0; d < 3; ++d) {
+            I.nb[d] = 1; I.idx[d] = f->idx[d] * s;
+            I.x0[d] = f->x0[d] + (double)crd[d] * (d == 0 ? g->nx : (d == 1 ? g->ny : g->nz)) / I.idx[d];
+        }
         for (int d = 0; d < 6; ++d) I.bc[d] = f->bc[d];
         I.q0 = q;
     }

@@ -274,7 +274,18 @@ struct EosHybrid {
 
 struct Atmo { double rho_fl, temp_fl, ye_fl, atmo_tol; };
 struct C2PPar { double tol, max_w, max_sigma, beta_fallback, alp_bh_thresh; int use_ent_backup; };
-struct ExcPar { double r_ex, alp_ex; int by_radius; };
+struct ExcPar { double r_ex, alp_ex; int by_radius, cks; double spin; };
+
+// Radius of a point for radius-based excision (cart_to_sph,
+// cartesian_coordinate_systems.hh:87-107): |x|, or in Cartesian Kerr-Schild
+// coordinates of a hole of spin a the spheroidal radius
+// r^2 = (R^2 - a^2 + sqrt((R^2 - a^2)^2 + 4 a^2 z^2)) / 2, floored at 1e-6.
+GHD double excision_radius_of(const ExcPar& ex, double x, double y, double z) {
+    const double R2 = x * x + y * y + z * z;
+    if (!ex.cks) return sqrt(R2);
+    const double a2 = ex.spin * ex.spin, d = R2 - a2;
+    return fmax(sqrt(d + sqrt(d * d + 4.0 * a2 * z * z)) / sqrt(2.0), 1.0e-6);
+}
 
 // ---- GRMHD algebra ----------------------------------------------------------
 GHD void lower3(const double g[6], double x, double y, double z, double o[3]) {

@@ -65,7 +65,7 @@ def forest_wave(nb=(4, 4, 4), n=8, ngz=4, refined=None, bc=BC_PERIODIC, extent=1
     bcs = (bc,) * 6 if isinstance(bc, int) else tuple(bc)
     dx = extent / (nb[0] * n)
     idx = (1.0 / dx,) * 3
-    forest = make_forest(nb, bcs, idx, refined)
+    forest = make_forest(nb, bcs, idx, refined, x0=[-0.5 * nb[d] * n * dx for d in range(3)])
     nq = nb[0] * nb[1] * nb[2] + 7 * forest.n_refined
     grid = Grid(n, n, n, ngz, nq)
     state, B, aux = alloc(grid)
@@ -90,7 +90,7 @@ def blast(nb=(8, 8, 8), n=16, ngz=3, seed=SEED, gamma=4.0 / 3.0, noise=0.01, emf
     nq = nb[0] * nb[1] * nb[2]
     grid = Grid(n, n, n, ngz, nq)
     N = [nb[d] * n for d in range(3)]
-    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(float(N[d]) for d in range(3)))]
+    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(float(N[d]) for d in range(3)), x0=(-0.5,) * 3)]
     inst = make_instances(spec)
     state, B, aux = alloc(grid)
     lo = np.array([-0.5, -0.5, -0.5])
@@ -113,10 +113,10 @@ def blast_ks(nb=(2, 2, 2), n=16, ngz=3, spin=0.9, center=(8.0, 0.0, 0.0), size=4
     nq = nb[0] * nb[1] * nb[2]
     grid = Grid(n, n, n, ngz, nq)
     N = [nb[d] * n for d in range(3)]
-    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(N[d] / size for d in range(3)))]
+    lo = np.array(center, dtype=np.float64) - 0.5 * size
+    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(N[d] / size for d in range(3)), x0=tuple(lo))]
     inst = make_instances(spec)
     state, B, aux = alloc(grid)
-    lo = np.array(center, dtype=np.float64) - 0.5 * size
     hi = lo + size
     Bv = np.array([Bx, 0.0, 0.0])
     rc = lib().grmhd_setup_blast_ks(grid, inst, 1, gamma, _p(lo), _p(hi), spin, rho, p, p_in, r_in, _p(Bv), noise,
@@ -139,7 +139,8 @@ def fm_torus(nb=(4, 4, 4), n=16, ngz=3, extent=48.0, spin=0.9375, r_in=6.0, r_ma
     nq = nb[0] * nb[1] * nb[2]
     grid = Grid(n, n, n, ngz, nq)
     N = [nb[d] * n for d in range(3)]
-    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(N[d] / (2.0 * extent) for d in range(3)))]
+    spec = [dict(nb=nb, q0=0, bc=(BC_OUTFLOW,) * 6, idx=tuple(N[d] / (2.0 * extent) for d in range(3)),
+                 x0=(-extent,) * 3)]
     inst = make_instances(spec)
     state, B, aux = alloc(grid)
     lo = np.array([-extent] * 3, dtype=np.float64)
@@ -171,7 +172,7 @@ def fm_torus_forest(nb=(6, 6, 6), n=16, ngz=4, extent=48.0, refined=None, spin=0
         refined = [(lo_b[0] + a) + nb[0] * ((lo_b[1] + b) + nb[1] * (lo_b[2] + e))
                    for e in range(2) for b in range(4) for a in range(4)]
     dx = 2.0 * extent / (nb[0] * n)
-    forest = make_forest(nb, (BC_OUTFLOW,) * 6, (1.0 / dx,) * 3, refined)
+    forest = make_forest(nb, (BC_OUTFLOW,) * 6, (1.0 / dx,) * 3, refined, x0=(-0.5 * nb[0] * n * dx,) * 3)
     nq = nb[0] * nb[1] * nb[2] + 7 * forest.n_refined
     grid = Grid(n, n, n, ngz, nq)
     state, B, aux = alloc(grid)
@@ -206,7 +207,7 @@ def multi_blast(n_inst=64, sizes=(32, 48, 64, 80, 96), n=16, ngz=3, seed=SEED, g
     specs, q0 = [], 0
     for s in sides:
         b = int(s) // n
-        specs.append(dict(nb=(b, b, b), q0=q0, bc=(BC_OUTFLOW,) * 6, idx=(float(s),) * 3))
+        specs.append(dict(nb=(b, b, b), q0=q0, bc=(BC_OUTFLOW,) * 6, idx=(float(s),) * 3, x0=(-0.5,) * 3))
         q0 += b ** 3
     grid = Grid(n, n, n, ngz, q0)
     inst = make_instances(specs)

@@ -73,9 +73,18 @@ typedef struct {
     double c2p_tol, max_lorentz, max_sigma, beta_fallback;
     int32_t use_entropy_backup;
     double bh_alp_thresh;
-    /* excision (grmhd_helpers.hh:89-96) */
+    /* excision (grmhd_helpers.hh:89-96).  excise_by_radius = 1 resets cells
+     * whose radius is <= excision_radius (c2p.cpp:345-348); the radius of a
+     * cell comes from its centre x = x0 + (global index + 1/2) / idx of its
+     * instance (or forest): |x|, or with excision_cks = 1 the Kerr-Schild
+     * radius of a hole of spin excision_spin at the origin
+     * (cart_to_sph, cartesian_coordinate_systems.hh:87-107).  Not accepted
+     * with periodic faces, where the reference evaluates a ghost cell at its
+     * unwrapped position while this library copies the periodic image. */
     int32_t excise_by_radius;
     double excision_radius, excision_lapse;
+    int32_t excision_cks;
+    double excision_spin;
     /* numerical scheme (CMakeLists.txt:206-256, cmake/code_modules.cmake:14-40) */
     int32_t recon, riemann, emf, thermo_recon, fd_order;
     double rusanov_c_limit;
@@ -112,8 +121,13 @@ typedef struct { int32_t nx, ny, nz, ngz; int64_t nq; } grmhd_grid;
 
 /* A uniform Cartesian instance cut into nb[0]*nb[1]*nb[2] blocks, stored at
  * block indices q = q0 + bi + nb0*(bj + nb1*bk).  bc[] per face
- * (-x,+x,-y,+y,-z,+z).  idx = inverse cell spacings. */
-typedef struct { int32_t nb[3]; int64_t q0; int32_t bc[6]; double idx[3]; } grmhd_instance;
+ * (-x,+x,-y,+y,-z,+z).  idx = inverse cell spacings.  x0 = coordinates of
+ * the lower corner of the instance's interior (the reference's per-block
+ * origin qx, cartesian_coordinate_systems.hh:56-66), used only for cell
+ * radii (excise_by_radius).  With a curved metric the spacing must be equal
+ * in x, y and z: the metric derivatives of the source terms use idx[0] in
+ * every direction, as the reference (grmhd.hh:231-235) on its cubic cells. */
+typedef struct { int32_t nb[3]; int64_t q0; int32_t bc[6]; double idx[3]; double x0[3]; } grmhd_instance;
 
 /* Per-launch c2p statistics (reduced on device, read with grmhd_c2p_stats). */
 typedef struct {
@@ -156,6 +170,7 @@ typedef struct {
     double idx[3];            /* inverse coarse cell spacings */
     int32_t n_refined;
     const int32_t* refined;   /* host array of coarse ids */
+    double x0[3];             /* lower corner of the domain (cell radii only) */
 } grmhd_forest;
 
 grmhd_status grmhd_ctx_create_forest(const grmhd_params* p, const grmhd_grid* grid, const grmhd_forest* forest,

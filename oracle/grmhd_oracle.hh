@@ -392,7 +392,22 @@ struct C2PParams {
     bool use_ent_backup = true;
     double alp_bh_thresh = 0.2;
 };
-struct ExcisionParams { double r_ex = 1.0, alp_ex = 0.05; bool excise_by_radius = false; };
+struct ExcisionParams {
+    double r_ex = 1.0, alp_ex = 0.05;
+    bool excise_by_radius = false;
+    bool cks = false;        // Cartesian Kerr-Schild coordinates: spheroidal radius
+    double spin = 0.0;
+};
+
+// rtp[0] of cart_to_sph (include/grace/coordinates/cartesian_coordinate_systems.hh:87-107)
+inline double sph_radius(const ExcisionParams& e, const double xyz[3]) {
+    const double rad = std::sqrt(xyz[0] * xyz[0] + xyz[1] * xyz[1] + xyz[2] * xyz[2]);
+    if (!e.cks) return rad;
+    const double a = e.spin;
+    return std::fmax(std::sqrt(rad * rad - a * a + std::sqrt((rad * rad - a * a) * (rad * rad - a * a)
+                                                             + 4.0 * a * a * xyz[2] * xyz[2])) / std::sqrt(2.0),
+                     1.0e-6);
+}
 
 // ---------------------------------------------------------------------------
 // GRMHD algebra (Valencia variables, z^i = W v^i).  The reference generates

@@ -77,6 +77,7 @@ struct Instance {
     int64_t q0;
     int bc[6];       // -x,+x,-y,+y,-z,+z
     double idx[3];   // inverse spacings
+    double x0[3] = {0.0, 0.0, 0.0};   // lower corner of the interior
 };
 
 struct BlockDesc { int inst; int b[3]; };
@@ -201,6 +202,18 @@ struct Stepper {
     }
 
     const Grid& G() const { return L.grid; }
+
+    // spherical radius of the centre of cell (i,j,k) of block q, ghosts
+    // included (get_physical_coordinates_sph, cartesian_coordinate_systems.hh:56-85)
+    double cell_rtp0(int i, int j, int k, int64_t q) const {
+        const Grid& g = G();
+        const BlockDesc& b = L.blk[q];
+        const Instance& I = L.inst[b.inst];
+        const int loc[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+        double xyz[3];
+        for (int d = 0; d < 3; ++d) xyz[d] = I.x0[d] + ((double)(b.b[d] * n[d] + loc[d] - g.ngz) + 0.5) / I.idx[d];
+        return sph_radius(exc, xyz);
+    }
 
     Metric metric_at(const double* st, int i, int j, int k, int64_t q) const {
         const Grid& g = G();
@@ -639,7 +652,8 @@ struct Stepper {
                         Metric m = metric_at(old.state, i, j, k, q);
                         double prims[NPRIM] = {0};
                         Bits err;
-                        C2PResult r = conservs_to_prims(cons, prims, m, eos, atmo, exc, lenient, nullptr, err, true);
+                        double rtp[3] = {cell_rtp0(i, j, k, q), 0.0, 0.0};
+                        C2PResult r = conservs_to_prims(cons, prims, m, eos, atmo, exc, lenient, rtp, err, true);
                         if (!(r.floored || dmp)) continue;
                         nflag++;
                         for (int t = 0; t < 2; ++t) {
@@ -811,7 +825,8 @@ struct Stepper {
                         A(BZ_) = cons[BSZL] / m.sqrtg;
                         Bits err;
                         double prims[NPRIM] = {0};
-                        C2PResult r = conservs_to_prims(cons, prims, m, eos, atmo, exc, c2pp, nullptr, err);
+                        double rtp[3] = {cell_rtp0(i, j, k, q), 0.0, 0.0};
+                        C2PResult r = conservs_to_prims(cons, prims, m, eos, atmo, exc, c2pp, rtp, err);
                         if (r.fatal) fatal++;
                         A(RHO_) = prims[RHOL]; A(EPS_) = prims[EPSL]; A(PRESS_) = prims[PRESSL];
                         A(TEMP_) = prims[TEMPL]; A(ENTROPY_) = prims[ENTL]; A(YE_) = prims[YEL];

@@ -45,6 +45,7 @@ C2PParams make_c2p(const grmhd_params* p) {
 }
 ExcisionParams make_exc(const grmhd_params* p) {
     ExcisionParams e; e.r_ex = p->excision_radius; e.alp_ex = p->excision_lapse; e.excise_by_radius = p->excise_by_radius != 0;
+    e.cks = p->excision_cks != 0; e.spin = p->excision_spin;
     return e;
 }
 Options make_opt(const grmhd_params* p) {
@@ -121,7 +122,7 @@ struct StepperT : StepperBase {
         L.blk.assign(g->nq, BlockDesc{-1, {0, 0, 0}});
         for (int a = 0; a < n_inst; ++a) {
             Instance I;
-            for (int d = 0; d < 3; ++d) { I.nb[d] = inst[a].nb[d]; I.idx[d] = inst[a].idx[d]; }
+            for (int d = 0; d < 3; ++d) { I.nb[d] = inst[a].nb[d]; I.idx[d] = inst[a].idx[d]; I.x0[d] = inst[a].x0[d]; }
             for (int d = 0; d < 6; ++d) I.bc[d] = inst[a].bc[d];
             I.q0 = inst[a].q0;
             L.inst.push_back(I);
@@ -202,7 +203,10 @@ StepperBase* make_forest_stepper(const grmhd_params* p, const grmhd_grid* g, con
     if (F->nq != g->nq) return nullptr;
     std::vector<grmhd_instance> inst(F->nq);
     for (int64_t q = 0; q < F->nq; ++q) {
-        for (int d = 0; d < 3; ++d) { inst[q].nb[d] = 1; inst[q].idx[d] = f->idx[d] * (F->level[q] ? 2.0 : 1.0); }
+        for (int d = 0; d < 3; ++d) {
+            inst[q].nb[d] = 1; inst[q].idx[d] = f->idx[d] * (F->level[q] ? 2.0 : 1.0);
+            inst[q].x0[d] = f->x0[d] + (double)F->crd[q][d] * n[d] / inst[q].idx[d];
+        }
         for (int d = 0; d < 6; ++d) inst[q].bc[d] = f->bc[d];
         inst[q].q0 = q;
     }

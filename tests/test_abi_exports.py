@@ -43,13 +43,43 @@ def test_io_library_exports_declared_symbols():
     assert not [n for n in sorted(names) if not hasattr(L, n)]
 
 
-def test_struct_layouts():
-    # grmhd_params ends with int32 flat_metric after a double; total size is a multiple of 8
-    assert C.sizeof(ga.Params) % 8 == 0
-    assert C.sizeof(ga.Grid) == 24
-    assert C.sizeof(ga.Instance) == 72  # 3 int32 + pad, int64, 6 int32, 3 double
+def _c_layouts(tmp_path):
+    """sizeof and field offsets of the header's structs, from the C compiler."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        import pytest
+        pytest.skip("no C compiler")
+    structs = {"grmhd_params": ga.Params, "grmhd_grid": ga.Grid, "grmhd_instance": ga.Instance,
+               "grmhd_forest": ga.Forest, "grmhd_c2p_stats_t": ga.C2PStats, "grmhd_tuning": ga.Tuning}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "grmhd_abi.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{name} {f[0]} %zu\\n", offsetof({name}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for line in out.splitlines():
+        name, field, val = line.split()
+        got[(name, field)] = int(val)
+    return structs, got
+
+
+def test_struct_layouts(tmp_path):
+    structs, got = _c_layouts(tmp_path)
+    for name, cls in structs.items():
+        assert C.sizeof(cls) == got[(name, "size")], name
+        for f in cls._fields_:
+            assert getattr(cls, f[0]).offset == got[(name, f[0])], (name, f[0])
     p = ga.default_params()
     assert p.rho_fl == 1e-14 and p.c2p_tol == 1e-10 and p.max_sigma == 100 and p.fd_order == 6
+    assert p.excise_by_radius == 0 and p.excision_cks == 0
 
 
 def test_setup_functions_run_on_host():

@@ -17,7 +17,7 @@ import grace_amd as ga
 from grace_amd import workloads as W
 import _oracle as O
 from test_oracle_amr import volume_integrals, interface_mismatch, div_b, ghost_mask
-from test_gpu_parity import to_dev, to_host, interior, face_interior, linf_rel
+from test_gpu_parity import check_cons, to_dev, to_host, interior, face_interior, linf_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -89,9 +89,7 @@ def test_gpu_forest_substep(dev, emf):
     ctx.advance_substep(dtd, 1.0, ng_, sg, nBg, Bg, ag)
     torch.cuda.synchronize()
     gn = to_host(ng_)
-    for v in range(7):
-        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
-        assert e < 1e-12, (v, e)
+    check_cons(wl, gn, new_o, aux, 1e-12)
     for d in range(3):
         e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
         assert e < 1e-12, (d, e)
@@ -122,9 +120,7 @@ def test_gpu_forest_rk3(dev, name, emf):
         ctx.rk3_step(dtd, sg, Bg, ag)
     torch.cuda.synchronize()
     gs, gB, gaux = to_host(sg), [to_host(b) for b in Bg], to_host(ag)
-    for v in range(7):
-        e = linf_rel(interior(wl.grid, gs[:, v]), interior(wl.grid, st[:, v]))
-        assert e < 1e-12, (v, e)
+    check_cons(wl, gs, st, aux, 1e-12)
     for d in range(3):
         e = linf_rel(face_interior(wl.grid, gB[d], d), face_interior(wl.grid, B[d], d))
         assert e < 1e-12, (d, e)
@@ -253,9 +249,7 @@ def test_gpu_torus_forest_rk3(dev):
     gs, gB, gaux = res[0]
     for arr in [gs, gaux] + gB:
         assert np.isfinite(arr).all()
-    for v in range(7):
-        e = linf_rel(interior(wl.grid, gs[:, v]), interior(wl.grid, st[:, v]))
-        assert e < 1e-12, (v, e)
+    check_cons(wl, gs, st, aux, 1e-12)
     for d in range(3):
         e = linf_rel(face_interior(wl.grid, gB[d], d), face_interior(wl.grid, B[d], d))
         assert e < 1e-12, (d, e)
@@ -328,9 +322,7 @@ def test_gpu_forest_fofc_substep(dev, ghost_emf, case):
     ctx.advance_substep(torch.tensor([dt], dtype=torch.float64, device=dev), 1.0, ng_, sg, nBg, Bg, ag)
     torch.cuda.synchronize()
     gn = to_host(ng_)
-    for v in range(7):
-        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
-        assert e < 1e-12, (v, e)
+    check_cons(wl, gn, new_o, aux, 1e-12)
     for d in range(3):
         e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
         assert e < 1e-12, (d, e)

@@ -9,7 +9,7 @@ import pytest
 import grace_amd as ga
 from grace_amd import workloads as W
 import _oracle as O
-from test_gpu_parity import dev_arrays, to_host, interior, face_interior, linf_rel, say, run_rk3_pair, check_close
+from test_gpu_parity import check_cons, dev_arrays, to_host, interior, face_interior, linf_rel, say, run_rk3_pair, check_close
 
 pytestmark = pytest.mark.gpu
 
@@ -36,9 +36,7 @@ def test_gpu_fofc_substep(dev):
     ctx.advance_substep(dtd, 1.0, ng, sg, nBg, Bg, ag)
     torch.cuda.synchronize()
     gn = to_host(ng)
-    for v in range(7):
-        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
-        assert e < 1e-12, (v, e)
+    check_cons(wl, gn, new_o, aux, 1e-12)
     for d in range(3):
         e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
         assert e < 1e-12, (d, e)
@@ -83,9 +81,7 @@ def test_gpu_fofc_torus_curved(dev):
     ctx.advance_substep(torch.tensor([dt], dtype=torch.float64, device=dev), 1.0, ng, sg, nBg, Bg, ag)
     torch.cuda.synchronize()
     gn = to_host(ng)
-    for v in range(7):
-        e = linf_rel(interior(wl.grid, gn[:, v]), interior(wl.grid, new_o[:, v]))
-        assert e < 1e-12, (v, e)
+    check_cons(wl, gn, new_o, aux, 1e-12)
     for d in range(3):
         e = linf_rel(face_interior(wl.grid, to_host(nBg[d]), d), face_interior(wl.grid, nB_o[d], d))
         assert e < 1e-12, (d, e)

@@ -650,3 +650,63 @@ def test_torus_stationary_residual_converges():
     for k in ("D", "Sx", "tau"):
         rate = math.log2(r1[k] / r2[k])
         assert rate > 1.5, (k, r1, r2, rate)
+
+
+# ---------------------------------------------------------------------------
+# radius-based excision (c2p.cpp:345-348 with rtp from cart_to_sph)
+# ---------------------------------------------------------------------------
+def ks_radius(x, y, z, a):
+    """Kerr-Schild radius: the root r > 0 of (x^2 + y^2)/(r^2 + a^2) + z^2/r^2 = 1."""
+    R2 = x * x + y * y + z * z
+    r2 = 0.5 * (R2 - a * a + np.sqrt((R2 - a * a) ** 2 + 4 * a * a * z * z))
+    return np.sqrt(r2)
+
+
+def test_ks_radius_definition():
+    rng = np.random.default_rng(9)
+    x, y, z = rng.uniform(-5, 5, size=(3, 1000))
+    a = 0.9375
+    r = ks_radius(x, y, z, a)
+    assert np.allclose((x * x + y * y) / (r * r + a * a) + z * z / (r * r), 1.0, rtol=0, atol=1e-12)
+    assert np.allclose(ks_radius(3.0, 0.0, 0.0, a), math.sqrt(9 - a * a))
+    assert np.allclose(ks_radius(0.0, 0.0, 2.5, a), 2.5)
+
+
+@pytest.mark.parametrize("cks", [0, 1])
+def test_radius_excision_resets_exactly_inside(cks):
+    """excise_by_radius: after the c2p pass every cell whose radius (Kerr-Schild
+    with cks, else |x|) is <= r_ex carries the atmosphere reset, and the
+    lapse criterion no longer applies (a cell outside r_ex with lapse below
+    excision_lapse is not excised)."""
+    a, r_ex = 0.9375, 3.0
+    wl = W.fm_torus(nb=(2, 2, 2), n=8, extent=6.0, beta=0.0, noise=0.0)
+    wl.params.excise_by_radius = 1
+    wl.params.excision_radius = r_ex
+    wl.params.excision_cks = cks
+    wl.params.excision_spin = a
+    wl.params.excision_lapse = 0.9        # would excise much more if it applied
+    S = O.Stepper(wl)
+    st, B, aux = wl.copy_arrays()
+    S.aux(st, B, aux)
+    g = wl.grid
+    N = g.nx + 2 * g.ngz
+    dx = wl.extra["dx"]
+    cc = (np.arange(N) - g.ngz + 0.5) * dx
+    atmo_bit = 1 << ga.C2P_BITS.index("ATMO_RESET")
+    inside_all, flagged_outside = 0, 0
+    for q in range(g.nq):
+        b = (q % 2, (q // 2) % 2, q // 4)
+        zc, yc, xc = np.meshgrid(-6.0 + b[2] * 8 * dx + cc, -6.0 + b[1] * 8 * dx + cc, -6.0 + b[0] * 8 * dx + cc,
+                                 indexing="ij")
+        r = ks_radius(xc, yc, zc, a) if cks else np.sqrt(xc ** 2 + yc ** 2 + zc ** 2)
+        bits = aux[q, ga.C2P_ERR].astype(np.uint64)
+        reset = (bits & np.uint64(atmo_bit)) != 0
+        inside = r <= r_ex
+        assert reset[inside].all()
+        inside_all += int(inside.sum())
+        # outside r_ex the torus body (rho well above the floor) is never reset
+        body = ~inside & (aux[q, ga.RHO] > 1e3 * wl.params.rho_fl)
+        flagged_outside += int((reset & body).sum())
+        alp = st[q, ga.ALP]
+        assert (alp[~inside] < 0.9).any()  # the lapse criterion would have applied here
+    assert inside_all > 100 and flagged_outside == 0
