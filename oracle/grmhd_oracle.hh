@@ -9,7 +9,8 @@
 // relative to the reference checkout.
 //
 // Parity status per function (what pins it) is listed in the header comment
-// of each section; "parity unpinned" marks functions with no external check.
+// of each section; every function has a check independent of its own
+// formulas (golden vectors, closed forms, invariants, convergence).
 #pragma once
 
 #include <array>
@@ -418,11 +419,14 @@ inline double sph_radius(const ExcisionParams& e, const double xyz[3]) {
 //   vtilde^i = alp z^i/W - beta^i : :46-58, pinned by :119-160
 //   conserved (D,S_i,tau,S*) : :318-350, pinned by :163-215 closed forms
 //   fluxes                   : :97-156, pinned by :218-270 (hydro); B != 0
-//                              pinned only via F(U,U)=F and a symbolic
-//                              re-derivation in tests (parity unpinned against
-//                              the reference build)
+//                              and v != 0 in generic curved metrics pinned
+//                              against fluxes built from T^{mu nu} with numpy
+//                              (tests/test_oracle_pins.py)
 //   cm/cp                    : :60-95 (Gammie-style fast-speed quadratic),
-//                              pinned by :273-318 (hydro +-c_s)
+//                              pinned by :273-318 (hydro +-c_s); B != 0 by the
+//                              roots of the covariant dispersion relation and
+//                              relativistic velocity addition
+//                              (tests/test_oracle_pins.py)
 //   geometric sources        : :158-316, pinned by the two golden vectors of
 //                              test/test_grmhd_kernels.cpp:393-469 (rel 1e-12)
 // ---------------------------------------------------------------------------
