@@ -10,11 +10,19 @@ all cells), replayed from a hipGraph.  value = interior zones x steps / time.
 Prints one JSON line.  Extra objects:
   roofline      dominant kernel (largest measured time per step; the three
                 direction instantiations of the face flux kernel k_flux count
-                as one kernel, averaged per launch) vs the HBM peak, with its
-                measured FP64 instruction throughput when counter summaries
-                are present under profiles/
-  cpu_baseline  the CPU oracle (OpenMP, contracted FMAs) timing one RK3 step of
-                the same 128^3 workload on the host cores
+                as one kernel, averaged per launch, HIP events on its launch
+                stream).  Its bound follows from its algorithmic intensity
+                (operation count of the oracle per face over the algorithmic
+                bytes per face) against the ridge point 78.6 TFLOP/s / 8 TB/s;
+                both the FP64 and the HBM fractions are given, the executed
+                FP64 rate and the counted HBM traffic from the committed
+                rocprofv3 counter summaries under profiles/, and the step's
+                algorithmic HBM and FP64 fractions
+  cpu_baseline  the CPU oracle (contracted FMAs, built for the host CPU) timing
+                RK3 steps of the same 128^3 workload on the host cores this
+                process may use (child process, OMP_PROC_BIND=close,
+                OMP_PLACES=cores): median of 5 steps, plus one step of the
+                64^3 blast on one thread
 """
 import argparse
 import ctypes as C
@@ -50,6 +58,24 @@ KERNEL_FAMILIES = {"flux": ("flux_x", "flux_y", "flux_z")}
 # Whole-step algorithmic bytes per zone-update (flat metric; RHS 3x(38..48
 # reads + 10 writes) + c2p 3x) -- 1552 B.
 STEP_ALG_BYTES = 1552
+RIDGE_FLOP_PER_BYTE = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+# Algorithmic FP64 operations, counted by running the CPU oracle with an
+# operation-counting scalar on this 128^3 workload (oracle/count_flops.py;
+# add/sub and mul, the oracle has no FMA; divisions and square roots counted
+# apart).  Per face of one direction (WENO-Z of 10 variables on both sides,
+# EOS, HLLE fluxes, UCT face values; flat metric: no face-metric set-up):
+FALG_FACE_FLOPS = 2041
+FALG_FACE_DIVSQRT = 122
+# per edge (UCT: 4 WENO-Z reconstructions + 2 HLL fluxes):
+FALG_EDGE_FLOPS = 565
+# per zone-update of the RK3 step (3 x (3 faces, 3 edges, flux divergence,
+# 3 CT faces, c2p) + 2 linear combinations), flat metric (no geometric
+# sources and no metric set-up, which a flat-metric kernel skips) and as
+# the reference evaluates it with the metric terms:
+FALG_STEP_FLOPS_FLAT = 26034
+FALG_STEP_FLOPS_REFERENCE = 30171
+# the same per face direction launch of the flux kernel (one face per zone)
+FALG_PER_ZONE = {"flux": FALG_FACE_FLOPS}
 
 
 def log(msg):
@@ -77,11 +103,60 @@ def read_profile_counters(kernel_tag):
     return out
 
 
-def cpu_baseline(wl, steps=1, threads=None):
-    """Times the oracle (liboracle_fast.so) on one RK3 step of the workload."""
-    path = os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so")
-    if not os.path.exists(path):
-        return None
+def _oracle_native_lib():
+    """The oracle (contracted FMAs) compiled for this host's CPU
+    (-O3 -march=native -fopenmp) into a temporary directory; the in-tree
+    portable build (-march=x86-64-v3) if no compiler is available."""
+    import shutil
+    import subprocess
+    import tempfile
+    src = os.path.join(ROOT, "oracle", "oracle_capi.cpp")
+    cxx = shutil.which("g++")
+    if cxx:
+        out = os.path.join(tempfile.mkdtemp(prefix="grmhd_oracle_"), "liboracle_native.so")
+        r = subprocess.run([cxx, "-std=c++17", "-O3", "-march=native", "-fPIC", "-shared", "-fopenmp",
+                            "-ffp-contract=fast", "-o", out, src], capture_output=True, text=True)
+        if r.returncode == 0:
+            return out, "-O3 -march=native -fopenmp -ffp-contract=fast"
+        log(f"[bench] native oracle build failed: {r.stderr[-500:]}")
+    return os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so"), "-O3 -march=x86-64-v3 -fopenmp (prebuilt)"
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_oracle_steps(L, wl, reps):
+    """Seconds of each of `reps` RK3 steps (device-CFL dt) after one warm-up."""
+    P = C.c_void_p
+    st, B, aux = wl.copy_arrays()
+    h = L.orc_stepper_create(C.byref(wl.params), C.byref(wl.grid), wl.instances, len(wl.inst_specs))
+    p = lambda a: C.c_void_p(a.ctypes.data)
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        dt = L.orc_find_dt(h, p(st), p(aux), 0.25)
+        L.orc_rk3_step(h, dt, p(st), p(B[0]), p(B[1]), p(B[2]), p(aux))
+        if r:
+            times.append(time.perf_counter() - t0)
+    L.orc_stepper_destroy(h)
+    return times
+
+
+def cpu_baseline_child(reps=5):
+    """Body of the CPU-baseline child process (no GPU): the oracle on the
+    host cores, OMP_PROC_BIND=close OMP_PLACES=cores set by the parent.
+    Median of `reps` RK3 steps of the full 128^3 workload after one warm-up
+    on all threads, and one step of the 64^3 blast (64 blocks) on 1 thread."""
+    from grace_amd import workloads as W
+    path, flags = _oracle_native_lib()
     L = C.CDLL(path)
     P, d = C.c_void_p, C.c_double
     L.orc_num_threads.argtypes = [C.c_int]
@@ -92,28 +167,42 @@ def cpu_baseline(wl, steps=1, threads=None):
     L.orc_rk3_step.argtypes = [P, d, P, P, P, P, P]
     L.orc_find_dt.argtypes = [P, P, P, d]
     L.orc_find_dt.restype = d
-    nthr = threads or min(16, os.cpu_count() or 1)
-    nthr = L.orc_num_threads(nthr)
-    st, B, aux = wl.copy_arrays()
-    h = L.orc_stepper_create(C.byref(wl.params), C.byref(wl.grid), wl.instances, len(wl.inst_specs))
-    p = lambda a: C.c_void_p(a.ctypes.data)
-    # warm-up on the same arrays (first touch of the scratch)
-    t0 = time.perf_counter()
-    dt = L.orc_find_dt(h, p(st), p(aux), 0.25)
-    L.orc_rk3_step(h, dt, p(st), p(B[0]), p(B[1]), p(B[2]), p(aux))
-    t_warm = time.perf_counter() - t0
-    times = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        dt = L.orc_find_dt(h, p(st), p(aux), 0.25)
-        L.orc_rk3_step(h, dt, p(st), p(B[0]), p(B[1]), p(B[2]), p(aux))
-        times.append(time.perf_counter() - t0)
-    L.orc_stepper_destroy(h)
+    nthr = L.orc_num_threads(0)
+    wl = W.blast(nb=(8, 8, 8), n=16)
+    times = _time_oracle_steps(L, wl, reps)
     t = float(np.median(times))
+    L.orc_num_threads(1)
+    small = W.blast(nb=(4, 4, 4), n=16)
+    t1 = _time_oracle_steps(L, small, 1)[0]
     return {"value": wl.zones / t, "unit": "zone-updates/s", "cores": nthr, "kind": "port",
-            "sample": f"{steps} timed RK3 step(s) (+1 warm-up, {t_warm:.1f}s) of the full {wl.name} workload "
-                      f"({wl.zones} zones), C++ oracle built -O3 -fopenmp, {nthr} threads, median step "
-                      f"{t:.2f}s"}
+            "cpu_model": _cpu_model(), "build": flags,
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "step_seconds": [round(x, 4) for x in times],
+            "one_thread": {"value": small.zones / t1, "unit": "zone-updates/s",
+                           "sample": f"one RK3 step of the 64^3 blast (64 blocks of 16^3), {t1:.2f}s"},
+            "sample": f"median of {reps} RK3 steps (after 1 warm-up) of the full 128^3 blast "
+                      f"({wl.zones} zones, device-CFL dt), C++ oracle, {nthr} OpenMP threads, "
+                      f"median step {t:.3f}s"}
+
+
+def cpu_baseline(threads=None):
+    """Runs cpu_baseline_child in a child process (the OpenMP binding
+    variables must be set before its runtime starts) on the host cores this
+    process may use; returns its JSON or None."""
+    import subprocess
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        ncpu = min(ncpu, int(os.environ["OMP_NUM_THREADS"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads or ncpu), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child"], env=env,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"[bench] cpu baseline child failed ({r.returncode}): {r.stderr[-800:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def state_residual(a, b):
@@ -412,7 +501,8 @@ def main():
     ap.add_argument("--no-torus", action="store_true", help="skip the Kerr-Schild torus leg")
     ap.add_argument("--no-tune", action="store_true", help="default workgroup sizes instead of grmhd_autotune")
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--one-stream", action="store_true",
                     help="every launch on one stream (grmhd_params.stage_streams = 0), for kernel traces "
@@ -422,10 +512,21 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.cpu_child:
+        print(json.dumps(cpu_baseline_child(reps=args.cpu_reps)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported: the kernels run one "
               "GPU's share of the blocks, the multi-GPU halo exchange is not implemented", file=sys.stderr)
         sys.exit(2)
+    cpu = None
+    if not (args.no_cpu or args.only_torus or args.copy_probe):
+        # before this process touches the GPU: a child process on the host cores
+        log("[bench] CPU baseline (oracle, child process)")
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
 
     import torch
     import grace_amd as ga
@@ -514,16 +615,38 @@ def main():
     profs = [read_profile_counters(f"{f}_blast128") for f in kern[dom][2]]
     prof = {key: sum(p[key] for p in profs) / len(profs) for key in profs[0] if all(key in p for p in profs)}
     traffic = prof.get("hbm_bytes_per_launch")
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
-            "timer_families": list(kern[dom][2]),
-            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_zone": ALG_BYTES[dom],
-            "kernel_ms_per_step": {k: round(v[0] * v[1] / 2, 4) for k, v in fam.items()},
-            "step_hbm_frac": round(STEP_ALG_BYTES * value / 1e9 / HBM_PEAK_GBS, 5)}
+    # Which roof bounds the kernel: its algorithmic intensity against the
+    # ridge point (peak FP64 / peak HBM bandwidth)
+    hbm = {"achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5), "alg_bytes_per_zone": ALG_BYTES[dom]}
+    roof = {"traffic": traffic, "kernel": dom, "timer_families": list(kern[dom][2]),
+            "avg_launch_ms": round(avg_ms, 4)}
+    if dom in FALG_PER_ZONE:
+        intensity = FALG_PER_ZONE[dom] / ALG_BYTES[dom]
+        tf = FALG_PER_ZONE[dom] * wl.zones / (avg_ms * 1e-3) / 1e12
+        fp64 = {"achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / FP64_PEAK_TFLOPS, 4), "alg_flops_per_zone": FALG_PER_ZONE[dom],
+                "source": "oracle operation count (oracle/count_flops.py), add + mul"}
+        bound = "fp64" if intensity > RIDGE_FLOP_PER_BYTE else "hbm"
+        roof["intensity_flop_per_byte"] = round(intensity, 2)
+        roof["ridge_flop_per_byte"] = round(RIDGE_FLOP_PER_BYTE, 2)
+    else:
+        fp64, bound = None, "hbm"
+    top = fp64 if bound == "fp64" else hbm
+    roof = {"bound": bound, "achieved": top["achieved"], "peak": top["peak"], "unit": top["unit"],
+            "frac": top["frac"], **roof, "hbm": hbm, "fp64_algorithmic": fp64}
     if "fp64_flops_per_launch" in prof:
-        tf = prof["fp64_flops_per_launch"] / (avg_ms * 1e-3) / 1e12
-        roof["fp64"] = {"achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(tf / FP64_PEAK_TFLOPS, 4), "source": "rocprofv3 SQ_INSTS_VALU_*_F64"}
+        tfx = prof["fp64_flops_per_launch"] / (avg_ms * 1e-3) / 1e12
+        roof["fp64_executed"] = {"achieved": round(tfx, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": round(tfx / FP64_PEAK_TFLOPS, 4),
+                                 "source": "rocprofv3 SQ_INSTS_VALU_*_F64 (FMA = 2)"}
+    roof["counters_from"] = [f"profiles/{f}_blast128_pmc.md" for f in kern[dom][2]]
+    roof["kernel_ms_per_step"] = {k: round(v[0] * v[1] / 2, 4) for k, v in fam.items()}
+    roof["step"] = {"hbm_frac": round(STEP_ALG_BYTES * value / 1e9 / HBM_PEAK_GBS, 5),
+                    "alg_bytes_per_zone_update": STEP_ALG_BYTES,
+                    "fp64_frac": round(FALG_STEP_FLOPS_FLAT * value / 1e12 / FP64_PEAK_TFLOPS, 5),
+                    "alg_flops_per_zone_update": FALG_STEP_FLOPS_FLAT,
+                    "alg_flops_per_zone_update_with_metric_terms": FALG_STEP_FLOPS_REFERENCE}
 
     c2p_leg = None
     if not args.no_c2p:
@@ -531,14 +654,6 @@ def main():
     c1 = None if args.no_c1 else c1_leg(ga, W, torch, dev)
     c3 = None if args.no_c3 else c3_leg(ga, W, torch, dev)
     torus = None if args.no_torus else torus_leg(ga, W, torch, dev, tune=not args.no_tune)
-
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU baseline (oracle)")
-        try:
-            cpu = cpu_baseline(wl, steps=args.cpu_steps)
-        except Exception as e:  # the baseline is reported, never required
-            log(f"[bench] cpu baseline failed: {e}")
 
     out = {
         "metric": "GRMHD zone-updates/s on 1 MI355X, learned vs plain c2p at <=1e-10 c2p err",
@@ -556,6 +671,7 @@ def main():
         "config": {"workload": "blast3d_128cube_512x16cube_blocks_ngz3_wenoz_hlle_uct_rk3_plain_c2p",
                    "zones": wl.zones, "blocks": int(wl.grid.nq), "c2p": "plain Kastaun (FP64)",
                    "graph": use_graph, "stage_streams": int(wl.params.stage_streams),
+                   "flat_metric": int(wl.params.flat_metric), "alg_bytes_variant": "flat metric, 1552 B",
                    "gpu_event_ms_per_step": gpu_ms / args.steps, "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -23,6 +23,19 @@
 #include <vector>
 #include <cstdio>
 
+// Operation-count hooks: the counting build (flop_count.hh) tallies the
+// operations of each pass under its id; no-ops in the checker build.
+// ORC_PASS: 1 fluxes, 2 edge EMFs, 3 flux divergence, 4 CT update, 5 c2p
+// over cells, 6 RK linear combinations, 7 FOFC, 0 everything else.
+// ORC_SUB(8) tags the geometric source terms within pass 3; ORC_METRIC
+// tags metric set-up and metric derivatives (id 10 + the enclosing pass),
+// which a flat-metric kernel does not evaluate.
+#ifndef ORC_PASS
+#define ORC_PASS(id) ((void)0)
+#define ORC_SUB(id) ((void)0)
+#define ORC_METRIC() ((void)0)
+#endif
+
 namespace oracle {
 
 enum Recon { REC_WENOZ = 0, REC_MC2 = 1, REC_MINMOD = 2, REC_DONOR = 3 };
@@ -216,6 +229,7 @@ struct Stepper {
     }
 
     Metric metric_at(const double* st, int i, int j, int k, int64_t q) const {
+        ORC_METRIC();
         const Grid& g = G();
         double gg[6], b[3];
         for (int a = 0; a < 6; ++a) gg[a] = st[g.cc(i, j, k, GXX_ + a, q, NSTATE)];
@@ -232,6 +246,7 @@ struct Stepper {
                              st[g.cc(i + di, j + dj, k + dk, v, q, NSTATE)]);
     }
     Metric face_metric(const double* st, int i, int j, int k, int64_t q, int d) const {
+        ORC_METRIC();
         double gg[6], b[3];
         for (int a = 0; a < 6; ++a) gg[a] = face4(st, i, j, k, GXX_ + a, q, d);
         for (int a = 0; a < 3; ++a) b[a] = face4(st, i, j, k, BETAX_ + a, q, d);
@@ -356,6 +371,7 @@ struct Stepper {
     }
 
     void compute_fluxes(const Fields& old, const double* aux) {
+        ORC_PASS(1);
         const Grid& g = G();
         for (int64_t q = 0; q < g.nq; ++q)
             for (int d = 0; d < 3; ++d) {
@@ -391,6 +407,7 @@ struct Stepper {
     double vb(int fd, int c, int i, int j, int k, int64_t q) const { return vbar[fd][G().fc(fd, i, j, k, c, q, 4)]; }
 
     void compute_emfs_uct(const Fields& old) {
+        ORC_PASS(2);
         const Grid& g = G();
         int ng = g.ngz;
         for (int64_t q = 0; q < g.nq; ++q) {
@@ -499,6 +516,7 @@ struct Stepper {
     // edge range [ngz, n+ngz] (evolve.cpp:1090-1104); one layer wider with
     // FOFC and fofc_ghost_emf
     void compute_emfs_gs() {
+        ORC_PASS(2);
         const Grid& g = G();
         int ng = g.ngz;
         const int w = (opt.fofc && opt.fofc_ghost_emf) ? 1 : 0;
@@ -516,6 +534,7 @@ struct Stepper {
 
     // Centered FD first derivative along d (fd_subexpressions.hh:21-57)
     double fd_der(const double* st, int i, int j, int k, int v, int64_t q, int d, double ih) const {
+        ORC_METRIC();
         const Grid& g = G();
         int di = d == 0, dj = d == 1, dk = d == 2;
         double u[7];
@@ -528,6 +547,7 @@ struct Stepper {
 
     // compute_source_terms (grmhd.hh:190-297) + divergence (evolve.cpp:1395-1401)
     void add_fluxes_and_sources(double dt, double dtfact, double* newst, const Fields& old, const double* aux) {
+        ORC_PASS(3);
         const Grid& g = G();
         int ng = g.ngz;
         for (int64_t q = 0; q < g.nq; ++q) {
@@ -538,6 +558,8 @@ struct Stepper {
                     for (int i = ng; i < g.nx + ng; ++i) {
                         const double* st = old.state;
                         Metric m = metric_at(st, i, j, k, q);
+                        {
+                        ORC_SUB(8);
                         double z[3], B[3];
                         for (int a = 0; a < 3; ++a) {
                             z[a] = aux[g.cc(i, j, k, ZVECX_ + a, q, NAUX)];
@@ -559,6 +581,7 @@ struct Stepper {
                         newst[g.cc(i, j, k, SX_, q, NSTATE)] += m.sqrtg * dt * dtfact * ss[0];
                         newst[g.cc(i, j, k, SY_, q, NSTATE)] += m.sqrtg * dt * dtfact * ss[1];
                         newst[g.cc(i, j, k, SZ_, q, NSTATE)] += m.sqrtg * dt * dtfact * ss[2];
+                        }
                         for (int v = 0; v < NHRSC; ++v) {
                             double dFx = (flux[0][g.fc(0, i, j, k, v, q, NHRSC)] - flux[0][g.fc(0, i + 1, j, k, v, q, NHRSC)]) * ix;
                             double dFy = (flux[1][g.fc(1, i, j, k, v, q, NHRSC)] - flux[1][g.fc(1, i, j + 1, k, v, q, NHRSC)]) * iy;
@@ -571,6 +594,7 @@ struct Stepper {
 
     // update_CT (evolve.cpp:1406-1479)
     void update_ct(double dt, double dtfact, double* nB[3]) {
+        ORC_PASS(4);
         const Grid& g = G();
         int ng = g.ngz;
         for (int64_t q = 0; q < g.nq; ++q) {
@@ -602,6 +626,7 @@ struct Stepper {
     // 27-cell neighbourhood of `newst`, and a lenient dry-run c2p; a cell
     // that fails tags its 6 faces and 12 edges.
     void flag_fofc(double dt, double dtfact, const double* newst, const Fields& old) {
+        ORC_PASS(7);
         const Grid& g = G();
         const int ng = g.ngz;
         for (int d = 0; d < 3; ++d) ftag[d].assign((size_t)g.fc_size(d, 1), 0);
@@ -676,6 +701,7 @@ struct Stepper {
     // tagged faces, then the GS edge EMFs on the tagged edges (the compaction
     // ranges of evolve.cpp:646-702).
     void apply_fofc(const Fields& old, const double* aux) {
+        ORC_PASS(7);
         const Grid& g = G();
         const int ng = g.ngz;
         const int n[3] = {g.nx, g.ny, g.nz};
@@ -799,6 +825,7 @@ struct Stepper {
     C2PStats c2p_stats;
     int64_t n_fatal = 0;
     void compute_auxiliaries(double* st, double* Bf[3], double* aux) {
+        ORC_PASS(5);
         const Grid& g = G();
         int64_t fatal = 0;
         for (int64_t q = 0; q < g.nq; ++q) {
@@ -903,6 +930,7 @@ struct Stepper {
 
     // linop_apply, 2-operand form (tstep_utils.hh:140-220): A = b B + c C
     void linop(double* A, const double* Bv, const double* Cv, double b, double c, int64_t n) {
+        ORC_PASS(6);
         for (int64_t t = 0; t < n; ++t) A[t] = b * Bv[t] + c * Cv[t];
     }
 

@@ -403,4 +403,16 @@ int orc_forest_block(void* h, int64_t q, int32_t* level, int32_t coords[3]) {
     return 0;
 }
 
+#ifdef ORACLE_COUNT_FLOPS
+// Operation counts (liboracle_count.so only, see flop_count.hh):
+// out[pass * 5 + kind], 20 pass ids x {add, mul, div, sqrt, pow/log/exp}.
+// Reset also pins the library to one thread (the counters are plain).
+void orc_flops_reset(void) {
+    std::memset(orcflop::g_counts, 0, sizeof(orcflop::g_counts));
+    orcflop::g_pass = 0;
+    omp_set_num_threads(1);
+}
+void orc_flops_read(int64_t* out) { std::memcpy(out, orcflop::g_counts, sizeof(orcflop::g_counts)); }
+#endif
+
 }  // extern "C"
