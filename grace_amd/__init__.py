@@ -96,6 +96,7 @@ ABI_SYMBOLS = [
     "grmhd_ctx_create_forest", "grmhd_forest_block", "grmhd_setup_forest_wave", "grmhd_setup_fm_torus_forest",
     "grmhd_fill_metric_ghosts",
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
+    "grmhd_c2p_batch_noalloc",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
     "grmhd_autotune", "grmhd_get_tuning", "grmhd_set_tuning",
@@ -158,6 +159,7 @@ def lib():
     L.grmhd_fill_metric_ghosts.argtypes = [P, P, P]
     L.grmhd_compute_auxiliaries.argtypes = [P, P, P * 3, P, P]
     L.grmhd_c2p_batch.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P, P]
+    L.grmhd_c2p_batch_noalloc.argtypes = [P, i64, P, P, P, P, P, i32, P, P, P, P]
     L.grmhd_prims_to_conservs_batch.argtypes = [P, i64, P, P, P, P]
     L.grmhd_c2p_stats.argtypes = [P, C.POINTER(C2PStats), P]
     L.grmhd_compute_dt.argtypes = [P, P, P, d, P, P]
@@ -316,6 +318,12 @@ class Context:
     def c2p_batch(self, n, cons, met, prims, err, floored, dry_run=0, r=None, guess=None, roots=None, stream=None):
         check(self.L.grmhd_c2p_batch(self.h, n, _ptr(cons), _ptr(met), _ptr(prims), _ptr(err), _ptr(floored),
                                      int(dry_run), _ptr(r), _ptr(guess), _ptr(roots), _ptr(stream)), "c2p_batch")
+
+    def c2p_batch_noalloc(self, n, cons, met, prims, err, floored, dry_run=0, r=None, guess=None, roots=None,
+                          stream=None):
+        check(self.L.grmhd_c2p_batch_noalloc(self.h, n, _ptr(cons), _ptr(met), _ptr(prims), _ptr(err),
+                                             _ptr(floored), int(dry_run), _ptr(r), _ptr(guess), _ptr(roots),
+                                             _ptr(stream)), "c2p_batch_noalloc")
 
     def p2c_batch(self, n, prims, met, cons, stream=None):
         check(self.L.grmhd_prims_to_conservs_batch(self.h, n, _ptr(prims), _ptr(met), _ptr(cons), _ptr(stream)),

@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <mutex>
 #include "phys.hh"
 #include "mlp.hh"
 #include "forest.hh"
@@ -1019,6 +1020,14 @@ __global__ __launch_bounds__(256) void k_stats_final(const DevStats* __restrict_
     }
 }
 
+// Per-workgroup statistics records of grmhd_c2p_batch_noalloc: a file-scope
+// device array instead of context memory that grows with the batch.  One
+// record per 256-state workgroup for at most GRMHD_NOALLOC_MAX_STATES states:
+// 65,536 records x 64 B = 4 MiB, held by every GPU the library runs on until
+// the process ends.  Host code takes its device address from
+// hipGetSymbolAddress (noalloc_scratch_ptr), never from the name.
+static __device__ __align__(256) DevStats scratch[GRMHD_NOALLOC_MAX_STATES / 256];
+
 // ---------------------------------------------------------------------------
 // c2p over cells (compute_auxiliaries)
 // ---------------------------------------------------------------------------
@@ -1324,6 +1333,7 @@ struct grmhd_ctx {
     DevStats* stats = nullptr;
     DevStats* stats_part = nullptr;
     int64_t n_part_cap = 0;
+    DevStats* noalloc_part = nullptr;         // device address of `scratch` on this context's device
     int32_t* ghost_list = nullptr;
     int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
     uint8_t* ftag[3] = {nullptr, nullptr, nullptr};   // FOFC face tags (per face of every block)
@@ -1451,6 +1461,28 @@ struct ForestIn {
     std::vector<int4> fblk;
 };
 
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+static grmhd_status noalloc_scratch_ptr(DevStats** out) {
+    static std::mutex mu;
+    static DevStats* ptr[64] = {};
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) { set_err("device ordinal %d out of range", dev); return GRMHD_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return GRMHD_ERR_HIP;
+        }
+        ptr[dev] = (DevStats*)p;
+    }
+    *out = ptr[dev];
+    return GRMHD_OK;
+}
+
 static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, const grmhd_instance* inst,
                                     int32_t n_inst, const ForestIn* fin, grmhd_ctx** out) {
     if (!p || !g || !out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
@@ -1567,6 +1599,10 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     // 2-D c2p grids with workgroups down to 64 threads, plus per-launch rounding
     c->n_part_cap = nblk_bs(G.ncell * G.nq, 64) + 2 * G.nq + 2;
     if ((e = hipMalloc(&c->stats_part, sizeof(DevStats) * c->n_part_cap)) != hipSuccess) return fail("stats part", e);
+    {
+        grmhd_status sst = noalloc_scratch_ptr(&c->noalloc_part);
+        if (sst != GRMHD_OK) { free_ctx(c); delete c; return sst; }
+    }
     {   // ghost-shell tables (local offsets), cells then x/y/z faces
         std::vector<int32_t> lst;
         for (int fd = -1; fd < 3; ++fd) {
@@ -2068,6 +2104,26 @@ extern "C" grmhd_status grmhd_compute_auxiliaries(grmhd_ctx* c, double* state, c
     return launch_check("compute_auxiliaries");
 }
 
+// Batch c2p + statistics reduction with the per-workgroup records in `part`
+// (nblk(n) records; every workgroup writes its own, nothing needs clearing).
+static void launch_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_soa, const double* metric_soa, double* prims_soa,
+                             uint64_t* err_bits, uint8_t* floored, int32_t dry_run, const double* r,
+                             const double* guess, double* roots, DevStats* part, hipStream_t s) {
+    if (n > 0) {
+        MlpDev none{};
+        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
+            k_c2p_batch<EosIdeal, false><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa,
+                                                                err_bits, floored, dry_run, r, guess, roots, part,
+                                                                none);
+        else
+            k_c2p_batch<EosHybrid, false><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa, prims_soa,
+                                                                 err_bits, floored, dry_run, r, guess, roots, part,
+                                                                 none);
+        k_stats_final<<<1, 256, 0, s>>>(part, nblk(n), c->stats);
+    }
+    c->last_c2p_n = n;
+}
+
 extern "C" grmhd_status grmhd_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_soa, const double* metric_soa,
                                         double* prims_soa, uint64_t* err_bits, uint8_t* floored, int32_t dry_run,
                                         const double* r, const double* guess, double* roots, grmhd_stream_t stream) {
@@ -2083,20 +2139,28 @@ extern "C" grmhd_status grmhd_c2p_batch(grmhd_ctx* c, int64_t n, double* cons_so
         c->n_part_cap = nblk(n);
     }
     Scope sc(c, "c2p", s);
-    if (n > 0) {
-        MlpDev none{};
-        if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS)
-            k_c2p_batch<EosIdeal, false><<<nblk(n), 256, 0, s>>>(n, c->ideal, c->O, cons_soa, metric_soa, prims_soa,
-                                                                err_bits, floored, dry_run, r, guess, roots,
-                                                                c->stats_part, none);
-        else
-            k_c2p_batch<EosHybrid, false><<<nblk(n), 256, 0, s>>>(n, c->hybrid, c->O, cons_soa, metric_soa, prims_soa,
-                                                                 err_bits, floored, dry_run, r, guess, roots,
-                                                                 c->stats_part, none);
-        k_stats_final<<<1, 256, 0, s>>>(c->stats_part, nblk(n), c->stats);
-    }
-    c->last_c2p_n = n;
+    launch_c2p_batch(c, n, cons_soa, metric_soa, prims_soa, err_bits, floored, dry_run, r, guess, roots,
+                     c->stats_part, s);
     return launch_check("c2p_batch");
+}
+
+extern "C" grmhd_status grmhd_c2p_batch_noalloc(grmhd_ctx* c, int64_t n, double* cons_soa, const double* metric_soa,
+                                                double* prims_soa, uint64_t* err_bits, uint8_t* floored,
+                                                int32_t dry_run, const double* r, const double* guess, double* roots,
+                                                grmhd_stream_t stream) {
+    if (!c || n < 0 || (n > 0 && (!cons_soa || !metric_soa || !prims_soa || !err_bits || !floored))) {
+        set_err("invalid c2p batch arguments"); return GRMHD_ERR_INVALID;
+    }
+    if (n > GRMHD_NOALLOC_MAX_STATES) {
+        set_err("c2p_batch_noalloc: n = %lld exceeds the %d states its static scratch holds", (long long)n,
+                GRMHD_NOALLOC_MAX_STATES);
+        return GRMHD_ERR_INVALID;
+    }
+    if (!c->noalloc_part) { set_err("c2p_batch_noalloc: no scratch address for this context"); return GRMHD_ERR_HIP; }
+    // no Scope: kernel timing creates events
+    launch_c2p_batch(c, n, cons_soa, metric_soa, prims_soa, err_bits, floored, dry_run, r, guess, roots,
+                     c->noalloc_part, (hipStream_t)stream);
+    return launch_check("c2p_batch_noalloc");
 }
 
 static grmhd_status ensure_stats_part(grmhd_ctx* c, int64_t n) {

@@ -236,6 +236,22 @@ grmhd_status grmhd_c2p_batch(grmhd_ctx* ctx, int64_t n, double* cons_soa, const 
                              const double* r_or_null, const double* guess_or_null, double* roots_or_null,
                              grmhd_stream_t stream);
 
+/* grmhd_c2p_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as grmhd_c2p_batch, whose statistics records
+ * grow a context buffer with the batch; here they live in a static device
+ * array of the library sized for at most GRMHD_NOALLOC_MAX_STATES states
+ * (4 MiB per GPU, held until the process ends).  Larger n returns
+ * GRMHD_ERR_INVALID.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (different streams, different contexts) race, and
+ * grmhd_c2p_stats then reports an undefined mix; serialise them. */
+#define GRMHD_NOALLOC_MAX_STATES (1 << 24)
+grmhd_status grmhd_c2p_batch_noalloc(grmhd_ctx* ctx, int64_t n, double* cons_soa, const double* metric_soa,
+                                     double* prims_soa, uint64_t* err_bits, uint8_t* floored, int32_t dry_run,
+                                     const double* r_or_null, const double* guess_or_null, double* roots_or_null,
+                                     grmhd_stream_t stream);
+
 /* prims_to_conservs (src/physics/eos/c2p.cpp:416-455) over a flat batch. */
 grmhd_status grmhd_prims_to_conservs_batch(grmhd_ctx* ctx, int64_t n, const double* prims_soa,
                                            const double* metric_soa, double* cons_soa,
