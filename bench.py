@@ -386,7 +386,8 @@ def torus_leg(ga, W, torch, dev, steps=5, warmup=2, tune=True):
            "blocks": int(wl.grid.nq), "fine_blocks": int(wl.extra["n_fine"]),
            "static_metric_cache": True,
            "zones": wl.zones, "steps": steps}
-    for mode in ("plain", "learned"):
+    runs = {}
+    for mode in ("plain", "learned", "warm"):
         ctx = ga.Context(wl.params, wl.grid, wl.instances, forest=wl.forest)
         st = torch.from_numpy(wl.state).to(dev)
         B = [torch.from_numpy(b).to(dev) for b in wl.B]
@@ -402,23 +403,36 @@ def torus_leg(ga, W, torch, dev, steps=5, warmup=2, tune=True):
             ctx.set_tuning(**out["tuning"])
         if mode == "learned":
             t0 = time.perf_counter()
-            rep = mlp.fit_cells(ctx, st, B, max_states=1 << 20)
+            mlp.fit_cells(ctx, st, B, max_states=1 << 20)
             out["fit_seconds"] = round(time.perf_counter() - t0, 3)
             ctx.set_learned_c2p(mlp)
-        dtd = torch.zeros(1, dtype=torch.float64, device=dev)
-        stream = torch.cuda.Stream()
+        if mode == "warm":
+            ctx.set_c2p_warm_start(8)
+        runs[mode] = (ctx, st, B, aux, torch.zeros(1, dtype=torch.float64, device=dev), torch.cuda.Stream())
+
+    def one_step(mode):
+        ctx, st, B, aux, dtd, stream = runs[mode]
+        sp = stream.cuda_stream
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            e0.record(stream)
+            ctx.compute_dt(st, aux, 0.25, dtd, stream=sp)
+            ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=True)
+            e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_time(e1)
+
+    # the two runs alternate step by step (same clocks and cache state)
+    for _ in range(warmup):
+        for mode in runs:
+            one_step(mode)
+    times = {m: [] for m in runs}
+    for _ in range(steps):
+        for mode in runs:
+            times[mode].append(one_step(mode))
+    for mode, (ctx, st, B, aux, dtd, stream) in runs.items():
         with torch.cuda.stream(stream):
             sp = stream.cuda_stream
-            for _ in range(warmup):
-                ctx.compute_dt(st, aux, 0.25, dtd, stream=sp)
-                ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=True)
-            stream.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                ctx.compute_dt(st, aux, 0.25, dtd, stream=sp)
-                ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=True)
-            stream.synchronize()
-            wall = time.perf_counter() - t0
             ctx.set_timing(1)
             ctx.rk3_step(dtd, st, B, aux, stream=sp, graph=False)
             stream.synchronize()
@@ -427,13 +441,17 @@ def torus_leg(ga, W, torch, dev, steps=5, warmup=2, tune=True):
         ctx.compute_auxiliaries(st, B, aux)
         torch.cuda.synchronize()
         stt = ctx.c2p_stats()
-        out[mode] = {"ms_per_step": round(1e3 * wall / steps, 4), "zone_updates_per_s": wl.zones * steps / wall,
+        ms = float(np.median(times[mode]))
+        out[mode] = {"ms_per_step": round(ms, 4), "zone_updates_per_s": wl.zones / (ms * 1e-3),
                      "c2p_ms_per_step": round(c2p_ms, 4), "dt": float(dtd.item()),
                      "finite": bool(torch.isfinite(st).all().item() and torch.isfinite(aux).all().item()),
                      "root_evals_per_cell": round(stt["n_root_evals"] / max(stt["n_cells"], 1), 3),
                      "bracket_evals_per_cell": round(stt["n_bracket_evals"] / max(stt["n_cells"], 1), 3),
                      "atmosphere_fraction": round(stt["n_atmo_reset"] / max(stt["n_cells"], 1), 4)}
-        del ctx, st, B, aux
+    out["timing"] = "median of per-step HIP-event times, plain, learned and warm-start steps alternated"
+    out["warm"]["guess"] = "Newton finish from each cell's previous primitives (grmhd_set_c2p_warm_start)"
+    out["warm_vs_plain_step"] = round(out["plain"]["ms_per_step"] / out["warm"]["ms_per_step"], 4)
+    del runs
     out["learned_vs_plain_step"] = round(out["plain"]["ms_per_step"] / out["learned"]["ms_per_step"], 4)
     return out
 
@@ -507,6 +525,8 @@ def main():
     ap.add_argument("--one-stream", action="store_true",
                     help="every launch on one stream (grmhd_params.stage_streams = 0), for kernel traces "
                          "whose launch durations are not overlapped")
+    ap.add_argument("--only-c3", action="store_true",
+                    help="only run the batched uneven-instance leg and print its JSON (for kernel traces)")
     ap.add_argument("--only-torus", action="store_true",
                     help="only run the two-level-forest torus leg and print its JSON (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -520,7 +540,7 @@ def main():
               "GPU's share of the blocks, the multi-GPU halo exchange is not implemented", file=sys.stderr)
         sys.exit(2)
     cpu = None
-    if not (args.no_cpu or args.only_torus or args.copy_probe):
+    if not (args.no_cpu or args.only_torus or args.only_c3 or args.copy_probe):
         # before this process touches the GPU: a child process on the host cores
         log("[bench] CPU baseline (oracle, child process)")
         try:
@@ -533,6 +553,9 @@ def main():
     from grace_amd import workloads as W
 
     dev = torch.device("cuda:0")
+    if args.only_c3:
+        print(json.dumps(c3_leg(ga, W, torch, dev), default=float), flush=True)
+        return
     if args.only_torus:
         print(json.dumps(torus_leg(ga, W, torch, dev, steps=args.steps, warmup=args.warmup, tune=not args.no_tune),
                          default=float), flush=True)
@@ -597,7 +620,8 @@ def main():
             ctx.rk3_step(dt, st, B, aux, stream=sp, graph=False)
         stream.synchronize()
     fam = {}
-    for f in ("flux_x", "flux_y", "flux_z", "emf", "update", "ghost", "ghost_faces", "c2p"):
+    for f in ("flux_x", "flux_y", "flux_z", "ring_x", "ring_y", "ring_z", "emf", "update", "ghost", "ghost_faces",
+              "c2p"):
         ms, n = ctx.kernel_time_ms(f)
         if n:
             fam[f] = (ms / n, n)

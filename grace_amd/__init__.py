@@ -96,7 +96,8 @@ ABI_SYMBOLS = [
     "grmhd_ctx_create_forest", "grmhd_forest_block", "grmhd_setup_forest_wave", "grmhd_setup_fm_torus_forest",
     "grmhd_fill_metric_ghosts",
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
-    "grmhd_c2p_batch_noalloc",
+    "grmhd_c2p_batch_noalloc", "grmhd_advance_substep_t", "grmhd_sync", "grmhd_find_stable_timestep",
+    "grmhd_set_c2p_warm_start",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
     "grmhd_autotune", "grmhd_get_tuning", "grmhd_set_tuning",
@@ -155,6 +156,9 @@ def lib():
                                           P, P, P, P, P]
     L.grmhd_last_error.restype = C.c_char_p
     L.grmhd_advance_substep.argtypes = [P, P, d, P, P, P * 3, P * 3, P, P]
+    L.grmhd_advance_substep_t.argtypes = [P, d, d, d, P, P, P * 3, P * 3, P, P]
+    L.grmhd_sync.argtypes = [P, P]
+    L.grmhd_find_stable_timestep.argtypes = [P, P, P, d, C.POINTER(d), P]
     L.grmhd_fill_ghosts.argtypes = [P, P, P * 3, P]
     L.grmhd_fill_metric_ghosts.argtypes = [P, P, P]
     L.grmhd_compute_auxiliaries.argtypes = [P, P, P * 3, P, P]
@@ -207,6 +211,7 @@ def lib():
     L.grmhd_c2p_batch_learned.argtypes = [P, P, i64, P, P, P, P, P, P, P, P]
     L.grmhd_mlp_fit_cells.argtypes = [P, P, P, P * 3, i64, C.POINTER(MlpFitOpts), C.POINTER(MlpFitReport), P]
     L.grmhd_set_learned_c2p.argtypes = [P, P]
+    L.grmhd_set_c2p_warm_start.argtypes = [P, i32]
     L.grmhd_set_static_metric.argtypes = [P, P, i32, P]
     for name in ABI_SYMBOLS:
         getattr(L, name)  # raises AttributeError if an exported symbol is missing
@@ -305,6 +310,20 @@ class Context:
         check(self.L.grmhd_advance_substep(self.h, _ptr(dt_dev), dtfact, _ptr(new_state), _ptr(old_state),
                                            ptr3(*newB), ptr3(*oldB), _ptr(aux), _ptr(stream)), "advance_substep")
 
+    def advance_substep_t(self, t, dt, dtfact, new_state, old_state, newB, oldB, aux, stream=None):
+        check(self.L.grmhd_advance_substep_t(self.h, t, dt, dtfact, _ptr(new_state), _ptr(old_state), ptr3(*newB),
+                                             ptr3(*oldB), _ptr(aux), _ptr(stream)), "advance_substep_t")
+
+    def sync(self, stream=None):
+        """Status of the sync point (0, or 5 = GRMHD_ERR_C2P_FATAL after fatal c2p cells)."""
+        return self.L.grmhd_sync(self.h, _ptr(stream))
+
+    def find_stable_timestep(self, state, aux, cfl, stream=None):
+        dt = C.c_double()
+        check(self.L.grmhd_find_stable_timestep(self.h, _ptr(state), _ptr(aux), cfl, C.byref(dt), _ptr(stream)),
+              "find_stable_timestep")
+        return dt.value
+
     def fill_ghosts(self, state, B, stream=None):
         check(self.L.grmhd_fill_ghosts(self.h, _ptr(state), ptr3(*B), _ptr(stream)), "fill_ghosts")
 
@@ -350,6 +369,9 @@ class Context:
         """Cell-pass c2p from the MLP's guess (mlp: Mlp or None for plain)."""
         check(self.L.grmhd_set_learned_c2p(self.h, mlp.h if mlp is not None else None), "set_learned_c2p")
         self._mlp = mlp   # keep it alive while the context uses it
+
+    def set_c2p_warm_start(self, max_newton_evals=8):
+        check(self.L.grmhd_set_c2p_warm_start(self.h, int(max_newton_evals)), "set_c2p_warm_start")
 
     def autotune(self, state, B, aux, reps=3, stream=None):
         t = Tuning()

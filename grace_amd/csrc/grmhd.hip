@@ -18,6 +18,7 @@
 //   k_dt_*         CFL reduction (find_stable_timestep.cpp:72-126)
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <cstdarg>
@@ -83,6 +84,9 @@ struct Opts {
     double c_limit;
     int fofc, dmp;      // first-order flux correction, its DMP test
     double dmp_M;
+    // cells that took a branch where the reference aborts (c2p.hh:160-322),
+    // counted by the c2p passes until a sync point reads and clears it
+    unsigned long long* fatal_count;
 };
 
 GHD int64_t cidx(const DevGrid& G, int i, int j, int k) { return i + (int64_t)G.N[0] * (j + (int64_t)G.N[1] * k); }
@@ -134,7 +138,8 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
                                               const double* __restrict__ mstate, int mnv,
                                               const double* __restrict__ Bd, double* __restrict__ flux,
                                               double* __restrict__ vbar, int wpb,
-                                              const uint8_t* __restrict__ only) {
+                                              const uint8_t* __restrict__ only,
+                                              const uint8_t* __restrict__ ring_own) {
     // wpb workgroups per block (see wg_pos).  UCT: the first
     // (n_D+1) n_T0 n_T1 threads take the faces whose transverse indices are
     // both interior (full fluxes), the next ones the faces with exactly one
@@ -143,7 +148,10 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     // faces with both transverse indices in the ghost layers are never read
     // and get no thread.  GS reads fluxes and face EMFs one layer into the
     // transverse ghosts, so it takes every face (normal [ng, n+ng],
-    // transverse [0, N)).
+    // transverse [0, N)).  ring_own (per block, bit 2 d + side for the six
+    // block faces): when given, a ring face is computed only if its ghost
+    // side lies beyond an outflow face of the instance; the others are
+    // interior faces of the neighbouring block, copied by k_ring_vbar.
     const WgPos wp = wg_pos(wpb);
     const int64_t q = wp.q;
     const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
@@ -174,15 +182,21 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
             } else if (e < nint + nA) {          // t0 ghost, t1 interior
                 const int x = e - nint;
                 ring = true;
-                if (D == 0) { a = x % Rn; const int r = x / Rn; b = ghost(r % g2, n0); c = ng + r / g2; }
-                else if (D == 1) { b = ghost(x % g2, n0); const int r = x / g2; a = r % Rn; c = ng + r / Rn; }
-                else { b = ghost(x % g2, n0); const int r = x / g2; c = ng + r % n1; a = r / n1; }
+                int gl;
+                if (D == 0) { a = x % Rn; const int r = x / Rn; gl = r % g2; c = ng + r / g2; }
+                else if (D == 1) { gl = x % g2; const int r = x / g2; a = r % Rn; c = ng + r / Rn; }
+                else { gl = x % g2; const int r = x / g2; c = ng + r % n1; a = r / n1; }
+                if (ring_own && !((ring_own[q] >> (2 * T0 + (gl >= ng))) & 1)) return;
+                b = ghost(gl, n0);
             } else if (e < nint + nA + nB) {     // t0 interior, t1 ghost
                 const int x = e - nint - nA;
                 ring = true;
-                if (D == 0) { a = x % Rn; const int r = x / Rn; b = ng + r % n0; c = ghost(r / n0, n1); }
-                else if (D == 1) { b = ng + x % n0; const int r = x / n0; a = r % Rn; c = ghost(r / Rn, n1); }
-                else { b = ng + x % n0; const int r = x / n0; c = ghost(r % g2, n1); a = r / g2; }
+                int gl;
+                if (D == 0) { a = x % Rn; const int r = x / Rn; b = ng + r % n0; gl = r / n0; }
+                else if (D == 1) { b = ng + x % n0; const int r = x / n0; a = r % Rn; gl = r / Rn; }
+                else { b = ng + x % n0; const int r = x / n0; gl = r % g2; a = r / g2; }
+                if (ring_own && !((ring_own[q] >> (2 * T1 + (gl >= ng))) & 1)) return;
+                c = ghost(gl, n1);
             } else {
                 return;
             }
@@ -349,6 +363,65 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
             V[nf] = -sg * hll_r(a, b, Bl[0], Br[0], cmin, cmax, ihll);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// UCT face values (HLL transverse velocities and wave speeds) of the ring
+// faces of direction D whose ghost side has a neighbouring block: the same
+// physical face is an interior face of that block (nbr[q][side]; the
+// periodic image across a periodic face), which k_flux computes from the
+// same inputs -- this block's ghost primitives and face B are copies of its
+// cells and faces -- so the four values are copied instead of recomputed.
+// Same enumeration of ring faces as k_flux; faces beyond an outflow face
+// (ring_own) are left to k_flux.  Needs every block extent >= ngz.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_ring_vbar(DevGrid G, const uint8_t* __restrict__ ring_own,
+                                                   const int32_t* __restrict__ nbr, double* vbar, int wpb) {
+    const WgPos wp = wg_pos(wpb);
+    const int64_t q = wp.q;
+    const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
+    const int ng = G.ng, Rn = G.n[D] + 1, g2 = 2 * ng;
+    constexpr int T0 = D == 0 ? 1 : 0, T1 = D == 2 ? 1 : 2;
+    const int n0 = G.n[T0], n1 = G.n[T1];
+    const int nA = Rn * g2 * n1, nB = Rn * n0 * g2;
+    if (e >= nA + nB) return;
+    int a, b, c, gl, td;   // normal, t0, t1 (interior-relative), ghost layer, its direction
+    if (e < nA) {
+        const int x = e;
+        td = T0;
+        if (D == 0) { a = x % Rn; const int r = x / Rn; gl = r % g2; c = r / g2; }
+        else if (D == 1) { gl = x % g2; const int r = x / g2; a = r % Rn; c = r / Rn; }
+        else { gl = x % g2; const int r = x / g2; c = r % n1; a = r / n1; }
+        b = gl < ng ? gl - ng : n0 + gl - ng;
+    } else {
+        const int x = e - nA;
+        td = T1;
+        if (D == 0) { a = x % Rn; const int r = x / Rn; b = r % n0; gl = r / n0; }
+        else if (D == 1) { b = x % n0; const int r = x / n0; a = r % Rn; gl = r / Rn; }
+        else { b = x % n0; const int r = x / n0; gl = r % g2; a = r / g2; }
+        c = gl < ng ? gl - ng : n1 + gl - ng;
+    }
+    const int side = 2 * td + (gl >= ng);
+    if ((ring_own[q] >> side) & 1) return;
+    const int64_t qs = nbr[q * 6 + side];
+    // the neighbour's index along td: one block extent over
+    int bs = b, cs = c;
+    if (td == T0) bs = gl < ng ? b + n0 : b - n0;
+    else cs = gl < ng ? c + n1 : c - n1;
+    auto at = [&](int aa, int bb, int cc) {
+        int i, j, k;
+        if (D == 0) { i = aa; j = bb; k = cc; }
+        else if (D == 1) { i = bb; j = aa; k = cc; }
+        else { i = bb; j = cc; k = aa; }
+        return fidx<D>(G, i + (D == 0 ? 0 : ng), j + (D == 1 ? 0 : ng), k + (D == 2 ? 0 : ng));
+    };
+    const int64_t nf = G.nface[D];
+    const int64_t fo = at(a + ng, b, c), fs = at(a + ng, bs, cs);
+    const double* S = vbar + qs * 4 * nf + fs;
+    double* T = vbar + q * 4 * nf + fo;
+    const double v0 = S[0], v1 = S[nf], v2 = S[2 * nf], v3 = S[3 * nf];
+    T[0] = v0; T[nf] = v1; T[2 * nf] = v2; T[3 * nf] = v3;
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +944,7 @@ struct GatherArgs {
     double* ecen;                            // GS cell-centred E (3 vars) or null
     double* B[3];                            // face arrays (null: skip faces)
     int cell_mode;
+    bool slab_only;                          // cells: only those with one ghost coordinate
 };
 // GSRC_OWN / GSRC_AMR: forest.hh
 
@@ -1040,9 +1114,13 @@ static __device__ __align__(256) DevStats scratch[GRMHD_NOALLOC_MAX_STATES / 256
 // per block, see wg_pos), CS_SHELL (the ghost-shell cells of every block),
 // CS_LIST (explicit {block, cell} pairs).
 enum { CS_INTERIOR = 0, CS_SHELL = 1, CS_LIST = 2 };
-// LEARNED: the root solves start from the MLP's prediction (ideal gas:
-// Newton finish, as k_c2p_batch<EOS, true, true>; other EOS: brackets).
-template <typename EOS, bool FLAT, bool LEARNED = false>
+// GUESS 1 (learned): the root solves start from the MLP's prediction (ideal
+// gas: Newton finish, as k_c2p_batch<EOS, true, true>; other EOS: brackets).
+// GUESS 2 (warm start, ideal gas): the Newton finish starts from the cell's
+// previous primitives in aux, mu = 1/(h W) of the last c2p of this cell
+// (the previous RK stage, or the caller's data), at most warm_newton
+// evaluations before the plain path.
+template <typename EOS, bool FLAT, int GUESS = 0>
 __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EOS eos, Opts o, const DevInst* __restrict__ inst,
                                                    const DevBlk* __restrict__ blk, double* st, int snv,
                                                    const double* __restrict__ mstate, int mnv,
@@ -1095,7 +1173,17 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
     C2PCount cnt;
     C2POut out{false, false};
     C2PGuess gs{0, 0, 0, 0, 0, 0};
-    if constexpr (LEARNED) {
+    // The forward holds block barriers, so the block decides together: it
+    // runs only if some cell of the block reaches a root solve (D / sqrt(gamma)
+    // at or above the atmosphere threshold, the test of c2p); cells below it
+    // are reset to atmosphere without a solve, so a block of them takes the
+    // plain path and skips the network.
+    bool use_net = false;
+    if constexpr (GUESS == 1) {
+        const bool solve = active && cons[DENSL] / m.sqrtg >= o.atmo.rho_fl * (1 + o.atmo.atmo_tol);
+        use_net = __syncthreads_or(solve) != 0;
+    }
+    if (use_net) {
         // every thread of the block takes part (the forward holds barriers);
         // inactive threads run on the valid cell ee = 0 and discard the result
         __shared__ __align__(16) uint16_t xs[4][64 * MLP_KP];
@@ -1121,11 +1209,28 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
         }
         gs.s_rel = 1;
     }
+    if constexpr (GUESS == 2 && EOS::is_ideal) {
+        if (active) {
+            // x = h0 / (h W) of the cell's previous primitives; both master
+            // functions' roots lie in (0, 1] in this unit, the entropy root
+            // near the energy root
+            const double rho0 = A[RHO_ * nc], eps0 = A[EPS_ * nc], p0 = A[PRESS_ * nc];
+            const double z0[3] = {A[ZVECX_ * nc], A[ZVECY_ * nc], A[ZVECZ_ * nc]};
+            const double x = eos.hmin() / ((1.0 + eps0 + p0 / rho0) * lorentz(m.g, z0));
+            if (x > 0.0 && x <= 1.0) {
+                gs.newton = mlp.newton;
+                gs.e_c = x;
+                gs.s_c = 1.0;
+                gs.s_rel = 1;
+            }
+        }
+    }
     if (active) {
         A[BX_ * nc] = Bc0 / m.sqrtg; A[BY_ * nc] = Bc1 / m.sqrtg; A[BZ_ * nc] = Bc2 / m.sqrtg;
         C2PRoots rt{-1, -1, -1, -1};
         const double rad = o.ex.by_radius ? cell_radius(o.ex, inst[blk[q].inst], blk[q].b, G.n, G.ng, i, j, k) : 1e300;
         out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, rad, err, false, cnt, gs, rt);
+        if (out.fatal && o.fatal_count) atomicAdd(o.fatal_count, 1ull);
         A[RHO_ * nc] = pr[RHOL]; A[EPS_ * nc] = pr[EPSL]; A[PRESS_ * nc] = pr[PRESSL];
         A[TEMP_ * nc] = pr[TEMPL]; A[ENTROPY_ * nc] = pr[ENTL]; A[YE_ * nc] = pr[YEL];
         A[ZVECX_ * nc] = pr[ZXL]; A[ZVECY_ * nc] = pr[ZYL]; A[ZVECZ_ * nc] = pr[ZZL];
@@ -1212,6 +1317,7 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_batch(int64_t n, EO
     C2POut out{false, false};
     if (active) {
         out = c2p(cons, pr, m, eos, o.atmo, o.ex, o.cp, rad ? rad[s] : 1e300, err, dry_run != 0, cnt, gs, rt);
+        if (out.fatal && o.fatal_count) atomicAdd(o.fatal_count, 1ull);
         if (roots) { roots[s] = rt.x_e; roots[n + s] = rt.x_s; roots[2 * n + s] = rt.m_e; roots[3 * n + s] = rt.m_s; }
         for (int v = 0; v < NPRIM; ++v) prims[v * n + s] = pr[v];
         for (int v = 0; v < NCONS; ++v) cons_soa[v * n + s] = cons[v];
@@ -1295,6 +1401,9 @@ __global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ par
     if (threadIdx.x == 0) *dt = red[0];
 }
 
+// A host value into a device scalar, in stream order (host-dt entry points).
+__global__ void k_set_scalar(double* p, double v) { *p = v; }
+
 __global__ __launch_bounds__(256) void k_reset_diag(int64_t ncell, int64_t nq, double* aux) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ncell * nq) return;
@@ -1334,6 +1443,19 @@ struct grmhd_ctx {
     DevStats* stats_part = nullptr;
     int64_t n_part_cap = 0;
     DevStats* noalloc_part = nullptr;         // device address of `scratch` on this context's device
+    unsigned long long* fatal_d = nullptr;    // fatal c2p cells since the last sync point (Opts::fatal_count)
+    double* dt_scalar = nullptr;              // dt of the host-dt entry points
+    // ring faces copied from the neighbouring block (k_ring_vbar): per block
+    // the face neighbours (-1 beyond an outflow face) and the mask of block
+    // faces beyond which k_flux computes the ring itself; null when off
+    int32_t* ring_nbr = nullptr;
+    uint8_t* ring_own = nullptr;
+    // set by the RK3 step around the flux launches of its second and third
+    // stages, whose ghost primitives and face B the step's own gathers wrote
+    // (the first stage and grmhd_advance_substep read the caller's ghosts and
+    // compute every ring face)
+    bool ring_copy_now = false;
+    int warm_newton = 0;                      // grmhd_set_c2p_warm_start (0: off)
     int32_t* ghost_list = nullptr;
     int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
     uint8_t* ftag[3] = {nullptr, nullptr, nullptr};   // FOFC face tags (per face of every block)
@@ -1342,6 +1464,7 @@ struct grmhd_ctx {
     int4* own_gather = nullptr;               // the same with their sources
     int64_t n_own = 0;
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
+    int64_t ncg_slab = 0;                     // ghost cells with one ghost coordinate (listed first)
     int64_t last_c2p_n = 0;
     bool timing = false;
     std::vector<TimerRec> timers;
@@ -1444,7 +1567,7 @@ static void free_ctx(grmhd_ctx* c) {
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
     F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->amr_faces_mid); F(c->rflux); F(c->remf);
-    F(c->vcm);
+    F(c->vcm); F(c->fatal_d); F(c->dt_scalar); F(c->ring_nbr); F(c->ring_own);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -1596,9 +1719,42 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     if ((e = hipMalloc(&c->dtpart, sizeof(double) * c->n_dtpart)) != hipSuccess) return fail("dtpart", e);
     if ((e = hipMalloc(&c->stats, sizeof(DevStats))) != hipSuccess) return fail("stats", e);
     if ((e = hipMemset(c->stats, 0, sizeof(DevStats))) != hipSuccess) return fail("stats clear", e);
+    if ((e = hipMalloc(&c->fatal_d, sizeof(unsigned long long))) != hipSuccess) return fail("fatal counter", e);
+    if ((e = hipMemset(c->fatal_d, 0, sizeof(unsigned long long))) != hipSuccess) return fail("fatal counter clear", e);
+    c->O.fatal_count = c->fatal_d;
+    if ((e = hipMalloc(&c->dt_scalar, sizeof(double))) != hipSuccess) return fail("dt scalar", e);
     // 2-D c2p grids with workgroups down to 64 threads, plus per-launch rounding
     c->n_part_cap = nblk_bs(G.ncell * G.nq, 64) + 2 * G.nq + 2;
     if ((e = hipMalloc(&c->stats_part, sizeof(DevStats) * c->n_part_cap)) != hipSuccess) return fail("stats part", e);
+    if (!fin && p->emf == GRMHD_EMF_UCT && !p->fofc && p->flat_metric && G.n[0] >= G.ng && G.n[1] >= G.ng &&
+        G.n[2] >= G.ng) {
+        // Ring faces whose ghost side has a neighbouring block are that
+        // block's interior faces; with a flat metric every input of those
+        // faces is a copy on both sides (ghost metric values from the caller
+        // could differ in the last bit from the neighbour's, hence flat only).
+        std::vector<int32_t> nbr((size_t)g->nq * 6, -1);
+        std::vector<uint8_t> own((size_t)g->nq, 0);
+        for (int64_t q = 0; q < g->nq; ++q) {
+            const DevBlk& bq = blk[q];
+            const grmhd_instance& I = inst[bq.inst];
+            for (int sd = 0; sd < 6; ++sd) {
+                const int d = sd / 2, dir = (sd & 1) ? 1 : -1;
+                int bc[3] = {bq.b[0], bq.b[1], bq.b[2]};
+                bc[d] += dir;
+                if (bc[d] < 0 || bc[d] >= I.nb[d]) {
+                    if (I.bc[sd] != GRMHD_BC_PERIODIC) { own[q] |= (uint8_t)(1u << sd); continue; }
+                    bc[d] = (bc[d] + I.nb[d]) % I.nb[d];
+                }
+                nbr[(size_t)q * 6 + sd] = (int32_t)(I.q0 + bc[0] + (int64_t)I.nb[0] * (bc[1] + (int64_t)I.nb[1] * bc[2]));
+            }
+        }
+        if ((e = hipMalloc(&c->ring_nbr, sizeof(int32_t) * nbr.size())) != hipSuccess) return fail("ring neighbours", e);
+        if ((e = hipMemcpy(c->ring_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail("ring neighbours copy", e);
+        if ((e = hipMalloc(&c->ring_own, own.size())) != hipSuccess) return fail("ring mask", e);
+        if ((e = hipMemcpy(c->ring_own, own.data(), own.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail("ring mask copy", e);
+    }
     {
         grmhd_status sst = noalloc_scratch_ptr(&c->noalloc_part);
         if (sst != GRMHD_OK) { free_ctx(c); delete c; return sst; }
@@ -1618,7 +1774,21 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                         lst.push_back((int32_t)(i + (int64_t)E[0] * (j + (int64_t)E[1] * k)));
                         cnt++;
                     }
-            if (fd < 0) c->ncg = cnt; else c->nfg[fd] = cnt;
+            if (fd < 0) {
+                // the cells with one ghost coordinate (the six face slabs) first:
+                // between RK stages with ring copies only they are read
+                c->ncg = cnt;
+                auto nghost = [&](int32_t off) {
+                    const int loc[3] = {off % E[0], (off / E[0]) % E[1], off / (E[0] * E[1])};
+                    int ng_ = 0;
+                    for (int d = 0; d < 3; ++d) ng_ += loc[d] < G.ng || loc[d] >= G.n[d] + G.ng;
+                    return ng_;
+                };
+                auto mid = std::stable_partition(lst.begin(), lst.end(), [&](int32_t off) { return nghost(off) == 1; });
+                c->ncg_slab = mid - lst.begin();
+            } else {
+                c->nfg[fd] = cnt;
+            }
         }
         if ((e = hipMalloc(&c->ghost_list, sizeof(int32_t) * lst.size())) != hipSuccess) return fail("ghost list", e);
         if ((e = hipMemcpy(c->ghost_list, lst.data(), sizeof(int32_t) * lst.size(), hipMemcpyHostToDevice)) != hipSuccess)
@@ -1812,6 +1982,21 @@ static grmhd_status launch_check(const char* what) {
 }
 
 // ---- dispatch helpers -------------------------------------------------------
+// Ring faces of direction d copied from the neighbouring blocks, after every
+// block's k_flux of that direction (same stream).
+static void launch_ring_vbar(grmhd_ctx* c, int d, hipStream_t s) {
+    static const char* fam[3] = {"ring_x", "ring_y", "ring_z"};
+    Scope sc(c, fam[d], s);
+    const DevGrid& G = c->G;
+    const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
+    const int nring = (G.n[d] + 1) * 2 * G.ng * (G.n[t0] + G.n[t1]);
+    const int wpb = (int)nblk_bs(nring, 256);
+    const dim3 grid((unsigned)(wpb * G.nq));
+    if (d == 0) k_ring_vbar<0><<<grid, 256, 0, s>>>(G, c->ring_own, c->ring_nbr, c->vbar[0], wpb);
+    else if (d == 1) k_ring_vbar<1><<<grid, 256, 0, s>>>(G, c->ring_own, c->ring_nbr, c->vbar[1], wpb);
+    else k_ring_vbar<2><<<grid, 256, 0, s>>>(G, c->ring_own, c->ring_nbr, c->vbar[2], wpb);
+}
+
 // only != null: FOFC correction pass over the tagged faces with the options O
 // (donor cell + LLF); otherwise every face the later passes read.
 template <typename EOS, int REC, bool FLAT>
@@ -1833,9 +2018,11 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
         const int wpb = (int)nblk_bs(nf[d], bs);
         const dim3 grid((unsigned)(wpb * G.nq));
         const uint8_t* od = only ? only[d] : nullptr;
-        if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], wpb, od);
-        else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], wpb, od);
-        else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], wpb, od);
+        const uint8_t* ro = (only || !c->ring_copy_now) ? nullptr : c->ring_own;
+        if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], wpb, od, ro);
+        else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], wpb, od, ro);
+        else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], wpb, od, ro);
+        if (ro) launch_ring_vbar(c, d, s);
     }
 }
 template <typename EOS, bool FLAT>
@@ -1946,9 +2133,10 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
         if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
         return;
     }
-    const int wpb = (int)nblk_bs(c->ncg, c->tune.gather_cells);
+    const int64_t ncells = a.slab_only ? c->ncg_slab : c->ncg;
+    const int wpb = (int)nblk_bs(ncells, c->tune.gather_cells);
     k_gather_cells<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per,
-                                                                                 (int)c->ncg, wpb, a.cell_mode, ch);
+                                                                                 (int)ncells, wpb, a.cell_mode, ch);
     if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
 }
 
@@ -1977,7 +2165,9 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
     const int ni = G.n[0] * G.n[1] * G.n[2];
     const int bs = c->tune.c2p;
     int64_t npart = 0;
-    const MlpDev md = c->mlp ? c->mlp->dev() : MlpDev{};
+    const bool warm = c->warm_newton > 0 && c->P.eos_kind == GRMHD_EOS_IDEAL_GAS;
+    MlpDev md = (c->mlp && !warm) ? c->mlp->dev() : MlpDev{};
+    if (warm) md.newton = c->warm_newton;
     // the explicit-list launch may go to its own stream (no statistics then:
     // k_stats_final below reads every partial record on s)
     const hipStream_t sl = (s_list && !with_stats) ? s_list : s;
@@ -1986,12 +2176,14 @@ static void launch_c2p_cells(grmhd_ctx* c, double* st, int snv, const double* ms
         npart += (int64_t)grid.x * grid.y;
         const hipStream_t ls = cset == CS_LIST ? sl : s;
 #define C2PL(EOSV, FL, LR) k_c2p_cells<decltype(EOSV), FL, LR><<<grid, bs, 0, ls>>>(G, EOSV, c->O, c->inst_d, c->blk_d, st, snv, ms, mnv, B[0], B[1], B[2], aux, ec, part, cset, c->ghost_list, c->own_cells, ne, wpb, md)
-        if (c->mlp) {
-            if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, true); else C2PL(c->ideal, false, true); }
-            else { if (flat) C2PL(c->hybrid, true, true); else C2PL(c->hybrid, false, true); }
+        if (warm) {
+            if (flat) C2PL(c->ideal, true, 2); else C2PL(c->ideal, false, 2);
+        } else if (c->mlp) {
+            if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, 1); else C2PL(c->ideal, false, 1); }
+            else { if (flat) C2PL(c->hybrid, true, 1); else C2PL(c->hybrid, false, 1); }
         } else {
-            if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, false); else C2PL(c->ideal, false, false); }
-            else { if (flat) C2PL(c->hybrid, true, false); else C2PL(c->hybrid, false, false); }
+            if (c->P.eos_kind == GRMHD_EOS_IDEAL_GAS) { if (flat) C2PL(c->ideal, true, 0); else C2PL(c->ideal, false, 0); }
+            else { if (flat) C2PL(c->hybrid, true, 0); else C2PL(c->hybrid, false, 0); }
         }
 #undef C2PL
     };
@@ -2062,6 +2254,36 @@ extern "C" grmhd_status grmhd_advance_substep(grmhd_ctx* c, const double* dt_dev
     a.dt = dt_dev; a.dtfact = dtfact;
     launch_update(c, a, s);
     return launch_check("advance_substep");
+}
+
+extern "C" grmhd_status grmhd_advance_substep_t(grmhd_ctx* c, double t, double dt, double dtfact, double* new_state,
+                                                const double* old_state, double* const new_B[3],
+                                                const double* const old_B[3], const double* aux,
+                                                grmhd_stream_t stream) {
+    (void)t;   // the Cowling GRMHD substep does not depend on t
+    if (!c) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    if (!(dt >= 0.0) || !std::isfinite(dt)) { set_err("invalid dt %g", dt); return GRMHD_ERR_INVALID; }
+    k_set_scalar<<<1, 1, 0, (hipStream_t)stream>>>(c->dt_scalar, dt);
+    return grmhd_advance_substep(c, c->dt_scalar, dtfact, new_state, old_state, new_B, old_B, aux, stream);
+}
+
+// Reads and clears the fatal-cell counter; the stream has completed.
+static grmhd_status read_fatal(grmhd_ctx* c, hipStream_t s) {
+    unsigned long long n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, c->fatal_d, sizeof(n), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (n == 0) return GRMHD_OK;
+    HIP_TRY(hipMemsetAsync(c->fatal_d, 0, sizeof(n), s));
+    HIP_TRY(hipStreamSynchronize(s));
+    set_err("%llu cell(s) took a c2p branch where the reference aborts (rho, eps, T or p above the EOS "
+            "range outside the lenient region) since the last sync point", n);
+    return GRMHD_ERR_C2P_FATAL;
+}
+
+extern "C" grmhd_status grmhd_sync(grmhd_ctx* c, grmhd_stream_t stream) {
+    if (!c) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return read_fatal(c, (hipStream_t)stream);
 }
 
 extern "C" grmhd_status grmhd_fill_ghosts(grmhd_ctx* c, double* state, double* const B[3], grmhd_stream_t stream) {
@@ -2276,6 +2498,16 @@ extern "C" grmhd_status grmhd_set_static_metric(grmhd_ctx* c, const double* stat
     return launch_check("metric_der");
 }
 
+extern "C" grmhd_status grmhd_set_c2p_warm_start(grmhd_ctx* c, int32_t max_newton_evals) {
+    if (!c) { set_err("null context"); return GRMHD_ERR_INVALID; }
+    if (max_newton_evals < 0 || max_newton_evals > 64) { set_err("max_newton_evals %d out of [0, 64]", max_newton_evals); return GRMHD_ERR_INVALID; }
+    if (max_newton_evals > 0 && c->P.eos_kind != GRMHD_EOS_IDEAL_GAS) {
+        set_err("the warm-started Newton finish is built for the ideal-gas EOS"); return GRMHD_ERR_UNSUPPORTED;
+    }
+    if (c->warm_newton != max_newton_evals) { c->warm_newton = max_newton_evals; drop_graph(c); }
+    return GRMHD_OK;
+}
+
 extern "C" grmhd_status grmhd_set_learned_c2p(grmhd_ctx* c, grmhd_mlp* mlp) {
     if (!c) { set_err("null context"); return GRMHD_ERR_INVALID; }
     if (c->mlp != mlp) { c->mlp = mlp; drop_graph(c); }
@@ -2352,6 +2584,19 @@ extern "C" grmhd_status grmhd_compute_dt(grmhd_ctx* c, const double* state, cons
     return launch_check("compute_dt");
 }
 
+extern "C" grmhd_status grmhd_find_stable_timestep(grmhd_ctx* c, const double* state, const double* aux, double cfl,
+                                                   double* dt_out, grmhd_stream_t stream) {
+    if (!c || !dt_out) { set_err("null argument"); return GRMHD_ERR_INVALID; }
+    hipStream_t s = (hipStream_t)stream;
+    grmhd_status r = grmhd_compute_dt(c, state, aux, cfl, c->dt_scalar, stream);
+    if (r != GRMHD_OK) return r;
+    double dt = 0.0;
+    HIP_TRY(hipMemcpyAsync(&dt, c->dt_scalar, sizeof(dt), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *dt_out = dt;
+    return read_fatal(c, s);
+}
+
 static grmhd_status ensure_stage_buffers(grmhd_ctx* c) {
     if (c->U1) return GRMHD_OK;
     const DevGrid& G = c->G;
@@ -2414,12 +2659,16 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
             fork(0, s, c->side[1]);
             if (sidx == 0) k_reset_diag<<<nblk(G.ncell * G.nq), 256, 0, c->side[1]>>>(G.ncell, G.nq, aux);
             c->flux_s[1] = c->side[0]; c->flux_s[2] = c->side[1];
+            c->ring_copy_now = sidx > 0;
             launch_fluxes(c, aux, state, NSTATE, S.oB, s);
+            c->ring_copy_now = false;
             c->flux_s[1] = c->flux_s[2] = nullptr;
             fork(1, c->side[0], s);
             fork(2, c->side[1], s);
         } else {
+            c->ring_copy_now = sidx > 0;
             launch_fluxes(c, aux, state, NSTATE, S.oB, s);
+            c->ring_copy_now = false;
         }
         // the cell part of the update needs only the fluxes (on a forest
         // after their reflux): it runs on side[0] beside the edge EMFs, the
@@ -2490,6 +2739,11 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
             g.st = (last || c->P.fofc) ? S.nst : nullptr; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC;
             g.ecen = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
             g.cell_mode = 3;
+            // the next stage copies the ring faces' values from the
+            // neighbouring blocks: its fluxes read ghost primitives only in
+            // the face slabs (and in ghosts beyond outflow faces, which run
+            // their own c2p)
+            g.slab_only = !last && c->ring_own != nullptr;
             launch_gather(c, g, s);
         }
     }

@@ -82,6 +82,22 @@ public:
               "advance_substep");
     }
 
+    // The reference's signature advance_substep(t, dt, dtfact, ...) with a
+    // host dt (t unused by the Cowling GRMHD substep).
+    void advance_substep(double t, double dt, double dtfact, var_array_t new_state, var_array_t old_state,
+                         staggered_variable_arrays_t new_stag, staggered_variable_arrays_t old_stag, var_array_t aux,
+                         hipStream_t stream = nullptr) {
+        const double* ob[3] = {old_stag.face_staggered_fields_x, old_stag.face_staggered_fields_y,
+                               old_stag.face_staggered_fields_z};
+        check(grmhd_advance_substep_t(ctx_, t, dt, dtfact, new_state.data, old_state.data, new_stag.faces(), ob,
+                                      aux.data, (grmhd_stream_t)stream),
+              "advance_substep");
+    }
+
+    // Waits for the stream; throws (status GRMHD_ERR_C2P_FATAL) if a c2p
+    // pass met a cell where the reference aborts since the last sync point.
+    void sync(hipStream_t stream = nullptr) { check(grmhd_sync(ctx_, (grmhd_stream_t)stream), "sync"); }
+
     // c2p over every cell (ghosts included), cell-centred B from the faces.
     void compute_auxiliary_quantities(var_array_t state, staggered_variable_arrays_t stag, var_array_t aux,
                                       hipStream_t stream = nullptr) {
@@ -100,6 +116,15 @@ public:
     void find_stable_timestep(var_array_t state, var_array_t aux, double cfl, double* dt_dev,
                               hipStream_t stream = nullptr) {
         check(grmhd_compute_dt(ctx_, state.data, aux.data, cfl, dt_dev, (grmhd_stream_t)stream), "find_stable_timestep");
+    }
+
+    // The reference's find_stable_timestep(): dt returned on the host (a
+    // sync point, as sync()).
+    double find_stable_timestep(var_array_t state, var_array_t aux, double cfl, hipStream_t stream = nullptr) {
+        double dt = 0.0;
+        check(grmhd_find_stable_timestep(ctx_, state.data, aux.data, cfl, &dt, (grmhd_stream_t)stream),
+              "find_stable_timestep");
+        return dt;
     }
 
     // One SSP-RK3 step in place; graph = true replays a captured hipGraph

@@ -190,6 +190,25 @@ grmhd_status grmhd_advance_substep(grmhd_ctx* ctx, const double* dt_dev, double 
                                    double* const new_B[3], const double* const old_B[3],
                                    const double* aux, grmhd_stream_t stream);
 
+/* advance_substep<eos_t>(t, dt, dtfact, ...) with the reference's host
+ * arguments (include/grace/evolution/evolve.hh:252-256): dt is a host value,
+ * placed in a context scalar in stream order ahead of the substep; t is
+ * accepted for the signature (the Cowling GRMHD substep does not depend on
+ * it).  Otherwise as grmhd_advance_substep. */
+grmhd_status grmhd_advance_substep_t(grmhd_ctx* ctx, double t, double dt, double dtfact,
+                                     double* new_state, const double* old_state,
+                                     double* const new_B[3], const double* const old_B[3],
+                                     const double* aux, grmhd_stream_t stream);
+
+/* Sync point: waits for the stream, then reports GRMHD_ERR_C2P_FATAL if any
+ * cell of a c2p pass (grmhd_compute_auxiliaries, the RK3 step, the c2p
+ * batches) took a branch where the reference calls Kokkos::abort
+ * (c2p.hh:160-322: rho, eps, T or p above the EOS range outside the lenient
+ * region) since the last sync point; the count is cleared.  The kernels
+ * still write every cell (the caller decides), so an asynchronous step never
+ * stops the stream. */
+grmhd_status grmhd_sync(grmhd_ctx* ctx, grmhd_stream_t stream);
+
 /* amr::apply_boundary_conditions (src/amr/boundary_conditions.cpp:55-81):
  * same-level ghost copy between blocks + outflow/periodic physical BC, for the
  * 7 hydro conserved variables and the 3 face-B arrays. */
@@ -264,6 +283,12 @@ grmhd_status grmhd_c2p_stats(grmhd_ctx* ctx, grmhd_c2p_stats_t* out, grmhd_strea
  * dt = min over interior cells of cfl*dx/c_max, written to dt_dev. */
 grmhd_status grmhd_compute_dt(grmhd_ctx* ctx, const double* state, const double* aux, double cfl,
                               double* dt_dev, grmhd_stream_t stream);
+
+/* find_stable_timestep (src/evolution/find_stable_timestep.cpp:72-126) with
+ * the reference's host result: dt (the reference's set_timestep value) into
+ * *dt_out after the stream completes.  A sync point, as grmhd_sync. */
+grmhd_status grmhd_find_stable_timestep(grmhd_ctx* ctx, const double* state, const double* aux, double cfl,
+                                        double* dt_out, grmhd_stream_t stream);
 
 /* One full SSP-RK3 step (evolve_impl, src/evolution/evolve.cpp:147-225):
  * reset of the per-step c2p diagnostics, three substeps each followed by the

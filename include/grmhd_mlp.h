@@ -142,6 +142,16 @@ grmhd_status grmhd_mlp_fit_cells(grmhd_ctx* ctx, grmhd_mlp* mlp, const double* s
  * graph is re-captured. */
 grmhd_status grmhd_set_learned_c2p(grmhd_ctx* ctx, grmhd_mlp* mlp);
 
+/* Makes the c2p of the cell passes start the ideal-gas Newton finish from
+ * each cell's previous primitives in aux (mu = 1/(h W) of the last c2p of
+ * that cell: the previous RK stage inside grmhd_rk3_step, the caller's data
+ * otherwise), at most max_newton_evals master-function evaluations before
+ * the plain solve (hook: the initial guess of kastaun_c2p.hh:355-372; the
+ * reference has no counterpart).  Takes precedence over an attached network
+ * in the cell passes; 0 turns it off.  Ideal gas only (GRMHD_ERR_UNSUPPORTED
+ * otherwise); a captured RK3 graph is re-captured. */
+grmhd_status grmhd_set_c2p_warm_start(grmhd_ctx* ctx, int32_t max_newton_evals);
+
 /* grmhd_c2p_batch with the learned guess: features, MLP and the FP64
  * Kastaun solves (Newton finish or brackets, grmhd_mlp_set_newton) in one
  * fused kernel.  Same outputs and statistics as grmhd_c2p_batch (no dry

@@ -1,9 +1,11 @@
 """GPU tests of the boundary's less-travelled entry points and options,
 against the oracle: prims_to_conservs over a batch (c2p.cpp:416-455), LLF
 with the light-cone floor (grmhd.hh:874-883, rusanov_c_limit > 0),
-radius-based excision (c2p.cpp:345-348) in the RK3 cell passes, and the
+radius-based excision (c2p.cpp:345-348) in the RK3 cell passes, the
 option combinations the library rejects instead of computing a wrong
-answer.  Tolerances as in test_gpu_parity."""
+answer, the host-dt entry points (advance_substep(t, dt, dtfact, ...),
+find_stable_timestep) and the fatal-c2p report at sync points.  Tolerances
+as in test_gpu_parity."""
 import ctypes as C
 
 import numpy as np
@@ -123,3 +125,74 @@ def test_gpu_rejected_option_combinations(dev):
     st, h = _create_status(p, grid, inst, 1)
     assert st == 0
     ga.lib().grmhd_ctx_destroy(h)
+
+
+def test_gpu_host_dt_entry_points(dev):
+    """advance_substep(t, dt, dtfact, ...) with a host dt (evolve.hh:252-256)
+    gives the bits of the device-dt substep; find_stable_timestep returns the
+    device dt on the host (find_stable_timestep.cpp:72-126), equal to the
+    oracle's to 1e-14."""
+    import torch
+    wl = W.blast(nb=(2, 2, 2), n=16)
+    say("[host dt] substep with a host dt, find_stable_timestep")
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    sg, Bg, ag = dev_arrays(wl, dev)
+    dt = ctx.find_stable_timestep(sg, ag, 0.25)
+    dtd = torch.zeros(1, dtype=torch.float64, device=dev)
+    ctx.compute_dt(sg, ag, 0.25, dtd)
+    torch.cuda.synchronize()
+    assert dt == float(dtd.item())
+    dto = O.Stepper(wl).dt(wl.state, wl.aux, 0.25)
+    assert abs(dt - dto) <= 1e-14 * dto
+    outs = []
+    for host in (False, True):
+        ng, nBg = sg.clone(), [b.clone() for b in Bg]
+        if host:
+            ctx.advance_substep_t(0.125, dt, 0.5, ng, sg, nBg, Bg, ag)
+        else:
+            ctx.advance_substep(dtd, 0.5, ng, sg, nBg, Bg, ag)
+        assert ctx.sync() == 0
+        outs.append((ng, nBg))
+    assert torch.equal(outs[0][0], outs[1][0])
+    for d in range(3):
+        assert torch.equal(outs[0][1][d], outs[1][1][d])
+    assert not torch.equal(outs[0][0], sg)
+
+
+def test_gpu_c2p_fatal_reported_at_sync(dev):
+    """A cell whose density exceeds the EOS range (where the reference's c2p
+    calls Kokkos::abort, c2p.hh:160-322) is counted by the c2p passes and
+    reported as GRMHD_ERR_C2P_FATAL (5) at the next sync point, once; clean
+    data report 0.  The oracle counts the same cell as fatal."""
+    import torch
+    wl = W.blast(nb=(2, 2, 2), n=16)
+    say("[fatal] one cell above the EOS density range")
+    st, B, aux = wl.copy_arrays()
+    g = wl.grid.ngz
+    st[3, ga.DENS, g + 5, g + 6, g + 7] = 1e101
+    st[3, ga.TAU, g + 5, g + 6, g + 7] = 1e100
+    for v in (ga.SX, ga.SY, ga.SZ):
+        st[3, v, g + 5, g + 6, g + 7] = 0.0
+    o = O.Stepper(wl)
+    s_o, B_o, a_o = st.copy(), [b.copy() for b in B], aux.copy()
+    o.aux(s_o, B_o, a_o)
+    assert o.fatal() == 1
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    sg, Bg, ag = to_dev(wl.state, dev), [to_dev(b, dev) for b in wl.B], to_dev(wl.aux, dev)
+    ctx.compute_auxiliaries(sg, Bg, ag)
+    assert ctx.sync() == 0
+    sb = to_dev(st, dev)
+    ctx.compute_auxiliaries(sb, Bg, ag)
+    assert ctx.sync() == 5
+    assert b"reference aborts" in ga.lib().grmhd_last_error()
+    assert ctx.sync() == 0          # cleared by the sync point that reported it
+    # an RK3 step over the same data counts it in its c2p passes too
+    dtd = torch.tensor([0.25 * wl.extra["dx"]], dtype=torch.float64, device=dev)
+    ctx.rk3_step(dtd, sb, Bg, ag)
+    assert ctx.sync() == 5
+    # find_stable_timestep is a sync point too (on a fresh copy of the bad
+    # state: the step's c2p has limited that cell's density)
+    sc = to_dev(st, dev)
+    ctx.compute_auxiliaries(sc, Bg, ag)
+    with pytest.raises(RuntimeError, match="status 5"):
+        ctx.find_stable_timestep(sc, ag, 0.25)

@@ -238,6 +238,58 @@ def test_gpu_learned_c2p_in_rk3_torus(dev):
     assert stl["n_guess_used"] > 0.5 * stl["n_cells"]
 
 
+@pytest.mark.parametrize("wl_name", ["torus", "blast"])
+def test_gpu_warm_start_c2p_in_rk3(dev, wl_name):
+    """The cell passes started from each cell's previous primitives
+    (grmhd_set_c2p_warm_start): two RK3 steps (graph replay, so stages 2-3
+    and the second step start from the library's own c2p results) agree
+    with the plain-c2p steps to the Newton finish's root tolerance, with
+    identical error bits, the guess used in most cells that solve, and fewer
+    master-function evaluations than plain."""
+    import torch
+    from test_gpu_parity import linf_rel, interior, face_interior
+    wl = W.fm_torus(nb=(2, 2, 2), n=16, extent=24.0) if wl_name == "torus" else W.blast(nb=(2, 2, 2), n=16)
+    say(f"[warm rk3] {wl_name}")
+    dt = torch.tensor([0.25 * wl.extra["dx"]], dtype=torch.float64, device=dev)
+    out = {}
+    for mode in ("plain", "warm"):
+        ctx = ga.Context(wl.params, wl.grid, wl.instances)
+        if mode == "warm":
+            ctx.set_c2p_warm_start(8)
+        st = to_dev(wl.state, dev)
+        B = [to_dev(b, dev) for b in wl.B]
+        aux = to_dev(wl.aux, dev)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            for _ in range(2):
+                ctx.rk3_step(dt, st, B, aux, stream=s.cuda_stream, graph=True)
+        s.synchronize()
+        res = (to_host(st), [to_host(b) for b in B], to_host(aux))
+        ctx.compute_auxiliaries(st, B, aux)      # a counted c2p pass over every cell
+        torch.cuda.synchronize()
+        out[mode] = res + (ctx.c2p_stats(),)
+        say(f"  {mode}: {out[mode][3]}")
+    (sp, Bp, ap, stp), (sw, Bw, aw, stw) = out["plain"], out["warm"]
+    for v in range(7):
+        e = linf_rel(interior(wl.grid, sw[:, v]), interior(wl.grid, sp[:, v]))
+        assert e < 1e-9, (v, e)
+    for d in range(3):
+        assert linf_rel(face_interior(wl.grid, Bw[d], d), face_interior(wl.grid, Bp[d], d)) < 1e-9, d
+    assert np.array_equal(aw[:, ga.C2P_ERR], ap[:, ga.C2P_ERR])
+    solving = stw["n_cells"] - stw["n_atmo_reset"]
+    assert stw["n_guess_used"] > 0.5 * solving
+    assert stw["n_root_evals"] + stw["n_bracket_evals"] < stp["n_root_evals"] + stp["n_bracket_evals"]
+
+
+def test_gpu_warm_start_rejected_for_hybrid_eos(dev):
+    from test_oracle_c2p import hybrid_params
+    ctx = ga.Context(hybrid_params(), ga.Grid(1, 1, 1, 3, 1),
+                     ga.make_instances([dict(nb=(1, 1, 1), q0=0, bc=(0,) * 6, idx=(1.0, 1.0, 1.0))]))
+    with pytest.raises(RuntimeError, match="status 4"):
+        ctx.set_c2p_warm_start(8)
+    ctx.set_c2p_warm_start(0)
+
+
 @pytest.mark.parametrize("case", ["sigma", "cold", "dense", "near_hole"])
 def test_gpu_learned_c2p_out_of_distribution(dev, case):
     """A network trained on the C4 distribution, run on states outside it

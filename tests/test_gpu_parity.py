@@ -64,6 +64,7 @@ def linf_rel(a, b, mask=None):
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
 
 
+ENT_ATM_TOL = 1e-11  # ENTROPYSTAR over atmosphere cells (see check_cons)
 CELL_FLOOR = 1e-8   # per-cell relative error floor, as a fraction of max |cpu|
 CELL_TOL = 1e-7     # bound of that per-cell error after one substep / RK3 step
 
@@ -94,7 +95,10 @@ def check_cons(wl, got, ref, aux_ref, tol, atm_tol=None, cell_tol=CELL_TOL, what
         e = (linf_rel(a, b), linf_rel(a, b, ~atm), linf_rel(a, b, atm), cell_rel(a, b))
         worst = [max(x, y) for x, y in zip(worst, e)]
         assert e[0] < tol and e[1] < tol, (what, v, e)
-        assert e[2] < atm_tol, (what, "atmosphere", v, e)
+        # the atmosphere's entropy is a difference of logarithms (GPU log
+        # against glibc log, each within an ulp) of nearly equal size, so
+        # it carries a relative error ~10x that of the other conserved there
+        assert e[2] < (max(atm_tol, ENT_ATM_TOL) if v == 6 else atm_tol), (what, "atmosphere", v, e)
         assert e[3] < cell_tol, (what, "per cell", v, e)
     say(f"  {what} conserved: all {worst[0]:.2e} body {worst[1]:.2e} atmosphere {worst[2]:.2e} "
         f"({int(atm.sum())} cells) per-cell {worst[3]:.2e}")
@@ -345,6 +349,25 @@ def test_gpu_rk3_step_blast(dev):
     # div B of the evolved field stays at round-off
     bdiv = interior(wl.grid, got[2][:, ga.BDIV])
     assert np.max(np.abs(bdiv)) < 1e-12
+
+
+@pytest.mark.parametrize("nb", [(2, 2, 2), (1, 1, 2)])
+def test_gpu_rk3_periodic(dev, nb):
+    """Periodic faces on every side (with one block along x and y, a block
+    is its own periodic neighbour): two RK3 steps against the oracle.  From
+    the second stage on, the UCT face values of a block's transverse ghost
+    ring are copied from the neighbouring block's interior faces (the
+    periodic image's across the domain faces) instead of recomputed."""
+    wl = W.blast(nb=nb, n=16)
+    spec = dict(wl.inst_specs[0])
+    spec["bc"] = (ga.BC_PERIODIC,) * 6
+    wl.inst_specs = [spec]
+    wl.instances = ga.make_instances(wl.inst_specs)
+    wl.extra["dx"] = 1.0 / max(wl.extra["N"])
+    say(f"[rk3 periodic] blast {nb} blocks of 16^3, two steps")
+    ref, got, _ = run_rk3_pair(wl, dev, steps=2)
+    check_close(wl, ref, got, 1e-12)
+    assert np.array_equal(got[2][:, ga.C2P_ERR], ref[2][:, ga.C2P_ERR])
 
 
 def test_gpu_rk3_graph_matches_eager(dev):
