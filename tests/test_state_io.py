@@ -31,3 +31,118 @@ def test_state_dump_errors(tmp_path):
     bad.write_bytes(b"not an hdf5 file")
     with pytest.raises(RuntimeError):
         ga.read_state(str(bad))
+
+
+def _h5():
+    """ctypes view of the HDF5 C library the dump library links against."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(os.environ.get("GRMHD_HDF5_PREFIX", "/opt/conda"), "lib", "libhdf5.so"))
+    hid = C.c_int64
+    L.H5open.restype = C.c_int
+    L.H5Fopen.argtypes = [C.c_char_p, C.c_uint, hid]
+    L.H5Fopen.restype = hid
+    L.H5Fclose.argtypes = [hid]
+    L.H5Aopen.argtypes = [hid, C.c_char_p, hid]
+    L.H5Aopen.restype = hid
+    L.H5Aget_type.argtypes = [hid]
+    L.H5Aget_type.restype = hid
+    L.H5Aget_space.argtypes = [hid]
+    L.H5Aget_space.restype = hid
+    L.H5Aclose.argtypes = [hid]
+    L.H5Dopen2.argtypes = [hid, C.c_char_p, hid]
+    L.H5Dopen2.restype = hid
+    L.H5Dget_type.argtypes = [hid]
+    L.H5Dget_type.restype = hid
+    L.H5Dget_space.argtypes = [hid]
+    L.H5Dget_space.restype = hid
+    L.H5Dclose.argtypes = [hid]
+    L.H5Tget_class.argtypes = [hid]
+    L.H5Tget_class.restype = C.c_int
+    L.H5Tget_size.argtypes = [hid]
+    L.H5Tget_size.restype = C.c_size_t
+    L.H5Tget_sign.argtypes = [hid]
+    L.H5Tget_sign.restype = C.c_int
+    L.H5Tclose.argtypes = [hid]
+    L.H5Sget_simple_extent_ndims.argtypes = [hid]
+    L.H5Sget_simple_extent_ndims.restype = C.c_int
+    L.H5Sget_simple_extent_dims.argtypes = [hid, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.H5Sget_simple_extent_dims.restype = C.c_int
+    L.H5Sclose.argtypes = [hid]
+    L.H5open()
+    return L
+
+
+def test_state_dump_layout_matches_reference_checkpoint(tmp_path):
+    """The names and types the reference's checkpoint writer uses
+    (checkpoint_handler.cpp:438-454: attributes "Time" H5T_NATIVE_DOUBLE and
+    "Iteration" H5T_NATIVE_UINT; :490-498 datasets grid_extent_x|y|z;
+    :625-634 the 1-D double datasets CellCenteredData and
+    FaceCenteredDataX|Y|Z), read back through the HDF5 C API."""
+    import ctypes as C
+    wl = W.blast(nb=(1, 2, 1), n=8)
+    path = str(tmp_path / "checkpoint_data_000007.h5")
+    ga.write_state(path, wl.grid, wl.state, wl.B, time=0.5, iteration=7, extents=(0, 1, 0, 2, 0, 1))
+    H = _h5()
+    H5T_INTEGER, H5T_FLOAT, H5T_SGN_NONE = 0, 1, 0
+    f = H.H5Fopen(path.encode(), 0, 0)
+    assert f >= 0
+
+    def attr(name):
+        a = H.H5Aopen(f, name.encode(), 0)
+        assert a >= 0, name
+        t, sp = H.H5Aget_type(a), H.H5Aget_space(a)
+        out = (H.H5Tget_class(t), H.H5Tget_size(t), H.H5Tget_sign(t), H.H5Sget_simple_extent_ndims(sp))
+        H.H5Tclose(t); H.H5Sclose(sp); H.H5Aclose(a)
+        return out
+
+    def dset(name):
+        d = H.H5Dopen2(f, name.encode(), 0)
+        assert d >= 0, name
+        t, sp = H.H5Dget_type(d), H.H5Dget_space(d)
+        nd = H.H5Sget_simple_extent_ndims(sp)
+        dims = (C.c_uint64 * max(nd, 1))()
+        H.H5Sget_simple_extent_dims(sp, dims, None)
+        out = (H.H5Tget_class(t), H.H5Tget_size(t), nd, tuple(dims[:nd]))
+        H.H5Tclose(t); H.H5Sclose(sp); H.H5Dclose(d)
+        return out
+
+    assert attr("Time") == (H5T_FLOAT, 8, -1, 0)
+    cls, size, sign, nd = attr("Iteration")
+    assert (cls, size, sign, nd) == (H5T_INTEGER, 4, H5T_SGN_NONE, 0)
+    for ax in "xyz":
+        assert dset(f"grid_extent_{ax}") == (H5T_FLOAT, 8, 1, (2,))
+    g = wl.grid
+    N = [g.nx + 2 * g.ngz, g.ny + 2 * g.ngz, g.nz + 2 * g.ngz]
+    assert dset("CellCenteredData") == (H5T_FLOAT, 8, 1, (ga.NSTATE * N[0] * N[1] * N[2] * g.nq,))
+    for d, name in enumerate(("FaceCenteredDataX", "FaceCenteredDataY", "FaceCenteredDataZ")):
+        n = g.nq
+        for e in range(3):
+            n *= N[e] + (e == d)
+        assert dset(name) == (H5T_FLOAT, 8, 1, (n,))
+    H.H5Fclose(f)
+
+
+@pytest.mark.gpu
+def test_gpu_state_dump_of_an_evolved_step(tmp_path, dev):
+    """A state evolved one RK3 step on the GPU, dumped and read back, is the
+    GPU state bit for bit, and agrees with the oracle's step to 1e-12."""
+    import torch
+    import _oracle as O
+    from test_gpu_parity import check_cons
+    wl = W.blast(nb=(2, 2, 1), n=16)
+    dt = 0.25 * wl.extra["dx"]
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    sg = torch.from_numpy(wl.state.copy()).to(dev)
+    Bg = [torch.from_numpy(b.copy()).to(dev) for b in wl.B]
+    ag = torch.from_numpy(wl.aux.copy()).to(dev)
+    ctx.rk3_step(torch.tensor([dt], dtype=torch.float64, device=dev), sg, Bg, ag)
+    torch.cuda.synchronize()
+    gs, gB = sg.cpu().numpy(), [b.cpu().numpy() for b in Bg]
+    path = str(tmp_path / "checkpoint_data_000001.h5")
+    ga.write_state(path, wl.grid, gs, gB, time=dt, iteration=1, extents=(-0.5, 0.5, -0.5, 0.5, -0.5, 0.5))
+    g, st, B, t, it, e = ga.read_state(path)
+    assert t == dt and it == 1
+    assert np.array_equal(st, gs) and all(np.array_equal(B[d], gB[d]) for d in range(3))
+    so, Bo, ao = wl.copy_arrays()
+    O.Stepper(wl).rk3(dt, so, Bo, ao)
+    check_cons(wl, st, so, ao, 1e-12, what="dump")
