@@ -58,6 +58,9 @@ KERNEL_FAMILIES = {"flux": ("flux_x", "flux_y", "flux_z")}
 # Whole-step algorithmic bytes per zone-update (flat metric; RHS 3x(38..48
 # reads + 10 writes) + c2p 3x) -- 1552 B.
 STEP_ALG_BYTES = 1552
+# The same with the metric arrays read (curved metric: + 16 metric doubles per
+# stage in the RHS and 10 in the c2p, x 3 stages) -- 2176 B.
+STEP_ALG_BYTES_CURVED = 2176
 RIDGE_FLOP_PER_BYTE = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
 # Algorithmic FP64 operations, counted by running the CPU oracle with an
 # operation-counting scalar on this 128^3 workload (oracle/count_flops.py;
@@ -368,6 +371,58 @@ def c3_leg(ga, W, torch, dev, steps=3):
             "ms_per_step_default_shapes": 1e3 * t_default, "tuning": tune}
 
 
+def curved_leg(ga, W, torch, dev, steps=10, warmup=3, tune=True):
+    """The 128^3 blast wave in the Kerr-Schild metric of a spin-0.9 hole
+    (a cube of side 4 M centred at x = 8 M, 512 blocks of 16^3): the same
+    step as the headline workload with the curved-metric kernels (face metric,
+    geometric sources from the static-metric cache, metric read by the c2p),
+    graph-replayed with the device CFL dt."""
+    log("[bench] curved-metric 128^3 blast")
+    wl = W.blast_ks(nb=(8, 8, 8), n=16)
+    ctx = ga.Context(wl.params, wl.grid, wl.instances)
+    st = torch.from_numpy(wl.state).to(dev)
+    B = [torch.from_numpy(b).to(dev) for b in wl.B]
+    aux = torch.from_numpy(wl.aux).to(dev)
+    ctx.set_static_metric(st)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    tuning = None
+    with torch.cuda.stream(stream):
+        if tune:
+            tuning = ctx.autotune(st, B, aux, reps=2, stream=sp)
+        for _ in range(warmup):
+            ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
+            ctx.rk3_step(dt, st, B, aux, stream=sp, graph=True)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.compute_dt(st, aux, 0.25, dt, stream=sp)
+            ctx.rk3_step(dt, st, B, aux, stream=sp, graph=True)
+        stream.synchronize()
+        wall = time.perf_counter() - t0
+        ctx.set_timing(1)
+        ctx.rk3_step(dt, st, B, aux, stream=sp, graph=False)
+        stream.synchronize()
+    fam = {}
+    for f in ("flux_x", "flux_y", "flux_z", "ring_x", "ring_y", "ring_z", "emf", "update", "ghost", "ghost_faces",
+              "c2p"):
+        ms, n = ctx.kernel_time_ms(f)
+        if n:
+            fam[f] = round(ms, 4)
+    ctx.set_timing(0)
+    ms_step = 1e3 * wall / steps
+    return {"workload": "blast wave 128^3 (512 blocks of 16^3, ngz 3) in the Kerr-Schild metric, spin 0.9, cube of "
+                        "side 4 M at x = 8 M, WENO-Z + HLLE + UCT, RK3 graph, device CFL dt, static-metric cache",
+            "zones": wl.zones, "steps": steps, "ms_per_step": round(ms_step, 4),
+            "zone_updates_per_s": wl.zones / (ms_step * 1e-3), "flat_metric": 0,
+            "alg_bytes_per_zone_update": STEP_ALG_BYTES_CURVED, "alg_flops_per_zone_update": FALG_STEP_FLOPS_REFERENCE,
+            "hbm_frac": round(STEP_ALG_BYTES_CURVED * wl.zones / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "fp64_frac": round(FALG_STEP_FLOPS_REFERENCE * wl.zones / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5),
+            "kernel_ms_per_step": fam, "tuning": tuning,
+            "finite": bool(torch.isfinite(st).all().item() and torch.isfinite(aux).all().item())}
+
+
 def torus_leg(ga, W, torch, dev, steps=5, warmup=2, tune=True):
     """Fishbone-Moncrief torus (r_in 6 M, r_max 12 M, Gamma 4/3, beta 100
     loop) around a spin-0.9375 hole in the Kerr-Schild metric on a two-level
@@ -517,6 +572,7 @@ def main():
     ap.add_argument("--no-c2p", action="store_true", help="skip the learned-vs-plain c2p leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven-instance leg")
     ap.add_argument("--no-torus", action="store_true", help="skip the Kerr-Schild torus leg")
+    ap.add_argument("--no-curved", action="store_true", help="skip the curved-metric 128^3 blast leg")
     ap.add_argument("--no-tune", action="store_true", help="default workgroup sizes instead of grmhd_autotune")
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
@@ -678,6 +734,7 @@ def main():
     c1 = None if args.no_c1 else c1_leg(ga, W, torch, dev)
     c3 = None if args.no_c3 else c3_leg(ga, W, torch, dev)
     torus = None if args.no_torus else torus_leg(ga, W, torch, dev, tune=not args.no_tune)
+    curved = None if args.no_curved else curved_leg(ga, W, torch, dev, tune=not args.no_tune)
 
     out = {
         "metric": "GRMHD zone-updates/s on 1 MI355X, learned vs plain c2p at <=1e-10 c2p err",
@@ -703,6 +760,7 @@ def main():
         "balsara1_full_run": c1,
         "batched_uneven": c3,
         "fm_torus": torus,
+        "curved_blast128": curved,
     }
     print(json.dumps(out), flush=True)
 

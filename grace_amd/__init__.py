@@ -117,11 +117,11 @@ MLP_NSCALE = 24
 
 class Tuning(C.Structure):
     _fields_ = [("flux", C.c_int32 * 3), ("emf", C.c_int32), ("update", C.c_int32), ("gather_faces", C.c_int32),
-                ("gather_cells", C.c_int32), ("c2p", C.c_int32)]
+                ("gather_cells", C.c_int32), ("c2p", C.c_int32), ("flux_prefetch", C.c_int32 * 3)]
 
     def as_dict(self):
         return {"flux": list(self.flux), "emf": self.emf, "update": self.update, "gather_faces": self.gather_faces,
-                "gather_cells": self.gather_cells, "c2p": self.c2p}
+                "gather_cells": self.gather_cells, "c2p": self.c2p, "flux_prefetch": list(self.flux_prefetch)}
 
 
 class MlpFitOpts(C.Structure):
@@ -384,10 +384,12 @@ class Context:
         check(self.L.grmhd_get_tuning(self.h, C.byref(t)), "get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, flux=(256, 256, 256), emf=256, update=256, gather_faces=256, gather_cells=256, c2p=256):
+    def set_tuning(self, flux=(256, 256, 256), emf=256, update=256, gather_faces=256, gather_cells=256, c2p=256,
+                   flux_prefetch=(0, 0, 0)):
         t = Tuning()
         for d in range(3):
             t.flux[d] = flux[d]
+            t.flux_prefetch[d] = flux_prefetch[d]
         t.emf, t.update, t.gather_faces, t.gather_cells, t.c2p = emf, update, gather_faces, gather_cells, c2p
         check(self.L.grmhd_set_tuning(self.h, C.byref(t)), "set_tuning")
 

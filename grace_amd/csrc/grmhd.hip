@@ -133,7 +133,11 @@ GHD void load_metric(Met& m, const double* __restrict__ ms, int64_t ncell, int64
 #else
 #define GRMHD_FLUX_BOUNDS __launch_bounds__(256)
 #endif
-template <typename EOS, int REC, bool FLAT, int D>
+// PF: every stencil value of the face is loaded before the first
+// reconstruction (54 loads in flight per lane instead of 6; about 150 VGPRs,
+// 3 waves per SIMD instead of 4): a tuning choice per direction
+// (grmhd_tuning.flux_prefetch), results bit-identical.
+template <typename EOS, int REC, bool FLAT, int D, bool PF = false>
 __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const double* __restrict__ aux,
                                               const double* __restrict__ mstate, int mnv,
                                               const double* __restrict__ Bd, double* __restrict__ flux,
@@ -241,17 +245,36 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     const int ridx[10] = {RHO_, ZVECX_, ZVECY_, ZVECZ_, YE_, th_aux, ENTROPY_, BX_, BY_, BZ_};
     const int rloc[10] = {RHOL, ZXL, ZYL, ZZL, YEL, o.thermo ? PRESSL : TEMPL, ENTL, BXL, BYL, BZL};
     double pL[NPRIM], pR[NPRIM];
+    if constexpr (PF) {
+        double us[10][6];
 #pragma unroll
-    for (int v = 0; v < 10; ++v) {
-        // Work whose result is never read: the normal B component is replaced
-        // by the face value below, and a ring face (UCT velocities and wave
-        // speeds only) reads neither Ye nor the entropy.
-        if (v == 7 + D || (ring && (v == 4 || v == 6))) { pL[rloc[v]] = 0.0; pR[rloc[v]] = 0.0; continue; }
-        const double* P = A + ridx[v] * nc + c0;
-        double uL, uR;
-        if (excise_override) { uL = P[-st]; uR = P[0]; }
-        else recon<REC>(P[-3 * st], P[-2 * st], P[-st], P[0], P[st], P[2 * st], uL, uR);
-        pL[rloc[v]] = uL; pR[rloc[v]] = uR;
+        for (int v = 0; v < 10; ++v) {
+            if (v == 7 + D) continue;
+            const double* P = A + ridx[v] * nc + c0;
+#pragma unroll
+            for (int t = 0; t < 6; ++t) us[v][t] = P[(t - 3) * st];
+        }
+#pragma unroll
+        for (int v = 0; v < 10; ++v) {
+            if (v == 7 + D || (ring && (v == 4 || v == 6))) { pL[rloc[v]] = 0.0; pR[rloc[v]] = 0.0; continue; }
+            double uL, uR;
+            if (excise_override) { uL = us[v][2]; uR = us[v][3]; }
+            else recon<REC>(us[v][0], us[v][1], us[v][2], us[v][3], us[v][4], us[v][5], uL, uR);
+            pL[rloc[v]] = uL; pR[rloc[v]] = uR;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 10; ++v) {
+            // Work whose result is never read: the normal B component is replaced
+            // by the face value below, and a ring face (UCT velocities and wave
+            // speeds only) reads neither Ye nor the entropy.
+            if (v == 7 + D || (ring && (v == 4 || v == 6))) { pL[rloc[v]] = 0.0; pR[rloc[v]] = 0.0; continue; }
+            const double* P = A + ridx[v] * nc + c0;
+            double uL, uR;
+            if (excise_override) { uL = P[-st]; uR = P[0]; }
+            else recon<REC>(P[-3 * st], P[-2 * st], P[-st], P[0], P[st], P[2 * st], uL, uR);
+            pL[rloc[v]] = uL; pR[rloc[v]] = uR;
+        }
     }
     double Bn = Bd[q * G.nface[D] + fidx<D>(G, i, j, k)] / mf.sqrtg;
     pL[BXL + D] = Bn; pR[BXL + D] = Bn;
@@ -1478,7 +1501,7 @@ struct grmhd_ctx {
     hipGraphExec_t gexec = nullptr;
     const void* gkey[8] = {0};
     // workgroup sizes picked by grmhd_autotune (all 256 until then)
-    grmhd_tuning tune{{256, 256, 256}, 256, 256, 256, 256, 256};
+    grmhd_tuning tune{{256, 256, 256}, 256, 256, 256, 256, 256, {0, 0, 0}};
     // two-level forest (grmhd_ctx_create_forest)
     bool forest = false;
     FForest ff{};                             // device tables
@@ -2019,9 +2042,18 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
         const dim3 grid((unsigned)(wpb * G.nq));
         const uint8_t* od = only ? only[d] : nullptr;
         const uint8_t* ro = (only || !c->ring_copy_now) ? nullptr : c->ring_own;
-        if (d == 0) k_flux<EOS, REC, FLAT, 0><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[0], c->flux[0], c->vbar[0], wpb, od, ro);
-        else if (d == 1) k_flux<EOS, REC, FLAT, 1><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[1], c->flux[1], c->vbar[1], wpb, od, ro);
-        else k_flux<EOS, REC, FLAT, 2><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[2], c->flux[2], c->vbar[2], wpb, od, ro);
+        // the prefetching variant is built for WENO-Z only (the other
+        // reconstructions load one or three values per side)
+        bool pf = false;
+        if constexpr (REC == 0) pf = c->tune.flux_prefetch[d] != 0 && !only;
+#define FLUXL(DD, PFV) k_flux<EOS, REC, FLAT, DD, PFV><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[DD], c->flux[DD], c->vbar[DD], wpb, od, ro)
+        if constexpr (REC == 0) {
+            if (pf) { if (d == 0) FLUXL(0, true); else if (d == 1) FLUXL(1, true); else FLUXL(2, true); }
+            else { if (d == 0) FLUXL(0, false); else if (d == 1) FLUXL(1, false); else FLUXL(2, false); }
+        } else {
+            if (d == 0) FLUXL(0, false); else if (d == 1) FLUXL(1, false); else FLUXL(2, false);
+        }
+#undef FLUXL
         if (ro) launch_ring_vbar(c, d, s);
     }
 }
@@ -2801,8 +2833,10 @@ extern "C" grmhd_status grmhd_rk3_step_graph(grmhd_ctx* c, const double* dt_dev,
 
 static bool tuning_valid(const grmhd_tuning& t) {
     auto ok = [](int v) { return v == 64 || v == 128 || v == 256; };
+    auto b = [](int v) { return v == 0 || v == 1; };
     return ok(t.flux[0]) && ok(t.flux[1]) && ok(t.flux[2]) && ok(t.emf) && ok(t.update) && ok(t.gather_faces) &&
-           ok(t.gather_cells) && ok(t.c2p);
+           ok(t.gather_cells) && ok(t.c2p) && b(t.flux_prefetch[0]) && b(t.flux_prefetch[1]) &&
+           b(t.flux_prefetch[2]);
 }
 static void drop_graph(grmhd_ctx* c) {
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -2873,8 +2907,19 @@ extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const 
     const double* const B1c[3] = {c->B1[0], c->B1[1], c->B1[2]};
     grmhd_tuning keep = c->tune;
     for (int d = 0; d < 3; ++d) {
-        // one direction at a time: the other two run at their current size
-        pick(&c->tune.flux[d], [&] { launch_fluxes(c, aux, state, NSTATE, Bc, s); });
+        // one direction at a time (the other two run at their current
+        // choice): workgroup size and, for WENO-Z, the prefetching variant
+        float best = 1e30f;
+        int bs = c->tune.flux[d], bp = c->tune.flux_prefetch[d];
+        for (int pf = 0; pf < (c->P.recon == GRMHD_RECON_WENOZ ? 2 : 1); ++pf)
+            for (int v : cand) {
+                c->tune.flux[d] = v;
+                c->tune.flux_prefetch[d] = pf;
+                const float ms = time_it([&] { launch_fluxes(c, aux, state, NSTATE, Bc, s); });
+                if (ms < best) { best = ms; bs = v; bp = pf; }
+            }
+        c->tune.flux[d] = bs;
+        c->tune.flux_prefetch[d] = bp;
     }
     pick(&c->tune.emf, [&] { launch_emf(c, Bc, s); });
     pick(&c->tune.update, [&] {

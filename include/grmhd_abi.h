@@ -315,7 +315,7 @@ grmhd_status grmhd_rk3_step_graph(grmhd_ctx* ctx, const double* dt_dev, double* 
 grmhd_status grmhd_set_static_metric(grmhd_ctx* ctx, const double* state, int32_t enable, grmhd_stream_t stream);
 
 /* Workgroup sizes (threads) of the kernel families, chosen by
- * grmhd_autotune; every entry is one of 64, 128, 256. */
+ * grmhd_autotune; every size is one of 64, 128, 256. */
 typedef struct {
     int32_t flux[3];        /* face-flux kernels, x/y/z */
     int32_t emf;            /* edge EMFs */
@@ -323,6 +323,9 @@ typedef struct {
     int32_t gather_faces;   /* ghost gather of face B */
     int32_t gather_cells;   /* ghost gather of cell variables */
     int32_t c2p;            /* c2p over cells */
+    int32_t flux_prefetch[3]; /* 1: face-flux kernel x/y/z loads its whole WENO-Z
+                               * stencil before reconstructing (more loads in
+                               * flight, fewer waves); 0 or 1, same results */
 } grmhd_tuning;
 
 /* Times each kernel family of one RK3 substep on the context's own block

@@ -1,7 +1,9 @@
 """Autotuned launch shapes: grmhd_autotune reads but never modifies its
-inputs, and workgroup sizes change nothing in the results (each thread's
-arithmetic is independent of the launch shape): RK3 steps under different
-tunings, eager and graph-replayed, are bit-identical."""
+inputs, and neither workgroup sizes nor the face-flux kernel's stencil
+prefetch change anything in the results (each thread's arithmetic is
+independent of the launch shape and of when its loads are issued): RK3
+steps under different tunings, eager and graph-replayed, are
+bit-identical."""
 import sys
 
 import numpy as np
@@ -37,11 +39,14 @@ def test_gpu_autotune_bitexact(dev):
     assert all(torch.equal(b, k) for b, k in zip(B0, keep[1]))
     vals = tune["flux"] + [tune[k] for k in ("emf", "update", "gather_faces", "gather_cells", "c2p")]
     assert all(v in (64, 128, 256) for v in vals)
+    assert all(v in (0, 1) for v in tune["flux_prefetch"])
     dt = torch.tensor([0.25 * wl.extra["dx"]], dtype=torch.float64, device=dev)
     results = []
     stream = torch.cuda.Stream()
     for name, t in (("default", dict()), ("all64", dict(flux=(64, 64, 64), emf=64, update=64, gather_faces=64,
                                                          gather_cells=64, c2p=64)),
+                    ("prefetch", dict(flux_prefetch=(1, 1, 1))),
+                    ("prefetch_x_z_128", dict(flux=(128, 256, 128), flux_prefetch=(1, 0, 1))),
                     ("tuned", None)):
         if t is None:
             ctx.autotune(s0, B0, a0, reps=1)
