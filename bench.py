@@ -312,15 +312,30 @@ def c4_leg(ga, W, torch, dev, n=1 << 22, n_train=1 << 20, reps=5):
                    "error_vs_plain": {"max_state_rel": float(res.max()), "median_state_rel": float(res.median()),
                                       "same_error_bits": float(same.double().mean())}}
 
+    counts = torch.zeros(n, dtype=torch.uint8, device=dev)
+
+    def eval_dist(use_learned):
+        """master-function + bracket evaluations per state (one untimed pass,
+        grmhd_set_c2p_eval_counts): mean and percentiles"""
+        ctx.set_c2p_eval_counts(counts)
+        once(use_learned, full, n, cons)
+        ctx.set_c2p_eval_counts(None)
+        c = counts.to(torch.int32)
+        pct = lambda f: int(torch.kthvalue(c, max(1, int(np.ceil(f * n)))).values.item())
+        return {"mean": round(float(c.double().mean().item()), 3), "p50": pct(0.5), "p99": pct(0.99),
+                "p999": pct(0.999), "max": int(c.max().item())}
+
+    evals_plain = eval_dist(False)
     _, brk = learned(0)
     t_ln, lrn = learned(newton)
+    lrn["evals_per_state"] = eval_dist(True)
     return {
         "workload": "c2p_4M_kerr_schild_a0.9_r1.5-50_ideal_gamma4/3",
         "states": n,
         "plain": {"ms": round(t_pl, 4), "states_per_s": n / (t_pl * 1e-3),
                   "root_evals_per_state": round(s_pl["n_root_evals"] / n, 3),
                   "bracket_evals_per_state": round(s_pl["n_bracket_evals"] / n, 3),
-                  "max_root_evals": s_pl["max_root_evals"]},
+                  "max_root_evals": s_pl["max_root_evals"], "evals_per_state": evals_plain},
         "learned": dict(lrn, finish=f"Newton from the predicted roots, <= {newton} evaluations, then plain"),
         "learned_bracket_only": dict(brk, bracket={"delta_e": best[1], "delta_s": ds0, "nexp": best[2]}),
         "speedup": lrn["speedup"],
@@ -574,6 +589,10 @@ def main():
     ap.add_argument("--no-torus", action="store_true", help="skip the Kerr-Schild torus leg")
     ap.add_argument("--no-curved", action="store_true", help="skip the curved-metric 128^3 blast leg")
     ap.add_argument("--no-tune", action="store_true", help="default workgroup sizes instead of grmhd_autotune")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed tuning of the 128^3 blast instead of grmhd_autotune, as the JSON object the bench "
+                         "line reports under config.tuning (profiling runs that must not contain the autotuner's "
+                         "launches)")
     ap.add_argument("--no-c1", action="store_true", help="skip the Balsara-1 full-run leg")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
@@ -643,7 +662,10 @@ def main():
 
     tuning = None
     with torch.cuda.stream(stream):
-        if not args.no_tune:
+        if args.tuning:
+            ctx.set_tuning(**json.loads(args.tuning))
+            tuning = ctx.get_tuning()
+        elif not args.no_tune:
             # workgroup sizes per kernel family picked by timing (inputs are
             # only read), before the warm-up and the timed steps
             tuning = ctx.autotune(st, B, aux, reps=3, stream=sp)

@@ -97,7 +97,7 @@ ABI_SYMBOLS = [
     "grmhd_fill_metric_ghosts",
     "grmhd_advance_substep", "grmhd_fill_ghosts", "grmhd_compute_auxiliaries", "grmhd_c2p_batch",
     "grmhd_c2p_batch_noalloc", "grmhd_advance_substep_t", "grmhd_sync", "grmhd_find_stable_timestep",
-    "grmhd_set_c2p_warm_start",
+    "grmhd_set_c2p_warm_start", "grmhd_set_c2p_eval_counts",
     "grmhd_prims_to_conservs_batch", "grmhd_c2p_stats", "grmhd_compute_dt", "grmhd_rk3_step",
     "grmhd_rk3_step_graph", "grmhd_set_timing", "grmhd_kernel_time_ms",
     "grmhd_autotune", "grmhd_get_tuning", "grmhd_set_tuning",
@@ -212,6 +212,7 @@ def lib():
     L.grmhd_mlp_fit_cells.argtypes = [P, P, P, P * 3, i64, C.POINTER(MlpFitOpts), C.POINTER(MlpFitReport), P]
     L.grmhd_set_learned_c2p.argtypes = [P, P]
     L.grmhd_set_c2p_warm_start.argtypes = [P, i32]
+    L.grmhd_set_c2p_eval_counts.argtypes = [P, P]
     L.grmhd_set_static_metric.argtypes = [P, P, i32, P]
     for name in ABI_SYMBOLS:
         getattr(L, name)  # raises AttributeError if an exported symbol is missing
@@ -372,6 +373,12 @@ class Context:
 
     def set_c2p_warm_start(self, max_newton_evals=8):
         check(self.L.grmhd_set_c2p_warm_start(self.h, int(max_newton_evals)), "set_c2p_warm_start")
+
+    def set_c2p_eval_counts(self, counts):
+        """Per-state evaluation counts of the batch c2p into a uint8 device
+        tensor of n elements (None: off)."""
+        check(self.L.grmhd_set_c2p_eval_counts(self.h, _ptr(counts)), "set_c2p_eval_counts")
+        self._eval_counts = counts   # keep it alive while the context writes it
 
     def autotune(self, state, B, aux, reps=3, stream=None):
         t = Tuning()

@@ -33,6 +33,7 @@
 // Gather-table flags (block field of a ghost source entry).
 constexpr int GSRC_OWN = 1 << 30;   // the ghost runs its own c2p
 constexpr int GSRC_AMR = 1 << 29;   // valued by prolongation / restriction, not copied
+constexpr int GSRC_COPY = 1 << 28;  // takes the c2p outputs of a first-layer outflow ghost of its own block
 
 struct FForest {
     int nb[3];

@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include "phys.hh"
 #include "mlp.hh"
@@ -87,6 +88,9 @@ struct Opts {
     // cells that took a branch where the reference aborts (c2p.hh:160-322),
     // counted by the c2p passes until a sync point reads and clears it
     unsigned long long* fatal_count;
+    // batch c2p: per-state master-function + bracket evaluations (clamped to
+    // 255) when set (grmhd_set_c2p_eval_counts)
+    uint8_t* eval_counts;
 };
 
 GHD int64_t cidx(const DevGrid& G, int i, int j, int k) { return i + (int64_t)G.N[0] * (j + (int64_t)G.N[1] * k); }
@@ -108,12 +112,21 @@ GHD int64_t eidx(const DevGrid& G, int i, int j, int k) {
 // robin; the remap hands each XCD a contiguous range of logical workgroups,
 // so the workgroups of one block share that XCD's L2 instead of all eight
 // L2s fetching the same stencil data.
+// (A 1-D grid has fewer than 2^32 workgroups, so 32-bit unsigned
+// arithmetic: one short scalar division instead of a 64-bit one.)
 struct WgPos { int tile; int64_t q; };
 __device__ __forceinline__ WgPos wg_pos(int wpb) {
-    const int64_t p = blockIdx.x, n = gridDim.x;
-    const int64_t x = p & 7, i = p >> 3, base = n >> 3, rem = n & 7;
-    const int64_t L = x * base + (x < rem ? x : rem) + i;
-    return WgPos{(int)(L % wpb), L / wpb};
+    const uint32_t p = blockIdx.x, n = gridDim.x;
+    const uint32_t x = p & 7u, i = p >> 3, base = n >> 3, rem = n & 7u;
+    const uint32_t L = x * base + (x < rem ? x : rem) + i;
+    const uint32_t w = (uint32_t)wpb, q = L / w;
+    return WgPos{(int)(L - q * w), (int64_t)q};
+}
+// x = q d + r for x >= 0, d > 0 (unsigned: no sign fix-ups)
+GHD void udivmod(int x, int d, int& q, int& r) {
+    const uint32_t uq = (uint32_t)x / (uint32_t)d;
+    q = (int)uq;
+    r = x - (int)uq * d;
 }
 
 // Read the Cowling metric of a cell (state vars GXX_..BETAZ_).
@@ -171,34 +184,39 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
             const int N0 = G.N[T0], N1 = G.N[T1];
             const int Rg = Rn + 2 * o.fofc;
             if (e >= Rg * N0 * N1) return;
-            if (D == 0) { a = e % Rg; const int r = e / Rg; b = r % N0; c = r / N0; }
-            else if (D == 1) { b = e % N0; const int r = e / N0; a = r % Rg; c = r / Rg; }
-            else { b = e % N0; const int r = e / N0; c = r % N1; a = r / N1; }
+            int r;
+            if (D == 0) { udivmod(e, Rg, r, a); udivmod(r, N0, c, b); }
+            else if (D == 1) { udivmod(e, N0, r, b); udivmod(r, Rg, c, a); }
+            else { udivmod(e, N0, r, b); udivmod(r, N1, a, c); }
             a -= o.fofc;
         } else {
             const int nint = Rn * n0 * n1, g2 = 2 * ng;
             const int nA = Rn * g2 * n1, nB = Rn * n0 * g2;
             auto ghost = [&](int g, int nn) { return g < ng ? g : nn + g; };
+            int r;
             if (e < nint) {
-                if (D == 0) { a = e % Rn; const int r = e / Rn; b = ng + r % n0; c = ng + r / n0; }
-                else if (D == 1) { b = ng + e % n0; const int r = e / n0; a = r % Rn; c = ng + r / Rn; }
-                else { b = ng + e % n0; const int r = e / n0; c = ng + r % n1; a = r / n1; }
+                if (D == 0) { udivmod(e, Rn, r, a); udivmod(r, n0, c, b); }
+                else if (D == 1) { udivmod(e, n0, r, b); udivmod(r, Rn, c, a); }
+                else { udivmod(e, n0, r, b); udivmod(r, n1, a, c); }
+                b += ng; c += ng;
             } else if (e < nint + nA) {          // t0 ghost, t1 interior
                 const int x = e - nint;
                 ring = true;
                 int gl;
-                if (D == 0) { a = x % Rn; const int r = x / Rn; gl = r % g2; c = ng + r / g2; }
-                else if (D == 1) { gl = x % g2; const int r = x / g2; a = r % Rn; c = ng + r / Rn; }
-                else { gl = x % g2; const int r = x / g2; c = ng + r % n1; a = r / n1; }
+                if (D == 0) { udivmod(x, Rn, r, a); udivmod(r, g2, c, gl); }
+                else if (D == 1) { udivmod(x, g2, r, gl); udivmod(r, Rn, c, a); }
+                else { udivmod(x, g2, r, gl); udivmod(r, n1, a, c); }
+                c += ng;
                 if (ring_own && !((ring_own[q] >> (2 * T0 + (gl >= ng))) & 1)) return;
                 b = ghost(gl, n0);
             } else if (e < nint + nA + nB) {     // t0 interior, t1 ghost
                 const int x = e - nint - nA;
                 ring = true;
                 int gl;
-                if (D == 0) { a = x % Rn; const int r = x / Rn; b = ng + r % n0; gl = r / n0; }
-                else if (D == 1) { b = ng + x % n0; const int r = x / n0; a = r % Rn; gl = r / Rn; }
-                else { b = ng + x % n0; const int r = x / n0; gl = r % g2; a = r / g2; }
+                if (D == 0) { udivmod(x, Rn, r, a); udivmod(r, n0, gl, b); }
+                else if (D == 1) { udivmod(x, n0, r, b); udivmod(r, Rn, gl, a); }
+                else { udivmod(x, n0, r, b); udivmod(r, g2, a, gl); }
+                b += ng;
                 if (ring_own && !((ring_own[q] >> (2 * T1 + (gl >= ng))) & 1)) return;
                 c = ghost(gl, n1);
             } else {
@@ -289,16 +307,16 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     }
     // compute_mhd_fluxes (grmhd.hh:774-970)
     const double *zl = &pL[ZXL], *zr = &pR[ZXL], *Bl = &pL[BXL], *Br = &pR[BXL];
-    double wl = lorentz(mf.g, zl), wr = lorentz(mf.g, zr);
+    double wl = lorentz<FLAT>(mf.g, zl), wr = lorentz<FLAT>(mf.g, zr);
     double bl[4], br[4], b2l, b2r;
-    smallb(mf, Bl, zl, wl, bl, b2l);
-    smallb(mf, Br, zr, wr, br, b2r);
+    smallb<FLAT>(mf, Bl, zl, wl, bl, b2l);
+    smallb<FLAT>(mf, Br, zr, wr, br, b2r);
     double vtl[3], vtr[3];
     vtilde(mf, zl, wl, vtl);
     vtilde(mf, zr, wr, vtr);
     double cml, cpl, cmr, cpr, cmin, cmax;
-    cmcp(mf, pL[RHOL], pL[PRESSL], pL[EPSL], pL[CS2L], wl, b2l, vtl, D, cml, cpl);
-    cmcp(mf, pR[RHOL], pR[PRESSL], pR[EPSL], pR[CS2L], wr, b2r, vtr, D, cmr, cpr);
+    cmcp<FLAT>(mf, pL[RHOL], pL[PRESSL], pL[EPSL], pL[CS2L], wl, b2l, vtl, D, cml, cpl);
+    cmcp<FLAT>(mf, pR[RHOL], pR[PRESSL], pR[EPSL], pR[CS2L], wr, b2r, vtr, D, cmr, cpr);
     if (o.riemann == 1) {
         cmax = fmax(fmax(fabs(cml), fabs(cmr)), fmax(fabs(cpl), fabs(cpr)));
         if (fmin(pL[RHOL], pR[RHOL]) < o.c_limit) {
@@ -323,8 +341,8 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
         if (ring) return;
     }
     double Dl, taul, Sl[3], Esl, Dr, taur, Sr[3], Esr;
-    conserved(mf, pL[RHOL], pL[PRESSL], pL[EPSL], zl, pL[ENTL], wl, b2l, bl, Dl, taul, Sl, Esl);
-    conserved(mf, pR[RHOL], pR[PRESSL], pR[EPSL], zr, pR[ENTL], wr, b2r, br, Dr, taur, Sr, Esr);
+    conserved<FLAT>(mf, pL[RHOL], pL[PRESSL], pL[EPSL], zl, pL[ENTL], wl, b2l, bl, Dl, taul, Sl, Esl);
+    conserved<FLAT>(mf, pR[RHOL], pR[PRESSL], pR[EPSL], zr, pR[ENTL], wr, b2r, br, Dr, taur, Sr, Esr);
     // fluxes along D (grmhd_subexpressions.hh:97-156)
     double bsh[3]; lower3(mf.g, mf.beta[0], mf.beta[1], mf.beta[2], bsh);
     double ia2 = 1. / (mf.alp * mf.alp);
@@ -335,12 +353,19 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
         double W2rh = wl * wl * rhoh;
         double fD = vtl[D] * Dl;
         Fl[0] = fD;
-        Fl[4] = W2rh * vtl[D] + ps * mf.beta[D] - mf.alp * mf.alp * bl[0] * bl[1 + D] - fD;
-        double blo[3]; lower3(mf.g, bl[1], bl[2], bl[3], blo);
-        double vlo[3]; lower3(mf.g, vtl[0], vtl[1], vtl[2], vlo);
-        double c = vtl[D] * W2rh * ia2;
-        for (int a = 0; a < 3; ++a)
-            Fl[1 + a] = mf.alp * (-bl[1 + D] * (blo[a] + bl[0] * bsh[a]) + (a == D ? ps : 0.0) + c * (vlo[a] + bsh[a]));
+        double blo[3]; lower3<FLAT>(mf.g, bl[1], bl[2], bl[3], blo);
+        double vlo[3]; lower3<FLAT>(mf.g, vtl[0], vtl[1], vtl[2], vlo);
+        if constexpr (FLAT) {
+            Fl[4] = W2rh * vtl[D] - bl[0] * bl[1 + D] - fD;
+            const double c = vtl[D] * W2rh;
+            for (int a = 0; a < 3; ++a)
+                Fl[1 + a] = a == D ? -bl[1 + D] * blo[a] + ps + c * vlo[a] : -bl[1 + D] * blo[a] + c * vlo[a];
+        } else {
+            Fl[4] = W2rh * vtl[D] + ps * mf.beta[D] - mf.alp * mf.alp * bl[0] * bl[1 + D] - fD;
+            const double c = vtl[D] * W2rh * ia2;
+            for (int a = 0; a < 3; ++a)
+                Fl[1 + a] = mf.alp * (-bl[1 + D] * (blo[a] + bl[0] * bsh[a]) + (a == D ? ps : 0.0) + c * (vlo[a] + bsh[a]));
+        }
         Fl[6] = vtl[D] * Esl;
         Fl[5] = pL[YEL] * fD;
     }
@@ -350,12 +375,19 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
         double W2rh = wr * wr * rhoh;
         double fD = vtr[D] * Dr;
         Fr[0] = fD;
-        Fr[4] = W2rh * vtr[D] + ps * mf.beta[D] - mf.alp * mf.alp * br[0] * br[1 + D] - fD;
-        double blo[3]; lower3(mf.g, br[1], br[2], br[3], blo);
-        double vlo[3]; lower3(mf.g, vtr[0], vtr[1], vtr[2], vlo);
-        double c = vtr[D] * W2rh * ia2;
-        for (int a = 0; a < 3; ++a)
-            Fr[1 + a] = mf.alp * (-br[1 + D] * (blo[a] + br[0] * bsh[a]) + (a == D ? ps : 0.0) + c * (vlo[a] + bsh[a]));
+        double blo[3]; lower3<FLAT>(mf.g, br[1], br[2], br[3], blo);
+        double vlo[3]; lower3<FLAT>(mf.g, vtr[0], vtr[1], vtr[2], vlo);
+        if constexpr (FLAT) {
+            Fr[4] = W2rh * vtr[D] - br[0] * br[1 + D] - fD;
+            const double c = vtr[D] * W2rh;
+            for (int a = 0; a < 3; ++a)
+                Fr[1 + a] = a == D ? -br[1 + D] * blo[a] + ps + c * vlo[a] : -br[1 + D] * blo[a] + c * vlo[a];
+        } else {
+            Fr[4] = W2rh * vtr[D] + ps * mf.beta[D] - mf.alp * mf.alp * br[0] * br[1 + D] - fD;
+            const double c = vtr[D] * W2rh * ia2;
+            for (int a = 0; a < 3; ++a)
+                Fr[1 + a] = mf.alp * (-br[1 + D] * (blo[a] + br[0] * bsh[a]) + (a == D ? ps : 0.0) + c * (vlo[a] + bsh[a]));
+        }
         Fr[6] = vtr[D] * Esr;
         Fr[5] = pR[YEL] * fD;
     }
@@ -466,8 +498,10 @@ __global__ __launch_bounds__(256) void k_emf_uct(DevGrid G, const double* __rest
     const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
     const int64_t q = wp.q;
-    const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
-    const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
+    int kz, rxy, jy, ix0;
+    udivmod(r, R0 * R1, kz, rxy);
+    udivmod(rxy, R0, jy, ix0);
+    const int i = G.ng + ix0, j = G.ng + jy, k = G.ng + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2];
     const double* bx = Bx + q * nfx; const double* by = By + q * nfy; const double* bz = Bz + q * nfz;
     const double* Vx = vx + q * 4 * nfx; const double* Vy = vy + q * 4 * nfy; const double* Vz = vz + q * 4 * nfz;
@@ -534,8 +568,10 @@ __global__ __launch_bounds__(256) void k_emf_gs(DevGrid G, const double* __restr
     const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
     const int64_t q = wp.q;
-    const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
-    const int i = G.ng - wide + rxy % R0, j = G.ng - wide + rxy / R0, k = G.ng - wide + kz;
+    int kz, rxy, jy, ix0;
+    udivmod(r, R0 * R1, kz, rxy);
+    udivmod(rxy, R0, jy, ix0);
+    const int i = G.ng - wide + ix0, j = G.ng - wide + jy, k = G.ng - wide + kz;
     const int64_t nfx = G.nface[0], nfy = G.nface[1], nfz = G.nface[2], nc = G.ncell;
     auto EF = [&](int d, int c, int ii, int jj, int kk) {
         const double* base = d == 0 ? ef0 : (d == 1 ? ef1 : ef2);
@@ -705,8 +741,10 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
     const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (r >= R0 * R1 * R2) return;
     const int64_t q = wp.q;
-    const int kz = r / (R0 * R1), rxy = r - kz * (R0 * R1);
-    const int i = G.ng + rxy % R0, j = G.ng + rxy / R0, k = G.ng + kz;
+    int kz, rxy, jy, ix0;
+    udivmod(r, R0 * R1, kz, rxy);
+    udivmod(rxy, R0, jy, ix0);
+    const int i = G.ng + ix0, j = G.ng + jy, k = G.ng + kz;
     const DevInst& I = a.inst[a.blk[q].inst];
     const double ix = I.idx[0], iy = I.idx[1], iz = I.idx[2];
     const double dtf = (*a.dt) * a.dtfact;
@@ -968,6 +1006,7 @@ struct GatherArgs {
     double* B[3];                            // face arrays (null: skip faces)
     int cell_mode;
     bool slab_only;                          // cells: only those with one ghost coordinate
+    bool own_copies;                         // mode 3: also the copying outflow ghosts (GSRC_COPY)
 };
 // GSRC_OWN / GSRC_AMR: forest.hh
 
@@ -991,6 +1030,8 @@ static void ghost_source(const DevGrid& G, const DevInst& I, const DevBlk& b, in
     off = (int)(fd < 0 ? cidx(G, sl[0], sl[1], sl[2]) : fidx_d(G, fd, sl[0], sl[1], sl[2]));
 }
 
+// Outflow ghosts that take a first-layer ghost's c2p outputs carry GSRC_COPY
+// with that ghost as their source.
 // One copy per thread.  A channel is one variable of one array
 // (base, vars per block, var, elements per var per block).
 struct GatherChan { double* base; int nv, v; int64_t n; };
@@ -1002,7 +1043,8 @@ struct GatherChans { GatherChan c[GATHER_MAXCH]; int nch; };
 // before their stores.  mode: 1 every ghost cell, 2 only those that run their
 // own c2p, 3 only those that take copies of c2p outputs.
 __global__ __launch_bounds__(256) void k_gather_cells(const int32_t* __restrict__ list, const int2* __restrict__ src,
-                                                      int per, int ncg, int wpb, int mode, GatherChans ch) {
+                                                      int per, int ncg, int wpb, int mode, int copies,
+                                                      GatherChans ch) {
     const WgPos wp = wg_pos(wpb);
     const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
     if (e >= ncg) return;
@@ -1010,7 +1052,11 @@ __global__ __launch_bounds__(256) void k_gather_cells(const int32_t* __restrict_
     const int2 sr = src[q * per + e];
     const bool own = (sr.x & GSRC_OWN) != 0;
     if ((sr.x & GSRC_AMR) || (mode >= 2 && (mode == 2) != own)) return;
-    const int64_t qs = sr.x & (GSRC_AMR - 1);
+    // an outflow ghost that copies a first-layer ghost of its block (source
+    // {block, first-layer cell}): its c2p outputs in mode 3 when asked; its
+    // conserved in mode 1 come after this launch, from that filled cell
+    if ((sr.x & GSRC_COPY) && (mode == 1 || !copies)) return;
+    const int64_t qs = sr.x & (GSRC_COPY - 1);
     const int off = list[e];
     for (int c0 = 0; c0 < ch.nch; c0 += 4) {
         double v[4];
@@ -1163,8 +1209,9 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
     if (cset == CS_INTERIOR) {
         q = wp.q;
         const int n0 = G.n[0], n01 = G.n[0] * G.n[1];
-        k = ee / n01; const int r = ee - k * n01;
-        j = r / n0; i = r - j * n0;
+        int r;
+        udivmod(ee, n01, k, r);
+        udivmod(r, n0, j, i);
         i += G.ng; j += G.ng; k += G.ng;
         c = cidx(G, i, j, k);
     } else {
@@ -1173,8 +1220,9 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_cells(DevGrid G, EO
         else { const int2 p = cl[ee]; q = p.x; off = p.y; }
         c = off;
         const int N0 = G.N[0], N01 = G.N[0] * G.N[1];
-        k = off / N01; const int r = off - k * N01;
-        j = r / N0; i = r - j * N0;
+        int r;
+        udivmod(off, N01, k, r);
+        udivmod(r, N0, j, i);
     }
     const int64_t nc = G.ncell;
     Met m;
@@ -1346,6 +1394,10 @@ __global__ __launch_bounds__(256, GRMHD_C2P_MINB) void k_c2p_batch(int64_t n, EO
         for (int v = 0; v < NCONS; ++v) cons_soa[v * n + s] = cons[v];
         errb[s] = err;
         floored[s] = out.floored ? 1 : 0;
+        if (o.eval_counts) {
+            const int ev = cnt.root + cnt.bracket;
+            o.eval_counts[s] = (uint8_t)(ev < 255 ? ev : 255);
+        }
     } else {
         cnt = C2PCount();
     }
@@ -1486,6 +1538,14 @@ struct grmhd_ctx {
     int2* own_cells = nullptr;                // {block, cell} of ghosts that run their own c2p
     int4* own_gather = nullptr;               // the same with their sources
     int64_t n_own = 0;
+    // Ghosts beyond outflow faces whose c2p inputs are bit-identical to those
+    // of the first ghost layer's cell of the same block (flat metric, no
+    // radius excision): {block, cell, block, first-layer cell}; they take
+    // copies of that cell's c2p outputs instead of running their own (the
+    // stage-end gather, through GSRC_COPY in the ghost table) and, when every
+    // ghost is filled, its conserved (this list, after the cell gather).
+    int4* copy_gather = nullptr;
+    int64_t n_copy = 0;
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
     int64_t ncg_slab = 0;                     // ghost cells with one ghost coordinate (listed first)
     int64_t last_c2p_n = 0;
@@ -1518,9 +1578,17 @@ struct grmhd_ctx {
     double* vcm = nullptr;                    // virtual coarse metric field (grmhd_fill_metric_ghosts)
 };
 
+// Timing-only events: no system-scope fence when recorded, so a timer does
+// not write back and invalidate the caches between the launches it brackets
+// (which would also slow the launch that follows it).
+static hipEvent_t timing_event() {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    return e;
+}
 static hipEvent_t pool_event(grmhd_ctx* c) {
     if (!c->ev_pool.empty()) { hipEvent_t e = c->ev_pool.back(); c->ev_pool.pop_back(); return e; }
-    hipEvent_t e; (void)hipEventCreate(&e); return e;
+    return timing_event();
 }
 struct Scope {
     grmhd_ctx* c; const char* fam; hipStream_t s; hipEvent_t a = nullptr, b = nullptr;
@@ -1587,7 +1655,7 @@ static void build_eos(grmhd_ctx* c) {
 
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather);
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather); F(c->copy_gather);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
     F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->amr_faces_mid); F(c->rflux); F(c->remf);
     F(c->vcm); F(c->fatal_d); F(c->dt_scalar); F(c->ring_nbr); F(c->ring_own);
@@ -1636,6 +1704,10 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->ngz < 3 || g->nq < 1) {
         set_err("invalid grid (need n>=1, ngz>=3, nq>=1): %d %d %d ngz=%d nq=%lld", g->nx, g->ny, g->nz, g->ngz,
                 (long long)g->nq);
+        return GRMHD_ERR_INVALID;
+    }
+    if (g->nq >= GSRC_COPY) {   // the ghost tables keep flags above the block index
+        set_err("too many blocks: %lld (at most %d)", (long long)g->nq, GSRC_COPY - 1);
         return GRMHD_ERR_INVALID;
     }
     if (p->eos_kind != GRMHD_EOS_IDEAL_GAS && p->eos_kind != GRMHD_EOS_HYBRID_PWPOLY) { set_err("eos_kind"); return GRMHD_ERR_UNSUPPORTED; }
@@ -1703,6 +1775,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
     O.riemann = p->riemann; O.emf = p->emf; O.thermo = p->thermo_recon; O.fd_order = p->fd_order;
     O.c_limit = p->rusanov_c_limit;
     O.fofc = p->fofc; O.dmp = p->fofc_dmp_enable; O.dmp_M = p->fofc_dmp_M;
+    O.eval_counts = nullptr;
     build_eos(c);
     c->blk_h = blk;
     for (int a = 0; a < n_inst; ++a) {
@@ -1855,11 +1928,50 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                 src[(size_t)(q * per + x)] = make_int2(qs, so);
             }
         }
+        // Beyond an outflow face every ghost layer holds the same clamped
+        // conserved and, through the clamped face copies, the same cell-centred
+        // B (both faces of a cell beyond the face copy the boundary face): with
+        // a flat metric and no radius-based excision its c2p inputs are those
+        // of the cell with every outflow-exceeding coordinate moved to the first
+        // ghost layer.  Only that cell solves; the others copy its outputs.
+        std::vector<int4> copyg;
+        std::vector<uint8_t> is_copy((size_t)(per * G.nq), 0);
+        if (!fin && c->P.flat_metric && !c->P.excise_by_radius) {
+            std::unordered_map<int32_t, int64_t> pos;   // local cell offset -> entry
+            for (int64_t x = 0; x < c->ncg; ++x) pos[lst[x]] = x;
+            for (int64_t q = 0; q < G.nq; ++q) {
+                const DevBlk& b = blk[q];
+                const DevInst& I = c->inst_h[b.inst];
+                for (int64_t x = 0; x < c->ncg; ++x) {
+                    if (!(src[(size_t)(q * per + x)].x & GSRC_OWN)) continue;
+                    const int off = lst[x];
+                    int loc[3] = {off % G.N[0], (off / G.N[0]) % G.N[1], off / (G.N[0] * G.N[1])};
+                    int rl[3] = {loc[0], loc[1], loc[2]};
+                    for (int d = 0; d < 3; ++d) {
+                        const int Nt = I.nb[d] * G.n[d];
+                        const int g = b.b[d] * G.n[d] + loc[d] - G.ng;
+                        if (g < 0 && I.bc[2 * d] == 0) rl[d] = -1 - b.b[d] * G.n[d] + G.ng;
+                        if (g >= Nt && I.bc[2 * d + 1] == 0) rl[d] = Nt - b.b[d] * G.n[d] + G.ng;
+                    }
+                    if (rl[0] == loc[0] && rl[1] == loc[1] && rl[2] == loc[2]) continue;
+                    bool inside = true;
+                    for (int d = 0; d < 3; ++d) inside &= rl[d] >= 0 && rl[d] < G.N[d];
+                    if (!inside) continue;
+                    const int32_t roff = (int32_t)cidx(G, rl[0], rl[1], rl[2]);
+                    auto it = pos.find(roff);
+                    if (it == pos.end() || !(src[(size_t)(q * per + it->second)].x & GSRC_OWN)) continue;
+                    is_copy[(size_t)(q * per + x)] = 1;
+                    src[(size_t)(q * per + x)] = make_int2((int)q | GSRC_COPY, roff);
+                    copyg.push_back(make_int4((int)q, off, (int)q, roff));
+                }
+            }
+        }
         std::vector<int2> own, amr_c, amr_f, amr_fm;
         std::vector<int4> owng;
         for (int64_t q = 0; q < G.nq; ++q)
             for (int64_t x = 0; x < per; ++x) {
                 const int2 sr = src[(size_t)(q * per + x)];
+                if (is_copy[(size_t)(q * per + x)]) continue;
                 if (x >= c->ncg) {
                     if (sr.x & GSRC_AMR) {
                         const int fd = x < c->ncg + c->nfg[0] ? 0 : (x < c->ncg + c->nfg[0] + c->nfg[1] ? 1 : 2);
@@ -1877,6 +1989,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
             }
         c->n_own = (int64_t)own.size();
         c->n_own_gather = (int64_t)owng.size();
+        c->n_copy = (int64_t)copyg.size();
         c->n_amr_cells = (int64_t)amr_c.size();
         c->n_amr_faces = (int64_t)amr_f.size();
         c->n_amr_faces_mid = (int64_t)amr_fm.size();
@@ -1890,6 +2003,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         };
         if (!upload(&c->own_cells, own, "own cells")) return fail("own cells", e);
         if (!upload(&c->own_gather, owng, "own gather")) return fail("own gather", e);
+        if (!upload(&c->copy_gather, copyg, "copy gather")) return fail("outflow ghost copies", e);
         if (!upload(&c->amr_cells, amr_c, "amr cells")) return fail("coarse-fine cells", e);
         if (!upload(&c->amr_faces, amr_f, "amr faces")) return fail("coarse-fine faces", e);
         if (!upload(&c->amr_faces_mid, amr_fm, "amr faces mid")) return fail("coarse-fine interior faces", e);
@@ -2168,7 +2282,12 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     const int64_t ncells = a.slab_only ? c->ncg_slab : c->ncg;
     const int wpb = (int)nblk_bs(ncells, c->tune.gather_cells);
     k_gather_cells<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per,
-                                                                                 (int)ncells, wpb, a.cell_mode, ch);
+                                                                                 (int)ncells, wpb, a.cell_mode,
+                                                                                 a.own_copies ? 1 : 0, ch);
+    // every ghost: the copying outflow ghosts take the first-layer cell the
+    // launch above filled
+    if (a.cell_mode == 1 && c->n_copy > 0)
+        k_gather_list<<<dim3(nblk(c->n_copy), 1, ch.nch), 256, 0, s>>>(c->copy_gather, (int)c->n_copy, ch);
     if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
 }
 
@@ -2530,6 +2649,12 @@ extern "C" grmhd_status grmhd_set_static_metric(grmhd_ctx* c, const double* stat
     return launch_check("metric_der");
 }
 
+extern "C" grmhd_status grmhd_set_c2p_eval_counts(grmhd_ctx* c, uint8_t* counts_or_null) {
+    if (!c) { set_err("null context"); return GRMHD_ERR_INVALID; }
+    c->O.eval_counts = counts_or_null;
+    return GRMHD_OK;
+}
+
 extern "C" grmhd_status grmhd_set_c2p_warm_start(grmhd_ctx* c, int32_t max_newton_evals) {
     if (!c) { set_err("null context"); return GRMHD_ERR_INVALID; }
     if (max_newton_evals < 0 || max_newton_evals > 64) { set_err("max_newton_evals %d out of [0, 64]", max_newton_evals); return GRMHD_ERR_INVALID; }
@@ -2771,6 +2896,7 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
             g.st = (last || c->P.fofc) ? S.nst : nullptr; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC;
             g.ecen = c->P.emf == GRMHD_EMF_GS ? c->ecen : nullptr;
             g.cell_mode = 3;
+            g.own_copies = true;
             // the next stage copies the ring faces' values from the
             // neighbouring blocks: its fluxes read ghost primitives only in
             // the face slabs (and in ghosts beyond outflow faces, which run
@@ -2873,9 +2999,7 @@ extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const 
     (void)hipMemcpyAsync(aux2, aux, sizeof(double) * NAUX * G.ncell * G.nq, hipMemcpyDeviceToDevice, s);
     const bool was_timing = c->timing;
     c->timing = false;
-    hipEvent_t a, b;
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
+    hipEvent_t a = timing_event(), b = timing_event();
     auto time_it = [&](auto&& fn) {
         fn();   // warm-up
         std::vector<float> t;
@@ -2937,7 +3061,7 @@ extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const 
     });
     pick(&c->tune.gather_cells, [&] {
         GatherArgs g{};
-        g.aux = aux2; g.aux_mask = 0x3FFu; g.cell_mode = 3;
+        g.aux = aux2; g.aux_mask = 0x3FFu; g.cell_mode = 3; g.own_copies = true;
         launch_gather(c, g, s);
     });
     {   // ghosts of the stage copy, so the c2p sees valid inputs everywhere

@@ -288,28 +288,47 @@ GHD double excision_radius_of(const ExcPar& ex, double x, double y, double z) {
 }
 
 // ---- GRMHD algebra ----------------------------------------------------------
+// FL = true: the metric is the identity (lapse 1, shift 0, gamma_ij =
+// delta_ij) and the terms that multiply a zero metric entry are left out.
+// IEEE arithmetic cannot drop them by itself (0 x is not 0 for infinite or
+// NaN x, and x + 0 turns a zero x of negative sign positive), so the
+// flat-metric kernels ask for it; on finite inputs the values are those of
+// the general formulas up to the sign of a zero.
+template <bool FL = false>
 GHD void lower3(const double g[6], double x, double y, double z, double o[3]) {
-    o[0] = g[0] * x + g[1] * y + g[2] * z;
-    o[1] = g[1] * x + g[3] * y + g[4] * z;
-    o[2] = g[2] * x + g[4] * y + g[5] * z;
+    if constexpr (FL) {
+        o[0] = x; o[1] = y; o[2] = z;
+    } else {
+        o[0] = g[0] * x + g[1] * y + g[2] * z;
+        o[1] = g[1] * x + g[3] * y + g[4] * z;
+        o[2] = g[2] * x + g[4] * y + g[5] * z;
+    }
 }
+template <bool FL = false>
 GHD double lorentz(const double g[6], const double z[3]) {
-    double zl[3]; lower3(g, z[0], z[1], z[2], zl);
+    double zl[3]; lower3<FL>(g, z[0], z[1], z[2], zl);
     return sqrt(z[0] * zl[0] + z[1] * zl[1] + z[2] * zl[2] + 1);
 }
+template <bool FL = false>
 GHD void smallb(const Met& m, const double B[3], const double z[3], double W, double b[4], double& b2) {
-    double zl[3]; lower3(m.g, z[0], z[1], z[2], zl);
+    double zl[3]; lower3<FL>(m.g, z[0], z[1], z[2], zl);
     double Bz = B[0] * zl[0] + B[1] * zl[1] + B[2] * zl[2];
     double iW = 1. / W;
-    b[0] = Bz / m.alp;
-    for (int i = 0; i < 3; ++i) b[1 + i] = iW * (B[i] + Bz * (z[i] - W * m.beta[i] / m.alp));
-    double Bl[3]; lower3(m.g, B[0], B[1], B[2], Bl);
+    if constexpr (FL) {
+        b[0] = Bz;
+        for (int i = 0; i < 3; ++i) b[1 + i] = iW * (B[i] + Bz * z[i]);
+    } else {
+        b[0] = Bz / m.alp;
+        for (int i = 0; i < 3; ++i) b[1 + i] = iW * (B[i] + Bz * (z[i] - W * m.beta[i] / m.alp));
+    }
+    double Bl[3]; lower3<FL>(m.g, B[0], B[1], B[2], Bl);
     b2 = (B[0] * Bl[0] + B[1] * Bl[1] + B[2] * Bl[2] + Bz * Bz) / (W * W);
 }
 GHD void vtilde(const Met& m, const double z[3], double W, double vt[3]) {
     double iW = 1. / W;
     for (int i = 0; i < 3; ++i) vt[i] = m.alp * iW * z[i] - m.beta[i];
 }
+template <bool FL = false>
 GHD void cmcp(const Met& m, double rho, double press, double eps, double cs2, double W, double b2,
               const double vt[3], int d, double& cm, double& cp) {
     const double gdd = d == 0 ? m.gi[0] : (d == 1 ? m.gi[3] : m.gi[5]);
@@ -318,7 +337,9 @@ GHD void cmcp(const Met& m, double rho, double press, double eps, double cs2, do
     double w2 = vA2 + cs2 * (1. - vA2);
     double W2_1mw2 = W * W * (1. - w2);
     double A = (W2_1mw2 + w2) * ia2;
-    double Bc = 2. * ia2 * (m.beta[d] * w2 - vt[d] * W2_1mw2);
+    double Bc;
+    if constexpr (FL) Bc = 2. * (-(vt[d] * W2_1mw2));
+    else Bc = 2. * ia2 * (m.beta[d] * w2 - vt[d] * W2_1mw2);
     double g4dd = gdd - m.beta[d] * m.beta[d] * ia2;
     double C = vt[d] * vt[d] * W2_1mw2 * ia2 - w2 * g4dd;
     double disc = sqrt(fmax(0., Bc * Bc - 4. * A * C));
@@ -327,16 +348,21 @@ GHD void cmcp(const Met& m, double rho, double press, double eps, double cs2, do
     cm = fmin(l1, l2); cp = fmax(l1, l2);
 }
 // conserved D, tau, S_j, S* (undensitised)
+template <bool FL = false>
 GHD void conserved(const Met& m, double rho, double press, double eps, const double z[3], double s, double W,
                    double b2, const double b[4], double& D, double& tau, double S[3], double& Ss) {
     double rhoh = b2 + eps * rho + press + rho;
     D = W * rho;
     tau = W * W * rhoh - press - 0.5 * b2 - m.alp * m.alp * b[0] * b[0] - D;
     double zl[3], bsh[3], bl[3];
-    lower3(m.g, z[0], z[1], z[2], zl);
-    lower3(m.g, m.beta[0], m.beta[1], m.beta[2], bsh);
-    lower3(m.g, b[1], b[2], b[3], bl);
-    for (int j = 0; j < 3; ++j) S[j] = rhoh * W * zl[j] - m.alp * b[0] * (bl[j] + b[0] * bsh[j]);
+    lower3<FL>(m.g, z[0], z[1], z[2], zl);
+    lower3<FL>(m.g, b[1], b[2], b[3], bl);
+    if constexpr (FL) {
+        for (int j = 0; j < 3; ++j) S[j] = rhoh * W * zl[j] - b[0] * bl[j];
+    } else {
+        lower3(m.g, m.beta[0], m.beta[1], m.beta[2], bsh);
+        for (int j = 0; j < 3; ++j) S[j] = rhoh * W * zl[j] - m.alp * b[0] * (bl[j] + b[0] * bsh[j]);
+    }
     Ss = s * D;
 }
 

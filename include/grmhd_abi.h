@@ -279,6 +279,17 @@ grmhd_status grmhd_prims_to_conservs_batch(grmhd_ctx* ctx, int64_t n, const doub
 /* Statistics of the last c2p launch (host copy; synchronises the stream). */
 grmhd_status grmhd_c2p_stats(grmhd_ctx* ctx, grmhd_c2p_stats_t* out, grmhd_stream_t stream);
 
+/* Per-state evaluation counts of the batch c2p.  No counterpart in the
+ * reference (its c2p reports no iteration counts): it gives a c2p benchmark
+ * the distribution of the work per state (mean and percentiles) that the
+ * block-reduced grmhd_c2p_stats cannot.  With a device array set, every
+ * grmhd_c2p_batch / grmhd_c2p_batch_learned call of n states writes
+ * min(255, master-function + bracket evaluations of state s) into
+ * counts[s] (n bytes); null, the default, turns the output off.  The array
+ * is a kernel argument: a captured graph keeps the setting it was captured
+ * with. */
+grmhd_status grmhd_set_c2p_eval_counts(grmhd_ctx* ctx, uint8_t* counts_or_null);
+
 /* find_stable_timestep (src/evolution/find_stable_timestep.cpp:72-126):
  * dt = min over interior cells of cfl*dx/c_max, written to dt_dev. */
 grmhd_status grmhd_compute_dt(grmhd_ctx* ctx, const double* state, const double* aux, double cfl,
@@ -294,7 +305,10 @@ grmhd_status grmhd_find_stable_timestep(grmhd_ctx* ctx, const double* state, con
  * reset of the per-step c2p diagnostics, three substeps each followed by the
  * ghost fill and the c2p pass.  state/B/aux are updated in place; the two
  * stage buffers live in the context.  The kernels of the step are fused
- * (linear combination + sources + flux divergence + CT in one pass). */
+ * (linear combination + sources + flux divergence + CT in one pass).
+ * With GS EMFs the first substep reads the cell-centred E the context's last
+ * c2p pass left (grmhd_compute_auxiliaries or the previous step), as the
+ * reference reads its E_center field: evolve one state per context. */
 grmhd_status grmhd_rk3_step(grmhd_ctx* ctx, const double* dt_dev, double* state, double* const B[3],
                             double* aux, grmhd_stream_t stream);
 
