@@ -1546,6 +1546,7 @@ struct grmhd_ctx {
     // ghost is filled, its conserved (this list, after the cell gather).
     int4* copy_gather = nullptr;
     int64_t n_copy = 0;
+    int64_t n_copy_edge = 0;                  // the first n_copy_edge: cells with two or three ghost coordinates
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
     int64_t ncg_slab = 0;                     // ghost cells with one ghost coordinate (listed first)
     int64_t last_c2p_n = 0;
@@ -1934,7 +1935,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         // a flat metric and no radius-based excision its c2p inputs are those
         // of the cell with every outflow-exceeding coordinate moved to the first
         // ghost layer.  Only that cell solves; the others copy its outputs.
-        std::vector<int4> copyg;
+        std::vector<int4> copyg, copy_slab;   // copies beyond the face slabs first, then those in them
         std::vector<uint8_t> is_copy((size_t)(per * G.nq), 0);
         if (!fin && c->P.flat_metric && !c->P.excise_by_radius) {
             std::unordered_map<int32_t, int64_t> pos;   // local cell offset -> entry
@@ -1962,7 +1963,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                     if (it == pos.end() || !(src[(size_t)(q * per + it->second)].x & GSRC_OWN)) continue;
                     is_copy[(size_t)(q * per + x)] = 1;
                     src[(size_t)(q * per + x)] = make_int2((int)q | GSRC_COPY, roff);
-                    copyg.push_back(make_int4((int)q, off, (int)q, roff));
+                    (x < c->ncg_slab ? copy_slab : copyg).push_back(make_int4((int)q, off, (int)q, roff));
                 }
             }
         }
@@ -1989,6 +1990,8 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
             }
         c->n_own = (int64_t)own.size();
         c->n_own_gather = (int64_t)owng.size();
+        c->n_copy_edge = (int64_t)copyg.size();
+        copyg.insert(copyg.end(), copy_slab.begin(), copy_slab.end());
         c->n_copy = (int64_t)copyg.size();
         c->n_amr_cells = (int64_t)amr_c.size();
         c->n_amr_faces = (int64_t)amr_f.size();
@@ -2285,9 +2288,13 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
                                                                                  (int)ncells, wpb, a.cell_mode,
                                                                                  a.own_copies ? 1 : 0, ch);
     // every ghost: the copying outflow ghosts take the first-layer cell the
-    // launch above filled
+    // launch above filled.  Face slabs only (between RK stages): the copying
+    // ghosts outside the slabs too, which the next stage's ring faces beyond
+    // outflow faces read
     if (a.cell_mode == 1 && c->n_copy > 0)
         k_gather_list<<<dim3(nblk(c->n_copy), 1, ch.nch), 256, 0, s>>>(c->copy_gather, (int)c->n_copy, ch);
+    if (a.cell_mode == 3 && a.own_copies && a.slab_only && c->n_copy_edge > 0)
+        k_gather_list<<<dim3(nblk(c->n_copy_edge), 1, ch.nch), 256, 0, s>>>(c->copy_gather, (int)c->n_copy_edge, ch);
     if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
 }
 
