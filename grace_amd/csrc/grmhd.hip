@@ -156,7 +156,8 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
                                               const double* __restrict__ Bd, double* __restrict__ flux,
                                               double* __restrict__ vbar, int wpb,
                                               const uint8_t* __restrict__ only,
-                                              const uint8_t* __restrict__ ring_own) {
+                                              const uint8_t* __restrict__ ring_own,
+                                              const int32_t* __restrict__ push_nbr) {
     // wpb workgroups per block (see wg_pos).  UCT: the first
     // (n_D+1) n_T0 n_T1 threads take the faces whose transverse indices are
     // both interior (full fluxes), the next ones the faces with exactly one
@@ -168,7 +169,9 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     // transverse [0, N)).  ring_own (per block, bit 2 d + side for the six
     // block faces): when given, a ring face is computed only if its ghost
     // side lies beyond an outflow face of the instance; the others are
-    // interior faces of the neighbouring block, copied by k_ring_vbar.
+    // interior faces of the neighbouring block, copied by k_ring_vbar or,
+    // with push_nbr (the neighbour on each of the six block faces; the
+    // relation is symmetric), written by that block's face thread here.
     const WgPos wp = wg_pos(wpb);
     const int64_t q = wp.q;
     const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
@@ -334,9 +337,11 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     const int64_t nf = G.nface[D];
     double* V = vbar + q * 4 * nf + fo;
     constexpr int J0 = D == 0 ? 1 : 0, J1 = D == 2 ? 1 : 2;
+    double vb0 = 0.0, vb1 = 0.0;
     if (o.emf == 0) {  // UCT: HLL-averaged transverse vtilde + wave speeds, slots strided by nf
-        V[0] = hll_r(vtl[J0], vtr[J0], 0, 0, cmin, cmax, ihll);
-        V[nf] = hll_r(vtl[J1], vtr[J1], 0, 0, cmin, cmax, ihll);
+        vb0 = hll_r(vtl[J0], vtr[J0], 0, 0, cmin, cmax, ihll);
+        vb1 = hll_r(vtl[J1], vtr[J1], 0, 0, cmin, cmax, ihll);
+        V[0] = vb0; V[nf] = vb1;
         V[2 * nf] = cmin; V[3 * nf] = cmax;
         if (ring) return;
     }
@@ -416,6 +421,29 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
             V[0] = sg * hll_r(a, b, Bl[1], Br[1], cmin, cmax, ihll);
             a = Bl[0] * vtl[2] - Bl[2] * vtl[0]; b = Br[0] * vtr[2] - Br[2] * vtr[0];
             V[nf] = -sg * hll_r(a, b, Bl[0], Br[0], cmin, cmax, ihll);
+        }
+    }
+    if (push_nbr) {
+        // UCT ring-copy stages: this interior face is a ring face of the
+        // neighbour across each block face within ngz of it (transverse
+        // index shifted by one block extent)
+        const int ng = G.ng;
+        constexpr int T0 = D == 0 ? 1 : 0, T1 = D == 2 ? 1 : 2;
+        const int loc[3] = {i, j, k};
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int td = t == 0 ? T0 : T1;
+            const int nt = G.n[td], it = loc[td] - ng;
+#pragma unroll
+            for (int hi = 0; hi < 2; ++hi) {
+                if (hi ? it < nt - ng : it >= ng) continue;
+                const int32_t qt = push_nbr[q * 6 + 2 * td + hi];
+                if (qt < 0) continue;
+                int l2[3] = {loc[0], loc[1], loc[2]};
+                l2[td] = hi ? it - nt + ng : it + nt + ng;
+                double* T = vbar + (int64_t)qt * 4 * nf + fidx<D>(G, l2[0], l2[1], l2[2]);
+                T[0] = vb0; T[nf] = vb1; T[2 * nf] = cmin; T[3 * nf] = cmax;
+            }
         }
     }
 }
@@ -1530,6 +1558,9 @@ struct grmhd_ctx {
     // (the first stage and grmhd_advance_substep read the caller's ghosts and
     // compute every ring face)
     bool ring_copy_now = false;
+    // the neighbour relation of ring_nbr is symmetric (q's neighbour across a
+    // face has q across the opposite face): the flux kernel pushes ring values
+    bool ring_push = false;
     int warm_newton = 0;                      // grmhd_set_c2p_warm_start (0: off)
     int32_t* ghost_list = nullptr;
     int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
@@ -1845,6 +1876,13 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                 nbr[(size_t)q * 6 + sd] = (int32_t)(I.q0 + bc[0] + (int64_t)I.nb[0] * (bc[1] + (int64_t)I.nb[1] * bc[2]));
             }
         }
+        bool sym = true;
+        for (int64_t q = 0; q < g->nq && sym; ++q)
+            for (int sd = 0; sd < 6; ++sd) {
+                const int32_t p2 = nbr[(size_t)q * 6 + sd];
+                if (p2 >= 0 && nbr[(size_t)p2 * 6 + (sd ^ 1)] != (int32_t)q) { sym = false; break; }
+            }
+        c->ring_push = sym;
         if ((e = hipMalloc(&c->ring_nbr, sizeof(int32_t) * nbr.size())) != hipSuccess) return fail("ring neighbours", e);
         if ((e = hipMemcpy(c->ring_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail("ring neighbours copy", e);
@@ -2163,7 +2201,8 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
         // reconstructions load one or three values per side)
         bool pf = false;
         if constexpr (REC == 0) pf = c->tune.flux_prefetch[d] != 0 && !only;
-#define FLUXL(DD, PFV) k_flux<EOS, REC, FLAT, DD, PFV><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[DD], c->flux[DD], c->vbar[DD], wpb, od, ro)
+        const int32_t* pn = (ro && c->ring_push) ? c->ring_nbr : nullptr;
+#define FLUXL(DD, PFV) k_flux<EOS, REC, FLAT, DD, PFV><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[DD], c->flux[DD], c->vbar[DD], wpb, od, ro, pn)
         if constexpr (REC == 0) {
             if (pf) { if (d == 0) FLUXL(0, true); else if (d == 1) FLUXL(1, true); else FLUXL(2, true); }
             else { if (d == 0) FLUXL(0, false); else if (d == 1) FLUXL(1, false); else FLUXL(2, false); }
@@ -2171,7 +2210,7 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
             if (d == 0) FLUXL(0, false); else if (d == 1) FLUXL(1, false); else FLUXL(2, false);
         }
 #undef FLUXL
-        if (ro) launch_ring_vbar(c, d, s);
+        if (ro && !pn) launch_ring_vbar(c, d, s);
     }
 }
 template <typename EOS, bool FLAT>
