@@ -169,9 +169,9 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     // transverse [0, N)).  ring_own (per block, bit 2 d + side for the six
     // block faces): when given, a ring face is computed only if its ghost
     // side lies beyond an outflow face of the instance; the others are
-    // interior faces of the neighbouring block, copied by k_ring_vbar or,
-    // with push_nbr (the neighbour on each of the six block faces; the
-    // relation is symmetric), written by that block's face thread here.
+    // interior faces of the neighbouring block, whose face threads write
+    // them through push_nbr (the neighbour across each of the six block
+    // faces; with periodicity per dimension the relation is symmetric).
     const WgPos wp = wg_pos(wpb);
     const int64_t q = wp.q;
     const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
@@ -424,7 +424,7 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
         }
     }
     if (push_nbr) {
-        // UCT ring-copy stages: this interior face is a ring face of the
+        // stages with shared ring faces: this interior face is a ring face of the
         // neighbour across each block face within ngz of it (transverse
         // index shifted by one block extent)
         const int ng = G.ng;
@@ -446,65 +446,6 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
             }
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// UCT face values (HLL transverse velocities and wave speeds) of the ring
-// faces of direction D whose ghost side has a neighbouring block: the same
-// physical face is an interior face of that block (nbr[q][side]; the
-// periodic image across a periodic face), which k_flux computes from the
-// same inputs -- this block's ghost primitives and face B are copies of its
-// cells and faces -- so the four values are copied instead of recomputed.
-// Same enumeration of ring faces as k_flux; faces beyond an outflow face
-// (ring_own) are left to k_flux.  Needs every block extent >= ngz.
-// ---------------------------------------------------------------------------
-template <int D>
-__global__ __launch_bounds__(256) void k_ring_vbar(DevGrid G, const uint8_t* __restrict__ ring_own,
-                                                   const int32_t* __restrict__ nbr, double* vbar, int wpb) {
-    const WgPos wp = wg_pos(wpb);
-    const int64_t q = wp.q;
-    const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
-    const int ng = G.ng, Rn = G.n[D] + 1, g2 = 2 * ng;
-    constexpr int T0 = D == 0 ? 1 : 0, T1 = D == 2 ? 1 : 2;
-    const int n0 = G.n[T0], n1 = G.n[T1];
-    const int nA = Rn * g2 * n1, nB = Rn * n0 * g2;
-    if (e >= nA + nB) return;
-    int a, b, c, gl, td;   // normal, t0, t1 (interior-relative), ghost layer, its direction
-    if (e < nA) {
-        const int x = e;
-        td = T0;
-        if (D == 0) { a = x % Rn; const int r = x / Rn; gl = r % g2; c = r / g2; }
-        else if (D == 1) { gl = x % g2; const int r = x / g2; a = r % Rn; c = r / Rn; }
-        else { gl = x % g2; const int r = x / g2; c = r % n1; a = r / n1; }
-        b = gl < ng ? gl - ng : n0 + gl - ng;
-    } else {
-        const int x = e - nA;
-        td = T1;
-        if (D == 0) { a = x % Rn; const int r = x / Rn; b = r % n0; gl = r / n0; }
-        else if (D == 1) { b = x % n0; const int r = x / n0; a = r % Rn; gl = r / Rn; }
-        else { b = x % n0; const int r = x / n0; gl = r % g2; a = r / g2; }
-        c = gl < ng ? gl - ng : n1 + gl - ng;
-    }
-    const int side = 2 * td + (gl >= ng);
-    if ((ring_own[q] >> side) & 1) return;
-    const int64_t qs = nbr[q * 6 + side];
-    // the neighbour's index along td: one block extent over
-    int bs = b, cs = c;
-    if (td == T0) bs = gl < ng ? b + n0 : b - n0;
-    else cs = gl < ng ? c + n1 : c - n1;
-    auto at = [&](int aa, int bb, int cc) {
-        int i, j, k;
-        if (D == 0) { i = aa; j = bb; k = cc; }
-        else if (D == 1) { i = bb; j = aa; k = cc; }
-        else { i = bb; j = cc; k = aa; }
-        return fidx<D>(G, i + (D == 0 ? 0 : ng), j + (D == 1 ? 0 : ng), k + (D == 2 ? 0 : ng));
-    };
-    const int64_t nf = G.nface[D];
-    const int64_t fo = at(a + ng, b, c), fs = at(a + ng, bs, cs);
-    const double* S = vbar + qs * 4 * nf + fs;
-    double* T = vbar + q * 4 * nf + fo;
-    const double v0 = S[0], v1 = S[nf], v2 = S[2 * nf], v3 = S[3 * nf];
-    T[0] = v0; T[nf] = v1; T[2 * nf] = v2; T[3 * nf] = v3;
 }
 
 // ---------------------------------------------------------------------------
@@ -1548,7 +1489,7 @@ struct grmhd_ctx {
     DevStats* noalloc_part = nullptr;         // device address of `scratch` on this context's device
     unsigned long long* fatal_d = nullptr;    // fatal c2p cells since the last sync point (Opts::fatal_count)
     double* dt_scalar = nullptr;              // dt of the host-dt entry points
-    // ring faces copied from the neighbouring block (k_ring_vbar): per block
+    // ring faces written by the neighbouring block's face threads: per block
     // the face neighbours (-1 beyond an outflow face) and the mask of block
     // faces beyond which k_flux computes the ring itself; null when off
     int32_t* ring_nbr = nullptr;
@@ -1558,9 +1499,6 @@ struct grmhd_ctx {
     // (the first stage and grmhd_advance_substep read the caller's ghosts and
     // compute every ring face)
     bool ring_copy_now = false;
-    // the neighbour relation of ring_nbr is symmetric (q's neighbour across a
-    // face has q across the opposite face): the flux kernel pushes ring values
-    bool ring_push = false;
     int warm_newton = 0;                      // grmhd_set_c2p_warm_start (0: off)
     int32_t* ghost_list = nullptr;
     int2* ghost_src = nullptr;                // [nq][ncg + nfg0 + nfg1 + nfg2]
@@ -1767,6 +1705,14 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
             set_err("instance %d out of range", a); return GRMHD_ERR_INVALID;
         }
         for (int d = 0; d < 6; ++d) if (I.bc[d] != 0 && I.bc[d] != 1) { set_err("bc kind"); return GRMHD_ERR_INVALID; }
+        // periodicity is a property of a dimension (periodic_x|y|z of the
+        // brick connectivity, connectivity.cpp:52-69): a periodic face needs a
+        // periodic opposite face
+        for (int d = 0; d < 3; ++d)
+            if ((I.bc[2 * d] == GRMHD_BC_PERIODIC) != (I.bc[2 * d + 1] == GRMHD_BC_PERIODIC)) {
+                set_err("instance %d: the two faces of dimension %d must both be periodic or both not", a, d);
+                return GRMHD_ERR_INVALID;
+            }
         for (int d = 0; d < 3; ++d) if (!(I.idx[d] > 0)) { set_err("idx must be > 0"); return GRMHD_ERR_INVALID; }
         // the source terms' metric derivatives take idx[0] in every direction
         // (grmhd.hh:231-235, written for the reference's cubic cells)
@@ -1876,13 +1822,6 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                 nbr[(size_t)q * 6 + sd] = (int32_t)(I.q0 + bc[0] + (int64_t)I.nb[0] * (bc[1] + (int64_t)I.nb[1] * bc[2]));
             }
         }
-        bool sym = true;
-        for (int64_t q = 0; q < g->nq && sym; ++q)
-            for (int sd = 0; sd < 6; ++sd) {
-                const int32_t p2 = nbr[(size_t)q * 6 + sd];
-                if (p2 >= 0 && nbr[(size_t)p2 * 6 + (sd ^ 1)] != (int32_t)q) { sym = false; break; }
-            }
-        c->ring_push = sym;
         if ((e = hipMalloc(&c->ring_nbr, sizeof(int32_t) * nbr.size())) != hipSuccess) return fail("ring neighbours", e);
         if ((e = hipMemcpy(c->ring_nbr, nbr.data(), sizeof(int32_t) * nbr.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail("ring neighbours copy", e);
@@ -2077,6 +2016,11 @@ extern "C" grmhd_status grmhd_ctx_create_forest(const grmhd_params* p, const grm
     for (int d = 0; d < 3; ++d)
         if (f->nb[d] < 1 || !(f->idx[d] > 0)) { set_err("forest: nb must be >= 1 and idx > 0"); return GRMHD_ERR_INVALID; }
     for (int d = 0; d < 6; ++d) if (f->bc[d] != 0 && f->bc[d] != 1) { set_err("forest: bc kind"); return GRMHD_ERR_INVALID; }
+    for (int d = 0; d < 3; ++d)
+        if ((f->bc[2 * d] == GRMHD_BC_PERIODIC) != (f->bc[2 * d + 1] == GRMHD_BC_PERIODIC)) {
+            set_err("forest: the two faces of dimension %d must both be periodic or both not", d);
+            return GRMHD_ERR_INVALID;
+        }
     const int64_t nc = (int64_t)f->nb[0] * f->nb[1] * f->nb[2];
     if (nc > (1 << 24)) { set_err("forest: too many coarse blocks"); return GRMHD_ERR_INVALID; }
     if (f->n_refined < 0 || f->n_refined > nc || (f->n_refined > 0 && !f->refined)) {
@@ -2160,21 +2104,6 @@ static grmhd_status launch_check(const char* what) {
 }
 
 // ---- dispatch helpers -------------------------------------------------------
-// Ring faces of direction d copied from the neighbouring blocks, after every
-// block's k_flux of that direction (same stream).
-static void launch_ring_vbar(grmhd_ctx* c, int d, hipStream_t s) {
-    static const char* fam[3] = {"ring_x", "ring_y", "ring_z"};
-    Scope sc(c, fam[d], s);
-    const DevGrid& G = c->G;
-    const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
-    const int nring = (G.n[d] + 1) * 2 * G.ng * (G.n[t0] + G.n[t1]);
-    const int wpb = (int)nblk_bs(nring, 256);
-    const dim3 grid((unsigned)(wpb * G.nq));
-    if (d == 0) k_ring_vbar<0><<<grid, 256, 0, s>>>(G, c->ring_own, c->ring_nbr, c->vbar[0], wpb);
-    else if (d == 1) k_ring_vbar<1><<<grid, 256, 0, s>>>(G, c->ring_own, c->ring_nbr, c->vbar[1], wpb);
-    else k_ring_vbar<2><<<grid, 256, 0, s>>>(G, c->ring_own, c->ring_nbr, c->vbar[2], wpb);
-}
-
 // only != null: FOFC correction pass over the tagged faces with the options O
 // (donor cell + LLF); otherwise every face the later passes read.
 template <typename EOS, int REC, bool FLAT>
@@ -2201,7 +2130,7 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
         // reconstructions load one or three values per side)
         bool pf = false;
         if constexpr (REC == 0) pf = c->tune.flux_prefetch[d] != 0 && !only;
-        const int32_t* pn = (ro && c->ring_push) ? c->ring_nbr : nullptr;
+        const int32_t* pn = ro ? c->ring_nbr : nullptr;
 #define FLUXL(DD, PFV) k_flux<EOS, REC, FLAT, DD, PFV><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[DD], c->flux[DD], c->vbar[DD], wpb, od, ro, pn)
         if constexpr (REC == 0) {
             if (pf) { if (d == 0) FLUXL(0, true); else if (d == 1) FLUXL(1, true); else FLUXL(2, true); }
@@ -2210,7 +2139,6 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
             if (d == 0) FLUXL(0, false); else if (d == 1) FLUXL(1, false); else FLUXL(2, false);
         }
 #undef FLUXL
-        if (ro && !pn) launch_ring_vbar(c, d, s);
     }
 }
 template <typename EOS, bool FLAT>

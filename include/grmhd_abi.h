@@ -121,7 +121,9 @@ typedef struct { int32_t nx, ny, nz, ngz; int64_t nq; } grmhd_grid;
 
 /* A uniform Cartesian instance cut into nb[0]*nb[1]*nb[2] blocks, stored at
  * block indices q = q0 + bi + nb0*(bj + nb1*bk).  bc[] per face
- * (-x,+x,-y,+y,-z,+z).  idx = inverse cell spacings.  x0 = coordinates of
+ * (-x,+x,-y,+y,-z,+z); periodicity is per dimension, as the reference's
+ * periodic_x|y|z (connectivity.cpp:52-69): both faces of a dimension are
+ * periodic or neither is (GRMHD_ERR_INVALID otherwise).  idx = inverse cell spacings.  x0 = coordinates of
  * the lower corner of the instance's interior (the reference's per-block
  * origin qx, cartesian_coordinate_systems.hh:56-66), used only for cell
  * radii (excise_by_radius).  With a curved metric the spacing must be equal

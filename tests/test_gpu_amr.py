@@ -210,6 +210,9 @@ def test_forest_rejects_bad_input(dev):
     g = ga.Grid(8, 8, 8, 3, wl.grid.nq)
     with pytest.raises(RuntimeError):
         ga.Context(p, g, wl.instances, forest=wl.forest)
+    f = ga.make_forest((2, 2, 2), (1, 0, 1, 1, 1, 1), (16.0, 16.0, 16.0), [0])   # one-sided periodicity
+    with pytest.raises(RuntimeError, match="both be periodic or both not"):
+        ga.Context(p, wl.grid, wl.instances, forest=f)
 
 
 def _torus_forest_small():

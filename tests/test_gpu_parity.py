@@ -390,6 +390,19 @@ def test_gpu_rk3_periodic(dev, nb):
     assert np.array_equal(got[2][:, ga.C2P_ERR], ref[2][:, ga.C2P_ERR])
 
 
+def test_gpu_one_sided_periodic_face_rejected(dev):
+    """Periodicity is a property of a dimension in the reference (periodic_x|y|z
+    of the brick connectivity, connectivity.cpp:52-69): a periodic face
+    opposite an outflow face is refused, for block grids and forests."""
+    wl = W.blast(nb=(2, 2, 2), n=16)
+    spec = dict(wl.inst_specs[0])
+    spec["bc"] = (ga.BC_PERIODIC, ga.BC_OUTFLOW, ga.BC_OUTFLOW, ga.BC_OUTFLOW, ga.BC_PERIODIC, ga.BC_PERIODIC)
+    with pytest.raises(RuntimeError, match="both be periodic or both not"):
+        ga.Context(wl.params, wl.grid, ga.make_instances([spec]))
+    spec["bc"] = (ga.BC_PERIODIC, ga.BC_PERIODIC, ga.BC_OUTFLOW, ga.BC_OUTFLOW, ga.BC_PERIODIC, ga.BC_PERIODIC)
+    ga.Context(wl.params, wl.grid, ga.make_instances([spec]))
+
+
 def test_gpu_rk3_graph_matches_eager(dev):
     import torch
     wl = small_blast()

@@ -1,8 +1,8 @@
-"""UCT ring values between RK stages (grmhd.hip k_flux push_nbr, k_ring_vbar):
-the pushes of the interior faces within ngz of a block face write exactly the
-ring faces that the copy kernel fills -- faces with one transverse index in
-the ghost layers whose side has a neighbour -- each from the same source face
-and once, whenever the neighbour relation is symmetric.  The index maps are
+"""UCT ring values between RK stages (grmhd.hip k_flux push_nbr): the pushes
+of the interior faces within ngz of a block face write exactly the ring faces
+a copy from the neighbouring block would fill -- faces with one transverse
+index in the ghost layers whose side has a neighbour -- each from the same
+source face and once, whenever the neighbour relation is symmetric.  The index maps are
 restated here from the kernels (block-local face indices, the six-neighbour
 table with periodic wrap) and compared on outflow, periodic, self-periodic
 and mixed grids."""
@@ -40,7 +40,7 @@ def neighbours(insts):
 
 
 def copies(n, ng, nq, nbr, D):
-    """ring face -> source face, as k_ring_vbar copies them"""
+    """ring face -> the neighbour's interior face it equals"""
     N = [n + 2 * ng] * 3
     T0, T1 = (1 if D == 0 else 0), (1 if D == 2 else 2)
     want = {}
@@ -107,7 +107,17 @@ def test_ring_pushes_fill_what_the_copies_fill(n, ng, insts):
         assert pushes(n, ng, nq, nbr, D) == copies(n, ng, nq, nbr, D)
 
 
-def test_asymmetric_neighbours_are_detected():
-    # a periodic face opposite an outflow face: the context keeps the copies
-    nq, nbr = neighbours([((1, 2, 3), 0, (1, 0, 0, 1, 1, 1))])
+def test_per_dimension_periodicity_gives_symmetric_neighbours():
+    # the contexts accept periodicity per dimension only (a periodic face
+    # needs a periodic opposite face, as the reference's brick connectivity):
+    # then q's neighbour across a face has q across the opposite face, which
+    # the pushes rely on; a one-sided periodic face would break it
+    for px in (0, 1):
+        for py in (0, 1):
+            for pz in (0, 1):
+                for nb in ((1, 1, 1), (2, 1, 3), (3, 2, 2)):
+                    nq, nbr = neighbours([(nb, 0, (px, px, py, py, pz, pz))])
+                    assert all(nbr[(q, s)] < 0 or nbr[(nbr[(q, s)], s ^ 1)] == q
+                               for q in range(nq) for s in range(6))
+    nq, nbr = neighbours([((1, 2, 3), 0, (1, 0, 0, 0, 1, 1))])
     assert not all(nbr[(q, s)] < 0 or nbr[(nbr[(q, s)], s ^ 1)] == q for q in range(nq) for s in range(6))
