@@ -794,6 +794,126 @@ __global__ __launch_bounds__(256) void k_update(DevGrid G, UpdArgs a, int fd_ord
     }
 }
 
+// Flat-metric update over pairs of sites along x (i0, i0 + 1 with i0 even).
+// With even extents N[d] the cell arrays, the y- and z-face arrays (fluxes,
+// face B) have even row widths, so the two values of a pair sit in one
+// aligned 16-byte word and move with one access; the x-face and edge arrays
+// (odd row widths) keep 8-byte accesses, three x faces serving the pair's
+// two cells.  Per cell and face the arithmetic is that of k_update<true>,
+// operation for operation.  All loads of a pair are issued before its first
+// store (the stores may alias the loads: the third RK stage updates in
+// place), which takes 194 VGPRs, 2 waves per SIMD: measured faster in the
+// RK3 graph than k_update<true> and than a variant storing each variable as
+// it completes (48 VGPRs, 8 waves).  pairs = pairs per row; wpb workgroups
+// per block.
+__global__ __launch_bounds__(256, 2) void k_update_pairs(DevGrid G, UpdArgs a, int pairs, int wpb) {
+    const int R1 = G.n[1] + 1, R2 = G.n[2] + 1;
+    const WgPos wp = wg_pos(wpb);
+    const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
+    if (r >= pairs * R1 * R2) return;
+    const int64_t q = wp.q;
+    int kz, rxy, jy, px;
+    udivmod(r, pairs * R1, kz, rxy);
+    udivmod(rxy, pairs, jy, px);
+    const int ng = G.ng;
+    const int i0 = (ng & ~1) + 2 * px, j = ng + jy, k = ng + kz;
+    const DevInst& I = a.inst[a.blk[q].inst];
+    const double ix = I.idx[0], iy = I.idx[1], iz = I.idx[2];
+    const double dtf = (*a.dt) * a.dtfact;
+    const int ilo = ng, ihi = G.n[0] + ng;             // cells [ilo, ihi), x faces [ilo, ihi]
+    const bool c0 = i0 >= ilo && i0 < ihi, c1 = i0 + 1 >= ilo && i0 + 1 < ihi;
+    const bool cj = j < G.n[1] + ng, ck = k < G.n[2] + ng;
+    const int64_t nc = G.ncell;
+    typedef HIP_vector_type<double, 2> d2;
+    if (a.part != 2 && cj && ck && (c0 || c1)) {
+        const int64_t c = cidx(G, i0, j, k);
+        double* Nn = a.nst + q * (int64_t)a.nnv * nc + c;
+        const double* An = a.ast + q * (int64_t)a.anv * nc + c;
+        const double* Bn = a.bst ? a.bst + q * (int64_t)a.bnv * nc + c : nullptr;
+        const int64_t f0 = fidx<0>(G, i0, j, k), f1 = fidx<1>(G, i0, j, k), f2 = fidx<2>(G, i0, j, k);
+        const double* F0 = a.flux[0] + q * NHRSC * G.nface[0];
+        const double* F1 = a.flux[1] + q * NHRSC * G.nface[1];
+        const double* F2 = a.flux[2] + q * NHRSC * G.nface[2];
+        const int64_t s1 = G.N[0], s2 = (int64_t)G.N[0] * G.N[1];
+        double u0[NHRSC], u1[NHRSC];
+#pragma unroll
+        for (int v = 0; v < NHRSC; ++v) {
+            const d2 x = *reinterpret_cast<const d2*>(An + v * nc);
+            if (a.cb == 0.0) { u0[v] = x.x; u1[v] = x.y; }
+            else {
+                const d2 y = *reinterpret_cast<const d2*>(Bn + v * nc);
+                u0[v] = a.ca * x.x + a.cb * y.x; u1[v] = a.ca * x.y + a.cb * y.y;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NHRSC; ++v) {
+            const double* X = F0 + v * G.nface[0] + f0;
+            const double xa = X[0], xb = X[1], xc = X[2];
+            const d2 ya = *reinterpret_cast<const d2*>(F1 + v * G.nface[1] + f1);
+            const d2 yb = *reinterpret_cast<const d2*>(F1 + v * G.nface[1] + f1 + s1);
+            const d2 za = *reinterpret_cast<const d2*>(F2 + v * G.nface[2] + f2);
+            const d2 zb = *reinterpret_cast<const d2*>(F2 + v * G.nface[2] + f2 + s2);
+            {
+                double dFx = (xa - xb) * ix, dFy = (ya.x - yb.x) * iy, dFz = (za.x - zb.x) * iz;
+                u0[v] += dtf * ((dFx + dFy) + dFz);
+            }
+            {
+                double dFx = (xb - xc) * ix, dFy = (ya.y - yb.y) * iy, dFz = (za.y - zb.y) * iz;
+                u1[v] += dtf * ((dFx + dFy) + dFz);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NHRSC; ++v) {
+            if (c0 && c1) *reinterpret_cast<d2*>(Nn + v * nc) = d2{u0[v], u1[v]};
+            else if (c0) Nn[v * nc] = u0[v];
+            else Nn[v * nc + 1] = u1[v];
+        }
+    }
+    if (a.part == 1) return;
+    const double* E = a.emf + q * 3 * G.nedge;
+    auto Ex = [&](int ii, int jj, int kk) { return E[eidx(G, ii, jj, kk)]; };
+    auto Ey = [&](int ii, int jj, int kk) { return E[G.nedge + eidx(G, ii, jj, kk)]; };
+    auto Ez = [&](int ii, int jj, int kk) { return E[2 * G.nedge + eidx(G, ii, jj, kk)]; };
+    if (cj && ck) {  // x faces i0, i0 + 1 within [ilo, ihi]
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int i = i0 + t;
+            if (i < ilo || i > ihi) continue;
+            int64_t f = q * G.nface[0] + fidx<0>(G, i, j, k);
+            double b = (a.cb == 0.0) ? a.aB[0][f] : a.ca * a.aB[0][f] + a.cb * a.bB[0][f];
+            a.nB[0][f] = b + dtf * ((Ey(i, j, k + 1) - Ey(i, j, k)) * iz + (Ez(i, j, k) - Ez(i, j + 1, k)) * iy);
+        }
+    }
+    if (ck && (c0 || c1)) {  // y faces (cells' x positions, j in [ng, n+ng])
+        const int64_t f = q * G.nface[1] + fidx<1>(G, i0, j, k);
+        d2 b = *reinterpret_cast<const d2*>(a.aB[1] + f);
+        if (a.cb != 0.0) {
+            const d2 bb = *reinterpret_cast<const d2*>(a.bB[1] + f);
+            b = d2{a.ca * b.x + a.cb * bb.x, a.ca * b.y + a.cb * bb.y};
+        }
+        const double ez0 = Ez(i0, j, k), ez1 = Ez(i0 + 1, j, k), ez2 = Ez(i0 + 2, j, k);
+        const double n0v = b.x + dtf * ((ez1 - ez0) * ix + (Ex(i0, j, k) - Ex(i0, j, k + 1)) * iz);
+        const double n1v = b.y + dtf * ((ez2 - ez1) * ix + (Ex(i0 + 1, j, k) - Ex(i0 + 1, j, k + 1)) * iz);
+        if (c0 && c1) *reinterpret_cast<d2*>(a.nB[1] + f) = d2{n0v, n1v};
+        else if (c0) a.nB[1][f] = n0v;
+        else a.nB[1][f + 1] = n1v;
+    }
+    if (cj && (c0 || c1)) {  // z faces
+        const int64_t f = q * G.nface[2] + fidx<2>(G, i0, j, k);
+        d2 b = *reinterpret_cast<const d2*>(a.aB[2] + f);
+        if (a.cb != 0.0) {
+            const d2 bb = *reinterpret_cast<const d2*>(a.bB[2] + f);
+            b = d2{a.ca * b.x + a.cb * bb.x, a.ca * b.y + a.cb * bb.y};
+        }
+        const double n0v = b.x + dtf * ((Ex(i0, j + 1, k) - Ex(i0, j, k)) * iy + (Ey(i0, j, k) - Ey(i0 + 1, j, k)) * ix);
+        const double n1v =
+            b.y + dtf * ((Ex(i0 + 1, j + 1, k) - Ex(i0 + 1, j, k)) * iy + (Ey(i0 + 1, j, k) - Ey(i0 + 2, j, k)) * ix);
+        if (c0 && c1) *reinterpret_cast<d2*>(a.nB[2] + f) = d2{n0v, n1v};
+        else if (c0) a.nB[2][f] = n0v;
+        else a.nB[2][f + 1] = n1v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Ghost gather.  Index space per block: [cells | x faces | y faces | z faces].
 // Each ghost element copies the element at the same global position (clamped
@@ -2199,6 +2319,15 @@ static void launch_emf(grmhd_ctx* c, const double* const B[3], hipStream_t s) {
         }
     });
 }
+// k_update_pairs needs an even row width (the cell and y/z-face arrays then
+// put a pair's two values in one aligned 16-byte word) and 16-byte aligned
+// arrays
+static bool update_pairs_ok(const DevGrid& G, const UpdArgs& a) {
+    auto al = [](const void* p) { return p == nullptr || ((uintptr_t)p & 15) == 0; };
+    bool ok = (G.N[0] % 2) == 0 && al(a.nst) && al(a.ast) && al(a.bst);
+    for (int d = 1; d < 3; ++d) ok = ok && al(a.flux[d]) && al(a.nB[d]) && al(a.aB[d]) && al(a.bB[d]);
+    return ok;
+}
 static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     Scope sc(c, "update", s);
     const DevGrid& G = c->G;
@@ -2206,6 +2335,12 @@ static void launch_update(grmhd_ctx* c, UpdArgs& a, hipStream_t s) {
     for (int d = 0; d < 3; ++d) a.flux[d] = c->flux[d];
     a.emf = c->emf;
     a.mder = (c->mder && a.mstate == c->mder_src && a.mnv == NSTATE) ? c->mder : nullptr;
+    if (c->P.flat_metric && update_pairs_ok(G, a)) {
+        const int pairs = (((G.n[0] + G.ng) & ~1) - (G.ng & ~1)) / 2 + 1;
+        const int wpb = (int)nblk_bs((int64_t)pairs * (G.n[1] + 1) * (G.n[2] + 1), c->tune.update);
+        k_update_pairs<<<dim3((unsigned)(wpb * G.nq)), c->tune.update, 0, s>>>(G, a, pairs, wpb);
+        return;
+    }
     site_launch(G, c->tune.update, [&](dim3 g, int wpb) {
         if (c->P.flat_metric) k_update<true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
         else if (a.mder) k_update<false, true><<<g, c->tune.update, 0, s>>>(G, a, c->P.fd_order, wpb);
