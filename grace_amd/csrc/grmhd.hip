@@ -1970,7 +1970,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                     }
             if (fd < 0) {
                 // the cells with one ghost coordinate (the six face slabs) first:
-                // between RK stages with ring copies only they are read
+                // between RK stages with shared ring faces only they are read
                 c->ncg = cnt;
                 auto nghost = [&](int32_t off) {
                     const int loc[3] = {off % E[0], (off / E[0]) % E[1], off / (E[0] * E[1])};
