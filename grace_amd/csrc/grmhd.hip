@@ -137,6 +137,36 @@ GHD void load_metric(Met& m, const double* __restrict__ ms, int64_t ncell, int64
     m.init();
 }
 
+// Metric of the face below cell c0 along stride st: 4-point interpolation of
+// the 10 Cowling components (face_interp_4, grmhd_helpers.hh:360-380), their
+// inverse and determinant, and the lapse test of the excision donor
+// override (the lapse of either neighbouring cell at or below alp_ex).
+// Shared by the flux kernel and the face-metric cache, so both give the
+// same bits.
+GHD void face_metric(const double* __restrict__ M, int64_t nc, int64_t c0, int64_t st, double alp_ex, Met& mf,
+                     bool& excise) {
+    double v[10];
+    const int comps[10] = {GXX_, GXY_, GXZ_, GYY_, GYZ_, GZZ_, BETAX_, BETAY_, BETAZ_, ALP_};
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+        const double* P = M + comps[a] * nc + c0;
+        double si = P[-st] + P[0];
+        double so = P[-2 * st] + P[st];
+        v[a] = 0.5625 * si + (-0.0625) * so;
+    }
+    for (int a = 0; a < 6; ++a) mf.g[a] = v[a];
+    for (int a = 0; a < 3; ++a) mf.beta[a] = v[6 + a];
+    mf.alp = v[9];
+    mf.init();
+    double alpL = M[ALP_ * nc + c0 - st], alpR = M[ALP_ * nc + c0];
+    excise = (alpL <= alp_ex) || (alpR <= alp_ex);
+}
+// Face-metric cache (grmhd_set_static_metric): per face of direction D the
+// values the flux kernel reads -- g (6), the diagonal inverse component of
+// D, beta (3), alpha, sqrt(gamma), the excision flag -- as FMC arrays of
+// nface[D] per block.
+constexpr int FMC = 13;
+
 // ---------------------------------------------------------------------------
 // Face flux kernel
 // ---------------------------------------------------------------------------
@@ -157,7 +187,8 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
                                               double* __restrict__ vbar, int wpb,
                                               const uint8_t* __restrict__ only,
                                               const uint8_t* __restrict__ ring_own,
-                                              const int32_t* __restrict__ push_nbr) {
+                                              const int32_t* __restrict__ push_nbr,
+                                              const double* __restrict__ fmc) {
     // wpb workgroups per block (see wg_pos).  UCT: the first
     // (n_D+1) n_T0 n_T1 threads take the faces whose transverse indices are
     // both interior (full fluxes), the next ones the faces with exactly one
@@ -243,23 +274,19 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
     bool excise_override = false;
     if constexpr (FLAT) {
         mf.flat();
+    } else if (fmc) {
+        // the face-metric cache: only the inverse component this direction
+        // uses (cmcp, the LLF speed limit) is kept
+        const int64_t nfd = G.nface[D];
+        const double* FM = fmc + q * FMC * nfd + fidx<D>(G, i, j, k);
+        for (int a = 0; a < 6; ++a) { mf.g[a] = FM[a * nfd]; mf.gi[a] = 0.0; }
+        mf.gi[D == 0 ? 0 : (D == 1 ? 3 : 5)] = FM[6 * nfd];
+        for (int a = 0; a < 3; ++a) mf.beta[a] = FM[(7 + a) * nfd];
+        mf.alp = FM[10 * nfd];
+        mf.sqrtg = FM[11 * nfd];
+        excise_override = FM[12 * nfd] != 0.0;
     } else {
-        const double* M = mstate + q * (int64_t)mnv * nc;
-        double v[10];
-        const int comps[10] = {GXX_, GXY_, GXZ_, GYY_, GYZ_, GZZ_, BETAX_, BETAY_, BETAZ_, ALP_};
-#pragma unroll
-        for (int a = 0; a < 10; ++a) {
-            const double* P = M + comps[a] * nc + c0;
-            double si = P[-st] + P[0];
-            double so = P[-2 * st] + P[st];
-            v[a] = 0.5625 * si + (-0.0625) * so;
-        }
-        for (int a = 0; a < 6; ++a) mf.g[a] = v[a];
-        for (int a = 0; a < 3; ++a) mf.beta[a] = v[6 + a];
-        mf.alp = v[9];
-        mf.init();
-        double alpL = M[ALP_ * nc + c0 - st], alpR = M[ALP_ * nc + c0];
-        excise_override = (alpL <= o.ex.alp_ex) || (alpR <= o.ex.alp_ex);
+        face_metric(mstate + q * (int64_t)mnv * nc, nc, c0, st, o.ex.alp_ex, mf, excise_override);
     }
     // reconstruction of RHO Z(3) YE THERMO ENT B(3)
     const int th_aux = o.thermo ? PRESS_ : TEMP_;
@@ -677,6 +704,37 @@ GHD void metric_ders(const double* M, int64_t nc, const int64_t sd[3], double ix
         for (int cc = 0; cc < 3; ++cc) db[cc + 3 * d] = fdder(M + (BETAX_ + cc) * nc, sd[d], ix, fd_order);
     for (int d = 0; d < 3; ++d)
         for (int b = 0; b < 6; ++b) dg[b + 6 * d] = fdder(M + (GXX_ + b) * nc, sd[d], ix, fd_order);
+}
+
+// The face-metric cache of direction D for every face whose 4-point stencil
+// lies in the block array (normal index [2, N_D - 2]; wpb workgroups per
+// block, see wg_pos).
+template <int D>
+__global__ __launch_bounds__(256) void k_face_metric(DevGrid G, const double* __restrict__ mstate, int mnv,
+                                                     double alp_ex, double* __restrict__ out, int wpb) {
+    const WgPos wp = wg_pos(wpb);
+    const int64_t q = wp.q;
+    const int e = wp.tile * (int)blockDim.x + (int)threadIdx.x;
+    const int64_t nf = G.nface[D];
+    if (e >= nf) return;
+    const int E0 = G.N[0] + (D == 0), E1 = G.N[1] + (D == 1);
+    int r, i, j, k;
+    udivmod(e, E0, r, i);
+    udivmod(r, E1, k, j);
+    const int a = D == 0 ? i : (D == 1 ? j : k);
+    if (a < 2 || a > G.N[D] - 2) return;
+    const int64_t nc = G.ncell;
+    const int64_t st = D == 0 ? 1 : (D == 1 ? G.N[0] : (int64_t)G.N[0] * G.N[1]);
+    Met mf;
+    bool ex;
+    face_metric(mstate + q * (int64_t)mnv * nc, nc, cidx(G, i, j, k), st, alp_ex, mf, ex);
+    double* O = out + q * FMC * nf + e;
+    for (int b = 0; b < 6; ++b) O[b * nf] = mf.g[b];
+    O[6 * nf] = mf.gi[D == 0 ? 0 : (D == 1 ? 3 : 5)];
+    for (int b = 0; b < 3; ++b) O[(7 + b) * nf] = mf.beta[b];
+    O[10 * nf] = mf.alp;
+    O[11 * nf] = mf.sqrtg;
+    O[12 * nf] = ex ? 1.0 : 0.0;
 }
 
 // The 30 metric derivatives of every interior cell, [q][30][k][j][i].
@@ -1585,6 +1643,7 @@ struct grmhd_ctx {
     grmhd_mlp* mlp = nullptr;                 // learned c2p guess for the cell passes (not owned)
     MlpDev gmlp{};                            // its kernel arguments in the captured graph
     double* mder = nullptr;                   // cached derivatives of the static metric (grmhd_set_static_metric)
+    double* fmet[3] = {nullptr, nullptr, nullptr};   // cached y/z face metrics (FMC per face), same source
     const double* mder_src = nullptr;         // the state array they were taken from
     grmhd_params P;
     DevGrid G;
@@ -1747,6 +1806,7 @@ static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather); F(c->copy_gather);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
+    F(c->fmet[0]); F(c->fmet[1]); F(c->fmet[2]);
     F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->amr_faces_mid); F(c->rflux); F(c->remf);
     F(c->vcm); F(c->fatal_d); F(c->dt_scalar); F(c->ring_nbr); F(c->ring_own);
     for (int d = 0; d < 3; ++d) { F(c->flux[d]); F(c->vbar[d]); F(c->B1[d]); F(c->B2[d]); }
@@ -2251,7 +2311,10 @@ static void launch_flux3(grmhd_ctx* c, const EOS& eos, const double* aux, const 
         bool pf = false;
         if constexpr (REC == 0) pf = c->tune.flux_prefetch[d] != 0 && !only;
         const int32_t* pn = ro ? c->ring_nbr : nullptr;
-#define FLUXL(DD, PFV) k_flux<EOS, REC, FLAT, DD, PFV><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[DD], c->flux[DD], c->vbar[DD], wpb, od, ro, pn)
+        // (x faces interpolate from four adjacent cells of a row, which is
+        // cheaper than streaming 13 cached values: cache for y and z only)
+        const double* fm = (!FLAT && d > 0 && c->fmet[d] && ms == c->mder_src && mnv == NSTATE) ? c->fmet[d] : nullptr;
+#define FLUXL(DD, PFV) k_flux<EOS, REC, FLAT, DD, PFV><<<grid, bs, 0, s>>>(G, eos, O, aux, ms, mnv, B[DD], c->flux[DD], c->vbar[DD], wpb, od, ro, pn, fm)
         if constexpr (REC == 0) {
             if (pf) { if (d == 0) FLUXL(0, true); else if (d == 1) FLUXL(1, true); else FLUXL(2, true); }
             else { if (d == 0) FLUXL(0, false); else if (d == 1) FLUXL(1, false); else FLUXL(2, false); }
@@ -2754,6 +2817,17 @@ extern "C" grmhd_status grmhd_set_static_metric(grmhd_ctx* c, const double* stat
     }
     k_metric_der<<<nblk(n), 256, 0, (hipStream_t)stream>>>(G, c->inst_d, c->blk_d, state, NSTATE, c->P.fd_order,
                                                             c->mder);
+    for (int d = 1; d < 3; ++d) {
+        if (!c->fmet[d]) {
+            hipError_t e = hipMalloc(&c->fmet[d], sizeof(double) * FMC * G.nface[d] * G.nq);
+            if (e != hipSuccess) { c->fmet[d] = nullptr; set_err("face metric cache: %s", hipGetErrorString(e)); return GRMHD_ERR_ALLOC; }
+        }
+        const int wpb = (int)nblk(G.nface[d]);
+        const dim3 grid((unsigned)(wpb * G.nq));
+        hipStream_t s = (hipStream_t)stream;
+        if (d == 1) k_face_metric<1><<<grid, 256, 0, s>>>(G, state, NSTATE, c->O.ex.alp_ex, c->fmet[1], wpb);
+        else k_face_metric<2><<<grid, 256, 0, s>>>(G, state, NSTATE, c->O.ex.alp_ex, c->fmet[2], wpb);
+    }
     c->mder_src = state;
     return launch_check("metric_der");
 }

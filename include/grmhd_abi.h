@@ -321,10 +321,12 @@ grmhd_status grmhd_rk3_step_graph(grmhd_ctx* ctx, const double* dt_dev, double* 
 
 /* Cowling: the metric in `state` is static, so the 30 finite-difference
  * metric derivatives of the geometric source terms (grmhd.hh:232-235,
- * recomputed by the reference in every substep) can be taken once:
- * enable = 1 computes them for the interior cells of `state` into a context
- * buffer (30 doubles per interior cell), and the update kernels of calls
- * whose metric source is that same array then read them instead (results
+ * recomputed by the reference in every substep) and the interpolated face
+ * metrics of the flux kernels (grmhd_helpers.hh:360-419, every face of every
+ * substep in the reference) can be taken once: enable = 1 computes them for
+ * `state` into context buffers (30 doubles per interior cell, 13 per y and
+ * z face), and the update and flux kernels of calls whose metric
+ * source is that same array then read them instead (results
  * bit-identical).  The cache is not refreshed automatically: call again
  * after changing the metric in that array, or enable = 0 to drop it.
  * No effect for flat_metric.  Invalidates a captured RK3 graph. */
