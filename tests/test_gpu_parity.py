@@ -356,6 +356,20 @@ def check_close(wl, ref, got, tol, atm_tol=None):
         assert e < tol, (d, e)
 
 
+@pytest.mark.parametrize("n", [15, 12])
+def test_gpu_rk3_step_blast_other_block_sizes(dev, n):
+    """Blocks of 15^3 (odd row width 21: the update takes its one-site
+    kernel instead of the paired one) and 12^3 (even, 18): two RK3 steps
+    against the oracle, primitives in every cell."""
+    wl = W.blast(nb=(2, 2, 2), n=n)
+    say(f"[rk3] blast 2x2x2 blocks of {n}^3, two steps")
+    ref, got, _ = run_rk3_pair(wl, dev, steps=2)
+    check_close(wl, ref, got, 1e-12)
+    for v in (ga.RHO, ga.PRESS, ga.ZVECX, ga.ZVECY, ga.ZVECZ):
+        assert linf_rel(got[2][:, v], ref[2][:, v]) < 1e-11, v
+    assert np.array_equal(got[2][:, ga.C2P_ERR], ref[2][:, ga.C2P_ERR])
+
+
 def test_gpu_rk3_step_blast(dev):
     wl = small_blast()
     say("[rk3] blast one full step")
