@@ -266,9 +266,32 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
         if (only && (a < ng || a > ng + G.n[D] || !only[q * G.nface[D] + fidx<D>(G, i, j, k)])) return;
     }
     const int64_t nc = G.ncell;
-    const double* A = aux + q * NAUX * nc;
     const int64_t c0 = cidx(G, i, j, k);
     const int64_t st = D == 0 ? 1 : (D == 1 ? G.N[0] : (int64_t)G.N[0] * G.N[1]);
+    const double* A = aux + q * NAUX * nc;
+    // flat metric: the six stencil cells along D (var 0).  With push_nbr (the
+    // second and third RK stages, whose gathers leave the face slabs alone)
+    // a full face takes a cell beyond a block face from the neighbouring
+    // block's interior (normal index shifted by one block extent); beyond an
+    // outflow face (no neighbour) and on ring faces, from its own ghosts,
+    // which run their own c2p or copy one that does.  (The curved kernels,
+    // which never get push_nbr, keep one row pointer per variable: six
+    // pointers cost them 7% on x faces.)
+    const double* Pt[6];
+    if constexpr (FLAT) {
+        const int an = D == 0 ? i : (D == 1 ? j : k), nD = G.n[D];
+        int32_t nl = -1, nh = -1;
+        if (push_nbr && !ring) { nl = push_nbr[q * 6 + 2 * D]; nh = push_nbr[q * 6 + 2 * D + 1]; }
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            const int ap = an + t - 3;
+            int64_t qb = q;
+            int sh = 0;
+            if (ap < G.ng && nl >= 0) { qb = nl; sh = nD; }
+            else if (ap >= nD + G.ng && nh >= 0) { qb = nh; sh = -nD; }
+            Pt[t] = aux + qb * NAUX * nc + c0 + (int64_t)(t - 3 + sh) * st;
+        }
+    }
 
     Met mf;
     bool excise_override = false;
@@ -298,9 +321,15 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
 #pragma unroll
         for (int v = 0; v < 10; ++v) {
             if (v == 7 + D) continue;
-            const double* P = A + ridx[v] * nc + c0;
+            if constexpr (FLAT) {
+                const int64_t ov = ridx[v] * nc;
 #pragma unroll
-            for (int t = 0; t < 6; ++t) us[v][t] = P[(t - 3) * st];
+                for (int t = 0; t < 6; ++t) us[v][t] = Pt[t][ov];
+            } else {
+                const double* P = A + ridx[v] * nc + c0;
+#pragma unroll
+                for (int t = 0; t < 6; ++t) us[v][t] = P[(t - 3) * st];
+            }
         }
 #pragma unroll
         for (int v = 0; v < 10; ++v) {
@@ -317,10 +346,15 @@ __global__ GRMHD_FLUX_BOUNDS void k_flux(DevGrid G, EOS eos, Opts o, const doubl
             // by the face value below, and a ring face (UCT velocities and wave
             // speeds only) reads neither Ye nor the entropy.
             if (v == 7 + D || (ring && (v == 4 || v == 6))) { pL[rloc[v]] = 0.0; pR[rloc[v]] = 0.0; continue; }
-            const double* P = A + ridx[v] * nc + c0;
             double uL, uR;
-            if (excise_override) { uL = P[-st]; uR = P[0]; }
-            else recon<REC>(P[-3 * st], P[-2 * st], P[-st], P[0], P[st], P[2 * st], uL, uR);
+            if constexpr (FLAT) {
+                const int64_t ov = ridx[v] * nc;
+                recon<REC>(Pt[0][ov], Pt[1][ov], Pt[2][ov], Pt[3][ov], Pt[4][ov], Pt[5][ov], uL, uR);
+            } else {
+                const double* P = A + ridx[v] * nc + c0;
+                if (excise_override) { uL = P[-st]; uR = P[0]; }
+                else recon<REC>(P[-3 * st], P[-2 * st], P[-st], P[0], P[st], P[2 * st], uL, uR);
+            }
             pL[rloc[v]] = uL; pR[rloc[v]] = uR;
         }
     }
@@ -1152,8 +1186,8 @@ struct GatherArgs {
     double* ecen;                            // GS cell-centred E (3 vars) or null
     double* B[3];                            // face arrays (null: skip faces)
     int cell_mode;
-    bool slab_only;                          // cells: only those with one ghost coordinate
     bool own_copies;                         // mode 3: also the copying outflow ghosts (GSRC_COPY)
+    bool copies_only;                        // only the copying outflow ghosts (all of them)
 };
 // GSRC_OWN / GSRC_AMR: forest.hh
 
@@ -1674,7 +1708,9 @@ struct grmhd_ctx {
     int32_t* ring_nbr = nullptr;
     uint8_t* ring_own = nullptr;
     // set by the RK3 step around the flux launches of its second and third
-    // stages, whose ghost primitives and face B the step's own gathers wrote
+    // stages: their ring faces come from the neighbours' pushes and their
+    // full faces read the cells beyond a block face from the neighbour's
+    // interior, so the gathers between stages leave the face slabs alone
     // (the first stage and grmhd_advance_substep read the caller's ghosts and
     // compute every ring face)
     bool ring_copy_now = false;
@@ -1691,12 +1727,11 @@ struct grmhd_ctx {
     // radius excision): {block, cell, block, first-layer cell}; they take
     // copies of that cell's c2p outputs instead of running their own (the
     // stage-end gather, through GSRC_COPY in the ghost table) and, when every
-    // ghost is filled, its conserved (this list, after the cell gather).
+    // ghost is filled, its conserved (this list, after the cell gather);
+    // between RK stages with shared ring faces this list is the whole gather.
     int4* copy_gather = nullptr;
     int64_t n_copy = 0;
-    int64_t n_copy_edge = 0;                  // the first n_copy_edge: cells with two or three ghost coordinates
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
-    int64_t ncg_slab = 0;                     // ghost cells with one ghost coordinate (listed first)
     int64_t last_c2p_n = 0;
     bool timing = false;
     std::vector<TimerRec> timers;
@@ -2029,17 +2064,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                         cnt++;
                     }
             if (fd < 0) {
-                // the cells with one ghost coordinate (the six face slabs) first:
-                // between RK stages with shared ring faces only they are read
                 c->ncg = cnt;
-                auto nghost = [&](int32_t off) {
-                    const int loc[3] = {off % E[0], (off / E[0]) % E[1], off / (E[0] * E[1])};
-                    int ng_ = 0;
-                    for (int d = 0; d < 3; ++d) ng_ += loc[d] < G.ng || loc[d] >= G.n[d] + G.ng;
-                    return ng_;
-                };
-                auto mid = std::stable_partition(lst.begin(), lst.end(), [&](int32_t off) { return nghost(off) == 1; });
-                c->ncg_slab = mid - lst.begin();
             } else {
                 c->nfg[fd] = cnt;
             }
@@ -2092,7 +2117,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         // a flat metric and no radius-based excision its c2p inputs are those
         // of the cell with every outflow-exceeding coordinate moved to the first
         // ghost layer.  Only that cell solves; the others copy its outputs.
-        std::vector<int4> copyg, copy_slab;   // copies beyond the face slabs first, then those in them
+        std::vector<int4> copyg;
         std::vector<uint8_t> is_copy((size_t)(per * G.nq), 0);
         if (!fin && c->P.flat_metric && !c->P.excise_by_radius) {
             std::unordered_map<int32_t, int64_t> pos;   // local cell offset -> entry
@@ -2120,7 +2145,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
                     if (it == pos.end() || !(src[(size_t)(q * per + it->second)].x & GSRC_OWN)) continue;
                     is_copy[(size_t)(q * per + x)] = 1;
                     src[(size_t)(q * per + x)] = make_int2((int)q | GSRC_COPY, roff);
-                    (x < c->ncg_slab ? copy_slab : copyg).push_back(make_int4((int)q, off, (int)q, roff));
+                    copyg.push_back(make_int4((int)q, off, (int)q, roff));
                 }
             }
         }
@@ -2147,8 +2172,6 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
             }
         c->n_own = (int64_t)own.size();
         c->n_own_gather = (int64_t)owng.size();
-        c->n_copy_edge = (int64_t)copyg.size();
-        copyg.insert(copyg.end(), copy_slab.begin(), copy_slab.end());
         c->n_copy = (int64_t)copyg.size();
         c->n_amr_cells = (int64_t)amr_c.size();
         c->n_amr_faces = (int64_t)amr_f.size();
@@ -2439,6 +2462,11 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     if (a.ecen)
         for (int v = 0; v < 3; ++v) add(a.ecen, 3, v);
     if (ch.nch == 0) return;
+    if (a.copies_only) {
+        if (c->n_copy > 0)
+            k_gather_list<<<dim3(nblk(c->n_copy), 1, ch.nch), 256, 0, s>>>(c->copy_gather, (int)c->n_copy, ch);
+        return;
+    }
     // coarse-fine ghost cells of the state (they run their own c2p)
     const bool amr = a.st && a.cell_mode <= 2 && c->n_amr_cells > 0;
     if (a.cell_mode == 2) {
@@ -2447,19 +2475,14 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
         if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
         return;
     }
-    const int64_t ncells = a.slab_only ? c->ncg_slab : c->ncg;
-    const int wpb = (int)nblk_bs(ncells, c->tune.gather_cells);
+    const int wpb = (int)nblk_bs(c->ncg, c->tune.gather_cells);
     k_gather_cells<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_cells, 0, s>>>(c->ghost_list, c->ghost_src, per,
-                                                                                 (int)ncells, wpb, a.cell_mode,
+                                                                                 (int)c->ncg, wpb, a.cell_mode,
                                                                                  a.own_copies ? 1 : 0, ch);
-    // every ghost: the copying outflow ghosts take the first-layer cell the
-    // launch above filled.  Face slabs only (between RK stages): the copying
-    // ghosts outside the slabs too, which the next stage's ring faces beyond
-    // outflow faces read
+    // every ghost (mode 1): the copying outflow ghosts take the first-layer
+    // cell the launch above filled
     if (a.cell_mode == 1 && c->n_copy > 0)
         k_gather_list<<<dim3(nblk(c->n_copy), 1, ch.nch), 256, 0, s>>>(c->copy_gather, (int)c->n_copy, ch);
-    if (a.cell_mode == 3 && a.own_copies && a.slab_only && c->n_copy_edge > 0)
-        k_gather_list<<<dim3(nblk(c->n_copy_edge), 1, ch.nch), 256, 0, s>>>(c->copy_gather, (int)c->n_copy_edge, ch);
     if (amr) k_amr_cells<<<nblk(c->n_amr_cells), 256, 0, s>>>(c->amr_cells, (int)c->n_amr_cells, a.st, a.nv, a.v0, a.vc, c->ff);
 }
 
@@ -3081,10 +3104,11 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
             g.cell_mode = 3;
             g.own_copies = true;
             // the next stage copies the ring faces' values from the
-            // neighbouring blocks: its fluxes read ghost primitives only in
-            // the face slabs (and in ghosts beyond outflow faces, which run
-            // their own c2p)
-            g.slab_only = !last && c->ring_own != nullptr;
+            // neighbouring blocks and its full faces read the cells beyond a
+            // block face from the neighbour's interior: of the ghost
+            // primitives it reads only those beyond outflow faces, which
+            // run their own c2p or copy a first-layer ghost
+            g.copies_only = !last && c->ring_own != nullptr;
             launch_gather(c, g, s);
         }
     }

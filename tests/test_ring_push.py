@@ -121,3 +121,53 @@ def test_per_dimension_periodicity_gives_symmetric_neighbours():
                                for q in range(nq) for s in range(6))
     nq, nbr = neighbours([((1, 2, 3), 0, (1, 0, 0, 0, 1, 1))])
     assert not all(nbr[(q, s)] < 0 or nbr[(nbr[(q, s)], s ^ 1)] == q for q in range(nq) for s in range(6))
+
+
+def stencil_reads(n, ng, nq, nbr, D, q, a):
+    """(block, normal index) of the six stencil cells of a full face at normal
+    index a of block q, as k_flux reads them in the second and third RK
+    stages: beyond a block face with a neighbour, that neighbour's interior"""
+    out = []
+    for t in range(6):
+        ap = a + t - 3
+        if ap < ng and nbr[(q, 2 * D)] >= 0:
+            out.append((nbr[(q, 2 * D)], ap + n))
+        elif ap >= n + ng and nbr[(q, 2 * D + 1)] >= 0:
+            out.append((nbr[(q, 2 * D + 1)], ap - n))
+        else:
+            out.append((q, ap))
+    return out
+
+
+@pytest.mark.parametrize("n,ng,insts", GRIDS)
+def test_full_face_stencils_read_what_the_gather_copies(n, ng, insts):
+    # the source of a ghost cell by global position in its instance (the
+    # ghost table's mapping, periodic wrap): every cell a full face reads
+    # beyond a block face must be that source; a cell beyond an outflow face
+    # is read from the block's own ghost layer, which runs its own c2p (or
+    # copies a first-layer ghost that does)
+    nq, nbr = neighbours(insts)
+    blk = {}
+    for a_, (nb, q0, bc) in enumerate(insts):
+        for bk in range(nb[2]):
+            for bj in range(nb[1]):
+                for bi in range(nb[0]):
+                    blk[q0 + bi + nb[0] * (bj + nb[1] * bk)] = (nb, q0, bc, (bi, bj, bk))
+    for D in range(3):
+        for q in range(nq):
+            nb, q0, bc, b = blk[q]
+            Nt = nb[D] * n
+            for a in range(ng, n + ng + 1):
+                for t, (qb, ib) in enumerate(stencil_reads(n, ng, nq, nbr, D, q, a)):
+                    ap = a + t - 3
+                    g = b[D] * n + ap - ng
+                    outside = g < 0 or g >= Nt
+                    if outside and bc[2 * D + (g >= Nt)] != 1:
+                        assert (qb, ib) == (q, ap)
+                        continue
+                    g %= Nt
+                    c = list(b)
+                    c[D] = g // n
+                    owner = q0 + c[0] + nb[0] * (c[1] + nb[1] * c[2])
+                    assert (qb, ib) == (owner, g % n + ng), (D, q, a, t)
+                    assert ng <= ib < n + ng
