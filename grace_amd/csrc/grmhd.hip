@@ -1188,6 +1188,7 @@ struct GatherArgs {
     int cell_mode;
     bool own_copies;                         // mode 3: also the copying outflow ghosts (GSRC_COPY)
     bool copies_only;                        // only the copying outflow ghosts (all of them)
+    bool faces_slab;                         // faces: the face slabs and the faces of ghosts beyond outflow faces
 };
 // GSRC_OWN / GSRC_AMR: forest.hh
 
@@ -1268,20 +1269,21 @@ __global__ __launch_bounds__(256) void k_gather_list(const int4* __restrict__ en
     C.base[((int64_t)t.x * C.nv + C.v) * C.n + t.y] = C.base[((int64_t)t.z * C.nv + C.v) * C.n + t.w];
 }
 
-// Face ghosts of the three face-B arrays (wpb workgroups per block).
+// Face ghosts of the three face-B arrays (wpb workgroups per block): the
+// first m0 / m1 / m2 of the nfg0 / nfg1 / nfg2 entries of each direction.
 __global__ __launch_bounds__(256) void k_gather_faces(const int32_t* __restrict__ list, const int2* __restrict__ src,
-                                                      int per, int ncg, int nfg0, int nfg1, int nfg2, int wpb,
-                                                      double* B0, double* B1, double* B2, int64_t nf0, int64_t nf1,
-                                                      int64_t nf2) {
+                                                      int per, int ncg, int nfg0, int nfg1, int m0, int m1, int m2,
+                                                      int wpb, double* B0, double* B1, double* B2, int64_t nf0,
+                                                      int64_t nf1, int64_t nf2) {
     const WgPos wp = wg_pos(wpb);
-    const int e = ncg + wp.tile * (int)blockDim.x + (int)threadIdx.x;
-    if (e >= ncg + nfg0 + nfg1 + nfg2) return;
+    const int r = wp.tile * (int)blockDim.x + (int)threadIdx.x;
+    if (r >= m0 + m1 + m2) return;
     const int64_t q = wp.q;
+    double* B; int64_t nf; int e;
+    if (r < m0) { B = B0; nf = nf0; e = ncg + r; }
+    else if (r < m0 + m1) { B = B1; nf = nf1; e = ncg + nfg0 + (r - m0); }
+    else { B = B2; nf = nf2; e = ncg + nfg0 + nfg1 + (r - m0 - m1); }
     const int2 sr = src[q * per + e];
-    double* B; int64_t nf;
-    if (e < ncg + nfg0) { B = B0; nf = nf0; }
-    else if (e < ncg + nfg0 + nfg1) { B = B1; nf = nf1; }
-    else { B = B2; nf = nf2; }
     if (sr.x & GSRC_AMR) return;
     B[q * nf + list[e]] = B[(int64_t)sr.x * nf + sr.y];
 }
@@ -1730,8 +1732,15 @@ struct grmhd_ctx {
     // ghost is filled, its conserved (this list, after the cell gather);
     // between RK stages with shared ring faces this list is the whole gather.
     int4* copy_gather = nullptr;
+    // with shared ring faces: the ghost faces outside the face slabs that
+    // bound a ghost cell beyond an outflow face {block, face, source block,
+    // source face}, x then y then z (n_ofl each); between RK stages the face
+    // gather is the face slabs and these
+    int4* ofl_faces = nullptr;
+    int64_t n_ofl[3] = {0, 0, 0};
     int64_t n_copy = 0;
     int64_t ncg = 0, nfg[3] = {0, 0, 0};
+    int64_t nfg_slab[3] = {0, 0, 0};          // face ghosts of the face slabs (listed first per direction)
     int64_t last_c2p_n = 0;
     bool timing = false;
     std::vector<TimerRec> timers;
@@ -1839,7 +1848,7 @@ static void build_eos(grmhd_ctx* c) {
 
 static void free_ctx(grmhd_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather); F(c->copy_gather);
+    F(c->inst_d); F(c->blk_d); F(c->emf); F(c->ecen); F(c->U1); F(c->U2); F(c->dtpart); F(c->stats); F(c->stats_part); F(c->ghost_list); F(c->ghost_src); F(c->own_cells); F(c->own_gather); F(c->copy_gather); F(c->ofl_faces);
     F(c->ftag[0]); F(c->ftag[1]); F(c->ftag[2]); F(c->etag); F(c->mder);
     F(c->fmet[0]); F(c->fmet[1]); F(c->fmet[2]);
     F(c->cmap_d); F(c->fblk_d); F(c->amr_cells); F(c->amr_faces); F(c->amr_faces_mid); F(c->rflux); F(c->remf);
@@ -2066,7 +2075,21 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
             if (fd < 0) {
                 c->ncg = cnt;
             } else {
+                // the face slabs first (face index in [ng, n+ng], exactly one
+                // of the other two coordinates in the ghost layers): between
+                // RK stages with shared ring faces the edge EMFs read no
+                // other ghost face
                 c->nfg[fd] = cnt;
+                auto slab = [&](int32_t off) {
+                    const int loc[3] = {off % E[0], (off / E[0]) % E[1], off / (E[0] * E[1])};
+                    if (loc[fd] < G.ng || loc[fd] > G.n[fd] + G.ng) return false;
+                    int ng_ = 0;
+                    for (int d = 0; d < 3; ++d)
+                        if (d != fd) ng_ += loc[d] < G.ng || loc[d] >= G.n[d] + G.ng;
+                    return ng_ == 1;
+                };
+                auto mid = std::stable_partition(lst.end() - cnt, lst.end(), slab);
+                c->nfg_slab[fd] = mid - (lst.end() - cnt);
             }
         }
         if ((e = hipMalloc(&c->ghost_list, sizeof(int32_t) * lst.size())) != hipSuccess) return fail("ghost list", e);
@@ -2173,6 +2196,33 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         c->n_own = (int64_t)own.size();
         c->n_own_gather = (int64_t)owng.size();
         c->n_copy = (int64_t)copyg.size();
+        std::vector<int4> ofl;
+        if (c->ring_own) {
+            int64_t base = c->ncg;
+            for (int fd = 0; fd < 3; ++fd) {
+                const int E0 = G.N[0] + (fd == 0), E1 = G.N[1] + (fd == 1);
+                const size_t before = ofl.size();
+                for (int64_t q = 0; q < G.nq; ++q) {
+                    const DevBlk& b = blk[q];
+                    const DevInst& I = c->inst_h[b.inst];
+                    for (int64_t x = base + c->nfg_slab[fd]; x < base + c->nfg[fd]; ++x) {
+                        // a face of a ghost cell beyond an outflow face (its
+                        // c2p, or the one it copies, reads the cell's faces)
+                        const int loc[3] = {lst[x] % E0, (lst[x] / E0) % E1, lst[x] / (E0 * E1)};
+                        int lo[3] = {loc[0], loc[1], loc[2]};
+                        lo[fd] -= 1;
+                        const bool hi_ok = loc[fd] < G.N[fd], lo_ok = loc[fd] >= 1;
+                        if (!(hi_ok && ghost_needs_own_c2p(G, I, b, loc[0], loc[1], loc[2])) &&
+                            !(lo_ok && ghost_needs_own_c2p(G, I, b, lo[0], lo[1], lo[2])))
+                            continue;
+                        const int2 sr = src[(size_t)(q * per + x)];
+                        ofl.push_back(make_int4((int)q, lst[x], sr.x & (GSRC_COPY - 1), sr.y));
+                    }
+                }
+                c->n_ofl[fd] = (int64_t)(ofl.size() - before);
+                base += c->nfg[fd];
+            }
+        }
         c->n_amr_cells = (int64_t)amr_c.size();
         c->n_amr_faces = (int64_t)amr_f.size();
         c->n_amr_faces_mid = (int64_t)amr_fm.size();
@@ -2187,6 +2237,7 @@ static grmhd_status ctx_create_impl(const grmhd_params* p, const grmhd_grid* g, 
         if (!upload(&c->own_cells, own, "own cells")) return fail("own cells", e);
         if (!upload(&c->own_gather, owng, "own gather")) return fail("own gather", e);
         if (!upload(&c->copy_gather, copyg, "copy gather")) return fail("outflow ghost copies", e);
+        if (!upload(&c->ofl_faces, ofl, "outflow faces")) return fail("outflow ghost faces", e);
         if (!upload(&c->amr_cells, amr_c, "amr cells")) return fail("coarse-fine cells", e);
         if (!upload(&c->amr_faces, amr_f, "amr faces")) return fail("coarse-fine faces", e);
         if (!upload(&c->amr_faces_mid, amr_fm, "amr faces mid")) return fail("coarse-fine interior faces", e);
@@ -2438,11 +2489,23 @@ static void launch_gather(grmhd_ctx* c, const GatherArgs& a, hipStream_t s) {
     const int per = (int)(c->ncg + c->nfg[0] + c->nfg[1] + c->nfg[2]);
     if (a.B[0]) {
         Scope sc(c, "ghost_faces", s);
-        const int nfg = (int)(c->nfg[0] + c->nfg[1] + c->nfg[2]);
-        const int wpb = (int)nblk_bs(nfg, c->tune.gather_faces);
+        const int64_t* m = a.faces_slab ? c->nfg_slab : c->nfg;
+        const int wpb = (int)nblk_bs(m[0] + m[1] + m[2], c->tune.gather_faces);
         k_gather_faces<<<dim3((unsigned)(wpb * G.nq)), c->tune.gather_faces, 0, s>>>(
-            c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0], (int)c->nfg[1], (int)c->nfg[2], wpb,
-            a.B[0], a.B[1], a.B[2], G.nface[0], G.nface[1], G.nface[2]);
+            c->ghost_list, c->ghost_src, per, (int)c->ncg, (int)c->nfg[0], (int)c->nfg[1], (int)m[0], (int)m[1],
+            (int)m[2], wpb, a.B[0], a.B[1], a.B[2], G.nface[0], G.nface[1], G.nface[2]);
+        if (a.faces_slab) {
+            int64_t o = 0;
+            for (int d = 0; d < 3; ++d) {
+                if (c->n_ofl[d] > 0) {
+                    GatherChans fc{};
+                    fc.c[0] = GatherChan{a.B[d], 1, 0, G.nface[d]};
+                    fc.nch = 1;
+                    k_gather_list<<<dim3(nblk(c->n_ofl[d]), 1, 1), 256, 0, s>>>(c->ofl_faces + o, (int)c->n_ofl[d], fc);
+                }
+                o += c->n_ofl[d];
+            }
+        }
         // the second pass reads what the copies and the first pass wrote
         if (c->n_amr_faces > 0)
             k_amr_faces<<<nblk(c->n_amr_faces), 256, 0, s>>>(c->amr_faces, (int)c->n_amr_faces, a.B[0], a.B[1], a.B[2], c->ff);
@@ -3062,6 +3125,11 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
             launch_update(c, a, s);
         }
         const double* const nBc[3] = {S.nB[0], S.nB[1], S.nB[2]};
+        // ghost faces: every one after the last stage; between stages with
+        // shared ring faces the next edge EMFs and the ring faces beyond
+        // outflow faces read only the face slabs, the ghost cells beyond
+        // outflow faces their own faces
+        const bool faces_slab = sidx < 2 && c->ring_own != nullptr;
         if (conc) {
             // side[0]: ghost faces, then (after the conserved fills on s) the
             // c2p of the ghosts that run their own; s: conserved fills,
@@ -3070,6 +3138,7 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
             GatherArgs gf{};
             for (int d = 0; d < 3; ++d) gf.B[d] = S.nB[d];
             gf.cell_mode = 0;
+            gf.faces_slab = faces_slab;
             launch_gather(c, gf, c->side[0]);
             GatherArgs gc{};
             gc.st = S.nst; gc.nv = S.nnv; gc.v0 = 0; gc.vc = NHRSC; gc.cell_mode = 2;
@@ -3083,6 +3152,7 @@ static void rk3_launches(grmhd_ctx* c, const double* dt_dev, double* state, doub
                 GatherArgs g{};
                 g.st = S.nst; g.nv = S.nnv; g.v0 = 0; g.vc = NHRSC; g.cell_mode = c->P.fofc ? 1 : 2;
                 for (int d = 0; d < 3; ++d) g.B[d] = S.nB[d];
+                g.faces_slab = faces_slab;
                 launch_gather(c, g, s);
             }
             launch_c2p_cells(c, S.nst, S.nnv, state, NSTATE, nBc, aux, false, true, s);
