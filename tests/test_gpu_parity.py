@@ -385,20 +385,28 @@ def test_gpu_rk3_step_blast(dev):
     assert np.max(np.abs(bdiv)) < 1e-12
 
 
-@pytest.mark.parametrize("nb", [(2, 2, 2), (1, 1, 2)])
-def test_gpu_rk3_periodic(dev, nb):
+_P, _O = ga.BC_PERIODIC, ga.BC_OUTFLOW
+
+
+@pytest.mark.parametrize("nb,bc", [((2, 2, 2), (_P,) * 6), ((1, 1, 2), (_P,) * 6),
+                                   ((2, 2, 2), (_P, _P, _O, _O, _P, _P)), ((1, 2, 2), (_P, _P, _P, _P, _O, _O))],
+                         ids=["periodic", "self_periodic", "y_outflow", "z_outflow_self_x"])
+def test_gpu_rk3_periodic(dev, nb, bc):
     """Periodic faces on every side (with one block along x and y, a block
-    is its own periodic neighbour): two RK3 steps against the oracle.  From
-    the second stage on, the UCT face values of a block's transverse ghost
-    ring are copied from the neighbouring block's interior faces (the
-    periodic image's across the domain faces) instead of recomputed."""
+    is its own periodic neighbour), and periodic dimensions beside outflow
+    ones: two RK3 steps against the oracle.  From the second stage on, the
+    UCT face values of a block's transverse ghost ring are pushed from the
+    neighbouring block's interior faces (the periodic image's across the
+    domain faces), full faces read the cells beyond a block face from the
+    neighbour's interior, and the gathers between stages fill only the
+    outflow-ghost copies and the ghost faces the next stage reads."""
     wl = W.blast(nb=nb, n=16)
     spec = dict(wl.inst_specs[0])
-    spec["bc"] = (ga.BC_PERIODIC,) * 6
+    spec["bc"] = tuple(bc)
     wl.inst_specs = [spec]
     wl.instances = ga.make_instances(wl.inst_specs)
     wl.extra["dx"] = 1.0 / max(wl.extra["N"])
-    say(f"[rk3 periodic] blast {nb} blocks of 16^3, two steps")
+    say(f"[rk3 periodic] blast {nb} blocks of 16^3, bc {bc}, two steps")
     ref, got, _ = run_rk3_pair(wl, dev, steps=2)
     check_close(wl, ref, got, 1e-12)
     assert np.array_equal(got[2][:, ga.C2P_ERR], ref[2][:, ga.C2P_ERR])
