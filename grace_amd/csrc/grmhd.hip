@@ -3307,6 +3307,31 @@ extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const 
     double* const B1[3] = {c->B1[0], c->B1[1], c->B1[2]};
     const double* const B1c[3] = {c->B1[0], c->B1[1], c->B1[2]};
     grmhd_tuning keep = c->tune;
+    // the three flux launches as the RK3 stages run them: on three streams
+    // when the stages use side streams, and in the form of the second and
+    // third stages (ring faces pushed, reads across block faces) when the
+    // context shares ring faces; the directions run concurrently, so each
+    // choice is timed against the other two
+    const bool conc = c->P.stage_streams && !c->P.fofc && c->side[0] && c->side[1];
+    auto fluxes_as_in_step = [&] {
+        if (conc) {
+            for (int k = 0; k < 2; ++k) {
+                (void)hipEventRecord(c->fev[k], s);
+                (void)hipStreamWaitEvent(c->side[k], c->fev[k], 0);
+            }
+            c->flux_s[1] = c->side[0]; c->flux_s[2] = c->side[1];
+        }
+        c->ring_copy_now = c->ring_own != nullptr;
+        launch_fluxes(c, aux, state, NSTATE, Bc, s);
+        c->ring_copy_now = false;
+        if (conc) {
+            c->flux_s[1] = c->flux_s[2] = nullptr;
+            for (int k = 0; k < 2; ++k) {
+                (void)hipEventRecord(c->fev[2 + k], c->side[k]);
+                (void)hipStreamWaitEvent(s, c->fev[2 + k], 0);
+            }
+        }
+    };
     for (int d = 0; d < 3; ++d) {
         // one direction at a time (the other two run at their current
         // choice): workgroup size and, for WENO-Z, the prefetching variant
@@ -3316,7 +3341,7 @@ extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const 
             for (int v : cand) {
                 c->tune.flux[d] = v;
                 c->tune.flux_prefetch[d] = pf;
-                const float ms = time_it([&] { launch_fluxes(c, aux, state, NSTATE, Bc, s); });
+                const float ms = time_it(fluxes_as_in_step);
                 if (ms < best) { best = ms; bs = v; bp = pf; }
             }
         c->tune.flux[d] = bs;
