@@ -668,7 +668,7 @@ def main():
         elif not args.no_tune:
             # workgroup sizes per kernel family picked by timing (inputs are
             # only read), before the warm-up and the timed steps
-            tuning = ctx.autotune(st, B, aux, reps=3, stream=sp)
+            tuning = ctx.autotune(st, B, aux, reps=7, stream=sp)
             stream.synchronize()
         log(f"[bench] warmup {args.warmup}")
         for _ in range(args.warmup):
