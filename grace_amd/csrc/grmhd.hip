@@ -3347,15 +3347,33 @@ extern "C" grmhd_status grmhd_autotune(grmhd_ctx* c, const double* state, const 
         c->tune.flux[d] = bs;
         c->tune.flux_prefetch[d] = bp;
     }
-    pick(&c->tune.emf, [&] { launch_emf(c, Bc, s); });
-    pick(&c->tune.update, [&] {
+    // the edge EMFs and the update as the stages run them: the update's cell
+    // part beside the EMFs on a side stream, its face part after them
+    const bool split = conc && !(c->forest && c->P.emf == GRMHD_EMF_GS);
+    auto emf_update_as_in_step = [&] {
         UpdArgs u{};
         u.nst = c->U1; u.nnv = NHRSC; u.ast = state; u.anv = NSTATE; u.bst = nullptr; u.bnv = 0;
         u.ca = 1.0; u.cb = 0.0; u.mstate = state; u.mnv = NSTATE; u.aux = aux;
         for (int d = 0; d < 3; ++d) { u.nB[d] = B1[d]; u.aB[d] = Bc[d]; u.bB[d] = nullptr; }
         u.dt = dt0; u.dtfact = 1.0;
-        launch_update(c, u, s);
-    });
+        if (split) {
+            (void)hipEventRecord(c->fev[0], s);
+            (void)hipStreamWaitEvent(c->side[0], c->fev[0], 0);
+            UpdArgs uc = u;
+            uc.part = 1;
+            launch_update(c, uc, c->side[0]);
+            launch_emf(c, Bc, s);
+            u.part = 2;
+            launch_update(c, u, s);
+            (void)hipEventRecord(c->fev[1], c->side[0]);
+            (void)hipStreamWaitEvent(s, c->fev[1], 0);
+        } else {
+            launch_emf(c, Bc, s);
+            launch_update(c, u, s);
+        }
+    };
+    pick(&c->tune.emf, emf_update_as_in_step);
+    pick(&c->tune.update, emf_update_as_in_step);
     pick(&c->tune.gather_faces, [&] {
         GatherArgs g{};
         for (int d = 0; d < 3; ++d) g.B[d] = B1[d];
